@@ -1,0 +1,416 @@
+"""FP64 CPU oracle of the Pfaffian/BCS backflow amplitude and local-energy path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``neuralhubbard_amd`` imports, links or runs this
+module: it is imported by ``tests/``, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of ``bench.py``, always as the checker of the HIP path.
+
+It restates the reference algorithm (ChenAo-Phys/NeuralHubbard) in float64 numpy; each
+function cites the reference lines it follows.  The reference's third-party pieces
+(``fermix.slogpf``/``slogdet`` 0.1.0, ``quantax`` 0.4.0 projection, ``Hubbard.Oloc``,
+samplers) are not vendored, so those functions restate the published algorithms (Parlett-Reid
+Pfaffian of Wimmer, ACM TOMS 38 (2012) 30; Gaussian elimination with partial pivoting;
+Metropolis; Christen & Fox (2005) delayed acceptance) and are pinned by the closed forms and
+brute-force checks in ``tests/test_oracle.py``.
+
+Pinning status per function (see the tests named in brackets):
+  lattice tables            closed form + brute force                [test_oracle_tables]
+  build_wtilde / wt_ud      antisymmetry identity + normalisation    [test_oracle_wtilde]
+  conv / features           scipy.ndimage.correlate(mode="wrap")     [test_oracle_conv]
+  pairing_occ               full 2N x 2N einsum path, sliced         [test_oracle_pairing]
+  slogpf                    Pf^2 = det, exact rational expansion,
+                            Pf(B A B^T) = det(B) Pf(A), block form    [test_oracle_pfaffian]
+  slogdet                   numpy (LAPACK) slogdet                   [test_oracle_det]
+  projection                U_g psi_P = psi_P with U_g built on bit
+                            strings; explicit inversion-count signs   [test_oracle_projection]
+  eloc                      sparse JW Hamiltonian built on bit
+                            strings over the full Hilbert space       [test_oracle_eloc]
+  philox4x32                published known-answer vectors           [test_oracle_philox]
+  jackknife / energy        closed forms for i.i.d. blocks           [test_oracle_jackknife]
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# ------------------------------------------------------------------------ lattice tables
+
+
+def lattice_tables(LX: int, LY: int):
+    """(disp (N,N), neg (N,), dist_r (N,)) as ``displacement_tables``
+    (``code/backflow/pf_kernel.py:83-96``), dist in float64."""
+    N = LX * LY
+    x, y = np.divmod(np.arange(N), LY)
+    disp = ((x[:, None] - x[None, :]) % LX) * LY + (y[:, None] - y[None, :]) % LY
+    neg = ((-x) % LX) * LY + (-y) % LY
+    mx, my = np.minimum(x, LX - x), np.minimum(y, LY - y)
+    return disp, neg, np.sqrt(mx**2 + my**2.0)
+
+
+def group_perm(g: int, LX: int, LY: int) -> np.ndarray:
+    """Mode map of group element g = fx | fy << 1 | si << 2 (Flip(0) @ Flip(1) @ SpinInverse,
+    ``code/states.py:274-279``)."""
+    N = LX * LY
+    m = np.arange(2 * N)
+    spin, site = np.divmod(m, N)
+    x, y = np.divmod(site, LY)
+    if g & 1:
+        x = (-x) % LX
+    if g & 2:
+        y = (-y) % LY
+    return ((spin ^ ((g >> 2) & 1)) * N + x * LY + y).astype(np.int64)
+
+
+# ------------------------------------------------------------------------ network
+
+
+def gelu(x):
+    """tanh form of GELU (the default of ``jax.nn.gelu``)."""
+    return 0.5 * x * (1.0 + np.tanh(math.sqrt(2.0 / math.pi) * (x + 0.044715 * x**3)))
+
+
+def conv3x3(h, w, b):
+    """Periodic 3x3 cross-correlation: out[o,x,y] = b[o] + sum w[o,c,1+dx,1+dy] h[c,x+dx,y+dy]."""
+    out = np.zeros((w.shape[0],) + h.shape[1:])
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            sh = np.roll(h, shift=(-dx, -dy), axis=(1, 2))
+            out += np.einsum("oc,cxy->oxy", w[:, :, dx + 1, dy + 1].astype(np.float64), sh)
+    return out + np.asarray(b, np.float64)[:, None, None]
+
+
+def features(s, conv_w, conv_b, LX, LY):
+    """(2, N, d) per-spin features of one configuration: embedding conv, residual layers
+    ``h + conv(gelu(h))``, channels split as y_up | y_dn (``code/backflow/models.py:163-167``)."""
+    x = np.asarray(s, np.float64).reshape(2, LX, LY)
+    h = conv3x3(x, conv_w[0], conv_b[0])
+    for w, b in zip(conv_w[1:], conv_b[1:]):
+        h = h + conv3x3(gelu(h), w, b)
+    C = h.shape[0]
+    return np.swapaxes(h.reshape(2, C // 2, LX * LY), 1, 2)
+
+
+def softplus(t):
+    return math.log1p(math.exp(-abs(t))) + max(t, 0.0)
+
+
+def build_wtilde(W, theta, LX, LY):
+    """(3,N,d) -> (4,N,d) [uu, ud, du, dd] with the distance envelope
+    (``code/backflow/pf_kernel.py:107-124``)."""
+    _, neg, dist = lattice_tables(LX, LY)
+    W = np.asarray(W, np.float64)
+    N = LX * LY
+    W_ud = W[0]
+    W_uu = (W[1] - W[1][neg]) / 2
+    W_dd = (W[2] - W[2][neg]) / 2
+    W_du = -W_ud[neg]
+    pref = np.exp(-softplus(theta) * dist)
+    pref = pref / (N * pref.sum())
+    return np.stack([W_uu, W_ud, W_du, W_dd]) * pref[None, :, None]
+
+
+def build_wt_ud(W, theta, LX, LY):
+    """(N,d) -> W * pref (``code/backflow/bcs_kernel.py:63-70``)."""
+    _, _, dist = lattice_tables(LX, LY)
+    pref = np.exp(-softplus(theta) * dist)
+    return np.asarray(W, np.float64) * (pref / (LX * LY * pref.sum()))[:, None]
+
+
+# ------------------------------------------------------------------------ pairing
+
+
+def pairing_occ(Y, modes, Wt, LX, LY):
+    """F[p,q] = sum_a Y_pa Y_qa Wt[2 s_p + s_q, r(p,q), a] over the listed modes
+    (``code/backflow/pf_kernel.py:371-382``); Y (2N, d) features by mode."""
+    disp, _, _ = lattice_tables(LX, LY)
+    N = LX * LY
+    m = np.asarray(modes)
+    sp, site = np.divmod(m, N)
+    r = disp[site[:, None], site[None, :]]
+    blk = 2 * sp[:, None] + sp[None, :]
+    Yo = np.asarray(Y, np.float64)[m]
+    # (Y_p * Y_q) first, then the table: commutative, so F is exactly antisymmetric
+    # (the order of ``code/backflow/pf_kernel.py:164-165``)
+    return ((Yo[:, None, :] * Yo[None, :, :]) * Wt[blk, r]).sum(-1)
+
+
+def pairing_full(y, W, theta, LX, LY):
+    """The reference's einsum path: full (2N, 2N) f, then (f - f^T) / 2
+    (``code/backflow/models.py:169-196``)."""
+    disp, neg, dist = lattice_tables(LX, LY)
+    W = np.asarray(W, np.float64)
+    y = np.asarray(y, np.float64)
+    W_ud = W[0]
+    W_uu = (W[1] - W[1][neg]) / 2
+    W_dd = (W[2] - W[2][neg]) / 2
+    f_ud = np.einsum("ia,ija,ja->ij", y[0], W_ud[disp], y[1])
+    f_uu = np.einsum("ia,ija,ja->ij", y[0], W_uu[disp], y[0])
+    f_dd = np.einsum("ia,ija,ja->ij", y[1], W_dd[disp], y[1])
+    full_dist = dist[disp]
+    pref = np.exp(-softplus(theta) * full_dist)
+    pref = pref / pref.sum()
+    f = np.block([[f_uu, f_ud], [-f_ud.T, f_dd]]) * np.tile(pref, (2, 2))
+    return (f - f.T) / 2
+
+
+def pairing_ud(Yu, Yd, sites_u, sites_d, Wt, LX, LY):
+    """F[p,q] = sum_a Yu_pa Wt[r(p,q), a] Yd_qa (``code/backflow/bcs_kernel.py:296-300``)."""
+    disp, _, _ = lattice_tables(LX, LY)
+    r = disp[np.asarray(sites_u)[:, None], np.asarray(sites_d)[None, :]]
+    return np.einsum("pa,pqa,qa->pq", np.asarray(Yu, np.float64), Wt[r], np.asarray(Yd, np.float64))
+
+
+# ------------------------------------------------------------------------ linear algebra
+
+
+def slogpf(A):
+    """(sign, log|Pf A|) of a real antisymmetric matrix by Parlett-Reid elimination with
+    partial pivoting (Wimmer 2012, algorithm ``pfaffian_LTL``), the role of
+    ``fermix.slogpf`` at ``code/backflow/models.py:210``.  sign 0, logabs -inf if singular."""
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    if n % 2:
+        return 0.0, -math.inf
+    sign, logabs = 1.0, 0.0
+    for k in range(0, n - 1, 2):
+        kp = k + 1 + int(np.argmax(np.abs(A[k + 1 :, k])))
+        if kp != k + 1:
+            A[[k + 1, kp], :] = A[[kp, k + 1], :]
+            A[:, [k + 1, kp]] = A[:, [kp, k + 1]]
+            sign = -sign
+        piv = A[k, k + 1]
+        if piv == 0.0:
+            return 0.0, -math.inf
+        sign *= math.copysign(1.0, piv)
+        logabs += math.log(abs(piv))
+        if k + 2 < n:
+            tau = A[k, k + 2 :] / piv
+            v = A[k + 2 :, k + 1].copy()
+            A[k + 2 :, k + 2 :] += np.outer(tau, v) - np.outer(v, tau)
+    return sign, logabs
+
+
+def slogdet(A):
+    """(sign, log|det A|) by Gaussian elimination with partial pivoting, the role of
+    ``fermix.slogdet`` at ``code/backflow/models.py:297``."""
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    sign, logabs = 1.0, 0.0
+    for k in range(n):
+        p = k + int(np.argmax(np.abs(A[k:, k])))
+        if A[p, k] == 0.0:
+            return 0.0, -math.inf
+        if p != k:
+            A[[k, p]] = A[[p, k]]
+            sign = -sign
+        piv = A[k, k]
+        sign *= math.copysign(1.0, piv)
+        logabs += math.log(abs(piv))
+        A[k + 1 :, k + 1 :] -= np.outer(A[k + 1 :, k] / piv, A[k, k + 1 :])
+    return sign, logabs
+
+
+def signed_logsumexp(signs, logs):
+    """LogArray sum: (sign, log|sum_i sign_i exp(log_i)|)."""
+    logs = np.asarray(logs, np.float64)
+    signs = np.asarray(signs, np.float64)
+    ok = signs != 0
+    if not ok.any():
+        return 0.0, -math.inf
+    m = logs[ok].max()
+    tot = float(np.sum(signs[ok] * np.exp(logs[ok] - m)))
+    if tot == 0.0:
+        return 0.0, -math.inf
+    return math.copysign(1.0, tot), m + math.log(abs(tot))
+
+
+# ------------------------------------------------------------------------ amplitudes
+
+
+class Model:
+    """FP64 restatement of ``PfaffianBackflow`` / ``BCSBackflow`` (``code/backflow/models.py``)
+    with the workload's synthetic network."""
+
+    def __init__(self, workload, params):
+        self.w = workload
+        self.p = params
+        LX, LY = workload.LX, workload.LY
+        if workload.kind == "pfaffian":
+            self.Wt = build_wtilde(params.W, params.theta, LX, LY)
+        else:
+            self.Wt = build_wt_ud(params.W, params.theta, LX, LY)
+
+    def features(self, s):
+        w = self.w
+        return features(s, self.p.conv_w, self.p.conv_b, w.LX, w.LY)
+
+    def image_matrix(self, s, g):
+        """Matrix whose Pfaffian (determinant for BCS, times the block sign) is
+        eta_g(s) * psi(g.s): the occupied modes of s mapped by g, kept in the ascending
+        order of s (the permutation sign of the reordering is then inside the Pfaffian)."""
+        w = self.w
+        LX, LY, N = w.LX, w.LY, w.N
+        perm = group_perm(g, LX, LY)
+        gs = np.empty_like(np.asarray(s))
+        gs[perm] = s
+        y = self.features(gs)  # (2, N, d)
+        idx = np.flatnonzero(np.asarray(s) > 0)
+        modes = perm[idx]
+        if w.kind == "pfaffian":
+            F = pairing_occ(y.reshape(2 * N, -1), modes, self.Wt, LX, LY)
+            if self.p.F0 is not None:
+                F = F + np.asarray(self.p.F0, np.float64)[np.ix_(modes, modes)]
+            return F
+        up = modes[modes < N]
+        dn = modes[modes >= N] - N
+        disp, _, _ = lattice_tables(LX, LY)
+        F = pairing_ud(y[0][up], y[1][dn], up, dn, self.Wt, LX, LY)
+        F = F + np.asarray(self.p.phi0, np.float64)[disp[np.ix_(up, dn)]]
+        return F
+
+    def image_amplitude(self, s, g):
+        w = self.w
+        F = self.image_matrix(s, g)
+        if w.kind == "pfaffian":
+            sg, la = slogpf(F)
+            n = F.shape[0]
+        else:
+            sg, la = slogdet(F)
+            n = F.shape[0]
+            # Pf [[0, Phi], [-Phi^T, 0]] = (-1)^{n(n-1)/2} det Phi (code/backflow/models.py:300);
+            # an image whose first block is spin down carries det(-Phi^T) = (-1)^n det Phi
+            sg *= (-1) ** (n * (n - 1) // 2)
+            if (g >> 2) & 1:
+                sg *= (-1) ** n
+        # fermion_inverse_sign: reversal of the Ne creation operators, (-1)^{Ne(Ne-1)/2}
+        ne = w.n_up + w.n_dn
+        sg *= (-1) ** (ne * (ne - 1) // 2)
+        return sg, la
+
+    def psi(self, s, project=True):
+        """Projected amplitude sum_g eta_g(s) psi(g.s) (trivial characters), as (sign, log)."""
+        if not project:
+            return self.image_amplitude(s, 0)
+        vals = [self.image_amplitude(s, g) for g in range(8)]
+        return signed_logsumexp([v[0] for v in vals], [v[1] for v in vals])
+
+    def psi_batch(self, S, project=True):
+        out = np.array([self.psi(s, project) for s in np.asarray(S)])
+        return out[:, 0], out[:, 1]
+
+
+# ------------------------------------------------------------------------ Hubbard local energy
+
+
+def neighbours(LX, LY, n_neighbor):
+    """Distinct bond partners per site and their shell (0: NN, 1: NNN)."""
+    shells = [[(1, 0), (-1, 0), (0, 1), (0, -1)]]
+    if n_neighbor >= 2:
+        shells.append([(1, 1), (1, -1), (-1, 1), (-1, -1)])
+    out = []
+    for i in range(LX * LY):
+        x, y = divmod(i, LY)
+        seen, row = set(), []
+        for c, shell in enumerate(shells):
+            for dx, dy in shell:
+                j = ((x + dx) % LX) * LY + (y + dy) % LY
+                if j != i and j not in seen:
+                    seen.add(j)
+                    row.append((j, c))
+        out.append(row)
+    return out
+
+
+def hops(s, LX, LY, t, tprime):
+    """Connected configurations of the Hubbard hopping term
+    H = -sum_{<ij>,sigma} t_ij (c+_i c_j + h.c.) (``code/states.py:244-247``):
+    list of (mode_from, mode_to, <s'|H|s>) in enumeration order (electron by ascending mode,
+    partner by table order).  The Jordan-Wigner sign is (-1)^(occupied modes strictly
+    between the two modes) in spin-major mode order."""
+    s = np.asarray(s)
+    N = LX * LY
+    nb = neighbours(LX, LY, 2 if tprime != 0 else 1)
+    tv = (t, tprime)
+    occ = s > 0
+    out = []
+    for m in np.flatnonzero(occ):
+        sp, i = divmod(int(m), N)
+        for j, c in nb[i]:
+            mt = sp * N + j
+            if occ[mt]:
+                continue
+            lo, hi = min(m, mt), max(m, mt)
+            between = int(occ[lo + 1 : hi].sum())
+            out.append((int(m), mt, -tv[c] * (-1.0) ** between))
+    return out
+
+
+def diag_energy(s, U):
+    """U * sum_i n_up,i n_dn,i."""
+    s = np.asarray(s)
+    N = s.shape[-1] // 2
+    return U * float(np.sum((s[:N] > 0) & (s[N:] > 0)))
+
+
+def eloc(model, s, U, t, tprime, project=True, psi_s=None):
+    """E_loc(s) = U D(s) + sum_{s'} <s|H|s'> psi(s') / psi(s) (``Hubbard.Oloc``,
+    ``code/quick_start.py:33``), every psi(s') by a full evaluation."""
+    w = model.w
+    if psi_s is None:
+        psi_s = model.psi(s, project)
+    e = diag_energy(s, U)
+    for a, b, h in hops(s, w.LX, w.LY, t, tprime):
+        sp = np.array(s, copy=True)
+        sp[a], sp[b] = -1, 1
+        sg, la = model.psi(sp, project)
+        e += h * sg * psi_s[0] * math.exp(la - psi_s[1])
+    return e
+
+
+# ------------------------------------------------------------------------ statistics
+
+
+BLOCK_SIZES = (1, 2, 4, 8)
+MIN_BLOCKS = 8
+
+
+def jackknife(f, *series):
+    """Block jackknife (``code/measure/assemble.py:58-82``)."""
+    x = [np.asarray(s, dtype=np.float64) for s in series]
+    value = f(*[s.mean(axis=0) for s in x])
+    errors = {}
+    for b in BLOCK_SIZES:
+        nb = x[0].shape[0] // b
+        if nb < MIN_BLOCKS:
+            continue
+        xb = [s[: nb * b].reshape(nb, b, *s.shape[1:]).mean(axis=1) for s in x]
+        tot = [s.sum(axis=0) for s in xb]
+        reps = np.array([f(*[(tt - s[i]) / (nb - 1) for tt, s in zip(tot, xb)]) for i in range(nb)])
+        errors[b] = np.sqrt(nb - 1) * reps.std(axis=0)
+    error = np.max(np.stack(list(errors.values())), axis=0) if errors else np.nan
+    return value, error, errors
+
+
+# ------------------------------------------------------------------------ counter RNG
+
+
+PHILOX_M0, PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
+PHILOX_W0, PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
+
+
+def philox4x32(ctr, key, rounds=10):
+    """Philox4x32-10 (Salmon et al., SC'11) on python ints: 4 x u32 counter, 2 x u32 key."""
+    c = [int(v) & 0xFFFFFFFF for v in ctr]
+    k = [int(v) & 0xFFFFFFFF for v in key]
+    for _ in range(rounds):
+        p0 = PHILOX_M0 * c[0]
+        p1 = PHILOX_M1 * c[2]
+        c = [
+            ((p1 >> 32) ^ c[1] ^ k[0]) & 0xFFFFFFFF,
+            p1 & 0xFFFFFFFF,
+            ((p0 >> 32) ^ c[3] ^ k[1]) & 0xFFFFFFFF,
+            p0 & 0xFFFFFFFF,
+        ]
+        k = [(k[0] + PHILOX_W0) & 0xFFFFFFFF, (k[1] + PHILOX_W1) & 0xFFFFFFFF]
+    return c

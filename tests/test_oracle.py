@@ -1,0 +1,384 @@
+"""Pins of the FP64 oracle: closed forms, brute force and independent constructions.
+
+The reference ships no tests or golden vectors for this path (its ``.gitignore:6-7`` drops
+``code/tests``), so every oracle function is checked against mathematics that fixes it, chosen
+so that a dropped term, a wrong sign or index, or a transposed operand fails.
+"""
+
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+from oracle import oracle as O
+from neuralhubbard_amd import lattice as L
+from neuralhubbard_amd.configs import make_params, small_workload
+
+
+def rand_skew(n, rng, scale=1.0):
+    A = rng.standard_normal((n, n)) * scale
+    return np.triu(A, 1) - np.triu(A, 1).T
+
+
+# ----------------------------------------------------------------- tables
+
+
+def test_oracle_tables():
+    for LX, LY in [(4, 4), (3, 5), (8, 8), (16, 16)]:
+        disp, neg, dist = O.lattice_tables(LX, LY)
+        N = LX * LY
+        for i in range(0, N, max(1, N // 13)):
+            for j in range(0, N, max(1, N // 11)):
+                xi, yi = divmod(i, LY)
+                xj, yj = divmod(j, LY)
+                r = ((xi - xj) % LX) * LY + (yi - yj) % LY
+                assert disp[i, j] == r
+        for r in range(N):
+            x, y = divmod(r, LY)
+            assert neg[r] == ((-x) % LX) * LY + (-y) % LY
+            assert disp[r, 0] == r and disp[0, r] == neg[r]
+            best = min(math.hypot(x + a * LX, y + b * LY) for a in (-1, 0, 1) for b in (-1, 0, 1))
+            assert abs(dist[r] - best) < 1e-12
+        # product table agrees with the package table used by the host
+        d2, n2, di2 = L.displacement_tables(LX, LY)
+        assert np.array_equal(d2, disp) and np.array_equal(n2, neg) and np.allclose(di2, dist)
+
+
+def test_group_perms_involutions_and_group_law():
+    LX, LY = 4, 6
+    P = [O.group_perm(g, LX, LY) for g in range(8)]
+    ident = np.arange(2 * LX * LY)
+    for g in range(8):
+        assert np.array_equal(P[g][P[g]], ident)
+        for h in range(8):
+            assert np.array_equal(P[g][P[h]], P[g ^ h])
+    assert np.array_equal(np.asarray(L.group_mode_perms(LX, LY)), np.stack(P))
+
+
+# ----------------------------------------------------------------- network
+
+
+def test_oracle_conv():
+    rng = np.random.default_rng(0)
+    h = rng.standard_normal((3, 5, 7))
+    w = rng.standard_normal((4, 3, 3, 3))
+    b = rng.standard_normal(4)
+    out = O.conv3x3(h, w, b)
+    for o in range(4):
+        ref = b[o] + sum(
+            scipy.ndimage.correlate(h[c], w[o, c], mode="wrap") for c in range(3)
+        )
+        np.testing.assert_allclose(out[o], ref, rtol=1e-12, atol=1e-12)
+    # a delta kernel at (dx, dy) is a periodic shift
+    w = np.zeros((1, 1, 3, 3))
+    w[0, 0, 2, 0] = 1.0  # dx = +1, dy = -1
+    out = O.conv3x3(h[:1], w, [0.0])
+    np.testing.assert_array_equal(out[0], np.roll(h[0], (-1, 1), axis=(0, 1)))
+
+
+def test_features_translation_equivariant():
+    wk = small_workload(LX=4, LY=4, channels=4, n_layers=2)
+    p = make_params(wk)
+    s = L.random_configurations(wk.geom, 1, 3)[0]
+    y = O.features(s, p.conv_w, p.conv_b, 4, 4)
+    st = np.roll(s.reshape(2, 4, 4), (1, 2), axis=(1, 2)).reshape(-1)
+    yt = O.features(st, p.conv_w, p.conv_b, 4, 4)
+    np.testing.assert_allclose(
+        np.roll(y.reshape(2, 4, 4, -1), (1, 2), axis=(1, 2)).reshape(y.shape), yt, atol=1e-12
+    )
+
+
+def test_gelu_values():
+    assert O.gelu(0.0) == 0.0
+    np.testing.assert_allclose(O.gelu(np.array([1.0, -1.0, 3.0])),
+                               [0.8411919906082768, -0.15880800939172324, 2.99636259], rtol=1e-7)
+
+
+# ----------------------------------------------------------------- W tilde and pairing
+
+
+def test_oracle_wtilde():
+    rng = np.random.default_rng(1)
+    LX, LY, d = 4, 6, 3
+    N = LX * LY
+    W = rng.standard_normal((3, N, d))
+    Wt = O.build_wtilde(W, 0.55, LX, LY)
+    _, neg, dist = O.lattice_tables(LX, LY)
+    for sp in range(2):
+        for sq in range(2):
+            # Wt[2 sq + sp, -r] = -Wt[2 sp + sq, r] exactly (pf_kernel.py:111-113)
+            np.testing.assert_array_equal(Wt[2 * sq + sp][neg], -Wt[2 * sp + sq])
+    # envelope normalisation sum_ij exp(-lam D_ij) = N sum_r exp(-lam D_r)
+    lam = math.log1p(math.exp(0.55))
+    disp, _, _ = O.lattice_tables(LX, LY)
+    tot = np.exp(-lam * dist[disp]).sum()
+    np.testing.assert_allclose(Wt[1] / W[0], np.broadcast_to(np.exp(-lam * dist)[:, None] / tot, (N, d)),
+                               rtol=1e-12)
+
+
+@pytest.mark.parametrize("LX,LY", [(4, 4), (3, 5)])
+def test_oracle_pairing(LX, LY):
+    """Gather path == the reference's full-matrix einsum path, sliced (pf_kernel.py:388)."""
+    rng = np.random.default_rng(2)
+    N, d = LX * LY, 3
+    W = rng.standard_normal((3, N, d))
+    y = rng.standard_normal((2, N, d))
+    Ffull = O.pairing_full(y, W, 0.3, LX, LY)
+    Wt = O.build_wtilde(W, 0.3, LX, LY)
+    idx = np.sort(rng.choice(2 * N, 9, replace=False))
+    F = O.pairing_occ(y.reshape(2 * N, d), idx, Wt, LX, LY)
+    np.testing.assert_allclose(F, Ffull[np.ix_(idx, idx)], rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(F, -F.T)
+    # explicit triple loop
+    disp, _, _ = O.lattice_tables(LX, LY)
+    p, q = idx[1], idx[7]
+    val = sum(y.reshape(2 * N, d)[p, a] * y.reshape(2 * N, d)[q, a]
+              * Wt[2 * (p // N) + q // N, disp[p % N, q % N], a] for a in range(d))
+    assert abs(F[1, 7] - val) < 1e-14
+
+
+# ----------------------------------------------------------------- Pfaffian / determinant
+
+
+def pf_exact(A):
+    """Exact Pfaffian by expansion along the first row, in rationals."""
+    n = len(A)
+    if n == 0:
+        return Fraction(1)
+    if n % 2:
+        return Fraction(0)
+    tot = Fraction(0)
+    for j in range(1, n):
+        if A[0][j] == 0:
+            continue
+        rest = [k for k in range(1, n) if k != j]
+        sub = [[A[a][b] for b in rest] for a in rest]
+        tot += (-1) ** (j + 1) * A[0][j] * pf_exact(sub)
+    return tot
+
+
+def test_oracle_pfaffian_exact_rational():
+    rng = np.random.default_rng(3)
+    for n in (2, 4, 6, 8, 10):
+        M = rng.integers(-5, 6, size=(n, n))
+        M = np.triu(M, 1) - np.triu(M, 1).T
+        exact = pf_exact([[Fraction(int(v)) for v in row] for row in M])
+        sg, la = O.slogpf(M.astype(float))
+        if exact == 0:
+            assert sg == 0 or la < -20
+            continue
+        assert sg == (1 if exact > 0 else -1)
+        assert abs(la - math.log(abs(float(exact)))) < 1e-10
+
+
+def test_oracle_pfaffian_identities():
+    rng = np.random.default_rng(4)
+    for n in (2, 8, 56, 120):
+        A = rand_skew(n, rng)
+        sg, la = O.slogpf(A)
+        ds, dl = np.linalg.slogdet(A)
+        assert ds > 0 and abs(2 * la - dl) < 1e-8 * max(1, abs(dl))
+        B = rng.standard_normal((n, n))
+        s2, l2 = O.slogpf(B @ A @ B.T)
+        bs, bl = np.linalg.slogdet(B)
+        assert s2 == sg * bs and abs(l2 - (la + bl)) < 1e-8 * max(1, abs(l2))
+    a = rng.standard_normal(5)
+    A = np.zeros((10, 10))
+    for k in range(5):
+        A[2 * k, 2 * k + 1], A[2 * k + 1, 2 * k] = a[k], -a[k]
+    sg, la = O.slogpf(A)
+    assert sg == np.sign(np.prod(a)) and abs(la - np.log(np.abs(a)).sum()) < 1e-12
+    # swapping two rows/columns flips the sign
+    P = np.eye(10)[[1, 0] + list(range(2, 10))]
+    s3, l3 = O.slogpf(P @ A @ P.T)
+    assert s3 == -sg and abs(l3 - la) < 1e-12
+    assert O.slogpf(np.zeros((4, 4)))[0] == 0
+
+
+def test_oracle_det():
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 28, 112):
+        A = rng.standard_normal((n, n))
+        s, l = O.slogdet(A)
+        s2, l2 = np.linalg.slogdet(A)
+        assert s == s2 and abs(l - l2) < 1e-9 * max(1, abs(l2))
+
+
+# ----------------------------------------------------------------- fermion algebra on bit strings
+
+
+def apply_cdag_c(occ, j, i):
+    """c+_j c_i |occ> for a sorted tuple of occupied modes; returns (sign, new) or None."""
+    if i not in occ:
+        return None
+    k = occ.index(i)
+    sign = (-1) ** k
+    rest = occ[:k] + occ[k + 1 :]
+    if j in rest:
+        return None
+    pos = sum(1 for m in rest if m < j)
+    sign *= (-1) ** pos
+    return sign, tuple(sorted(rest + (j,)))
+
+
+def permute_state(occ, perm):
+    """U_g c+_{m1} ... c+_{mn} |0> = c+_{pi m1} ... c+_{pi mn} |0>, sorted with its sign."""
+    mapped = [int(perm[m]) for m in occ]
+    inv = sum(1 for a in range(len(mapped)) for b in range(a + 1, len(mapped)) if mapped[a] > mapped[b])
+    return (-1) ** inv, tuple(sorted(mapped))
+
+
+def config_of(occ, N2):
+    s = -np.ones(N2, np.int8)
+    s[list(occ)] = 1
+    return s
+
+
+def hamiltonian_row(occ, LX, LY, t, tp, U):
+    """<s|H|s'> for all s' by operator algebra: H = -sum t_ij (c+_i c_j + c+_j c_i) + U sum n n."""
+    N = LX * LY
+    bonds = set()
+    for i in range(N):
+        x, y = divmod(i, LY)
+        for (dx, dy), tt in [((1, 0), t), ((0, 1), t), ((1, 1), tp), ((1, -1), tp)]:
+            if tt == 0:
+                continue
+            j = ((x + dx) % LX) * LY + (y + dy) % LY
+            if j != i:
+                bonds.add((min(i, j), max(i, j), tt))
+    row = {}
+    for i, j, tt in bonds:
+        for sp in range(2):
+            for a, b in ((i, j), (j, i)):
+                r = apply_cdag_c(occ, sp * N + b, sp * N + a)
+                if r is not None:
+                    row[r[1]] = row.get(r[1], 0.0) - tt * r[0]
+    diag = U * sum(1 for m in occ if m < N and (m + N) in occ)
+    return diag, row
+
+
+@pytest.mark.parametrize("kind", ["pfaffian", "bcs"])
+def test_oracle_eloc_matches_operator_algebra(kind):
+    wk = small_workload(kind=kind, LX=4, LY=4, n_up=3, n_dn=1 if kind == "pfaffian" else 3,
+                        tprime=-0.3, channels=4, n_layers=1, base_f0=(kind == "pfaffian"))
+    p = make_params(wk)
+    m = O.Model(wk, p)
+    rng = np.random.default_rng(6)
+    for trial in range(2):
+        s = L.random_configurations(wk.geom, 1, 100 + trial)[0]
+        occ = tuple(np.flatnonzero(s > 0))
+        diag, row = hamiltonian_row(occ, 4, 4, 1.0, -0.3, 8.0)
+        ps = m.psi(s)
+        ref = diag
+        for occ2, h in row.items():
+            sg, la = m.psi(config_of(occ2, 32))
+            ref += h * sg * ps[0] * math.exp(la - ps[1])
+        e = O.eloc(m, s, 8.0, 1.0, -0.3)
+        assert abs(e - ref) < 1e-9 * max(1, abs(ref)), (e, ref)
+        # same set of connected states, with the same matrix elements
+        hl = O.hops(s, 4, 4, 1.0, -0.3)
+        assert len(hl) == len(row)
+        for a, b, h in hl:
+            occ2 = tuple(sorted(set(occ) - {a} | {b}))
+            assert abs(row[occ2] - h) < 1e-15
+
+
+def test_oracle_projection_invariance():
+    """U_g |psi_P> = |psi_P>: psi_P(s) = eps * psi_P(g.s) with eps from operator algebra, and
+    psi_P = sum_g eps_g(s) psi(g.s) computed with explicitly sorted modes."""
+    wk = small_workload(LX=4, LY=4, n_up=2, n_dn=2, channels=4, n_layers=1, base_f0=True)
+    p = make_params(wk)
+    m = O.Model(wk, p)
+    s = L.random_configurations(wk.geom, 1, 11)[0]
+    occ = tuple(np.flatnonzero(s > 0))
+    ps = m.psi(s)
+    terms = []
+    for g in range(8):
+        perm = O.group_perm(g, 4, 4)
+        eps, occ_g = permute_state(occ, perm)
+        pg = m.psi(config_of(occ_g, 32))
+        assert pg[0] * eps == ps[0] and abs(pg[1] - ps[1]) < 1e-9
+        # explicit: psi(g.s) from the sorted modes of g.s
+        sgs = config_of(occ_g, 32)
+        F = O.pairing_occ(m.features(sgs).reshape(32, -1), np.array(occ_g), m.Wt, 4, 4)
+        F = F + p.F0.astype(np.float64)[np.ix_(occ_g, occ_g)]
+        sg, la = O.slogpf(F)
+        terms.append((eps * sg, la))
+    sg, la = O.signed_logsumexp([t[0] for t in terms], [t[1] for t in terms])
+    assert sg == ps[0] and abs(la - ps[1]) < 1e-10
+
+
+def test_oracle_bcs_is_pfaffian_of_block_matrix():
+    """The BCS image amplitude equals Pf of the 2N x 2N [[0, Phi], [-Phi^T, 0]] over the
+    image's modes in the order of s (spin-down-first images included)."""
+    wk = small_workload(kind="bcs", LX=4, LY=4, n_up=3, n_dn=3, tprime=0.0, channels=4, n_layers=1)
+    p = make_params(wk)
+    m = O.Model(wk, p)
+    s = L.random_configurations(wk.geom, 1, 12)[0]
+    N = 16
+    disp, _, _ = O.lattice_tables(4, 4)
+    for g in range(8):
+        perm = O.group_perm(g, 4, 4)
+        gs = np.empty_like(s)
+        gs[perm] = s
+        y = m.features(gs)
+        Phi = np.einsum("pa,pqa,qa->pq", y[0], m.Wt[disp], y[1]) + p.phi0.astype(np.float64)[disp]
+        A = np.zeros((2 * N, 2 * N))
+        A[:N, N:] = Phi
+        A[N:, :N] = -Phi.T
+        modes = perm[np.flatnonzero(s > 0)]
+        sg, la = O.slogpf(A[np.ix_(modes, modes)])
+        ne = 6
+        sg *= (-1) ** (ne * (ne - 1) // 2)
+        s2, l2 = m.image_amplitude(s, g)
+        assert s2 == sg and abs(l2 - la) < 1e-10
+
+
+def test_oracle_variational_identity_3x3():
+    """sum_s |psi|^2 E_loc / sum |psi|^2 = <psi|H|psi> / <psi|psi> with H from operator algebra
+    over the whole Hilbert space of a 3x3 lattice with 2 + 2 electrons."""
+    wk = small_workload(LX=3, LY=3, n_up=2, n_dn=2, tprime=-0.2, channels=2, n_layers=1)
+    p = make_params(wk)
+    m = O.Model(wk, p)
+    N = 9
+    basis = [tuple(sorted(u + tuple(N + x for x in d)))
+             for u in itertools.combinations(range(N), 2) for d in itertools.combinations(range(N), 2)]
+    index = {b: k for k, b in enumerate(basis)}
+    psi = np.array([(lambda r: r[0] * math.exp(r[1]))(m.psi(config_of(b, 2 * N), project=False))
+                    for b in basis])
+    H = np.zeros((len(basis), len(basis)))
+    for k, b in enumerate(basis):
+        diag, row = hamiltonian_row(b, 3, 3, 1.0, -0.2, 8.0)
+        H[k, k] += diag
+        for b2, h in row.items():
+            H[k, index[b2]] += h
+    assert np.allclose(H, H.T)
+    E_exact = psi @ H @ psi / (psi @ psi)
+    el = np.array([O.eloc(m, config_of(b, 2 * N), 8.0, 1.0, -0.2, project=False) for b in basis])
+    E_mc = np.sum(psi**2 * el) / np.sum(psi**2)
+    assert abs(E_mc - E_exact) < 1e-10
+
+
+# ----------------------------------------------------------------- RNG and statistics
+
+
+def test_oracle_philox():
+    # Random123 known-answer vectors for philox4x32-10
+    assert O.philox4x32([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert O.philox4x32([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert O.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_oracle_jackknife():
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(64)
+    v, e, errs = O.jackknife(lambda m: m, x)
+    assert abs(v - x.mean()) < 1e-15
+    assert abs(errs[1] - x.std(ddof=1) / math.sqrt(64)) < 1e-12
+    xb = x.reshape(8, 8).mean(1)
+    assert abs(errs[8] - xb.std(ddof=1) / math.sqrt(8)) < 1e-12
+    assert e == max(errs.values())
