@@ -1,0 +1,130 @@
+/* nh.h - C ABI of the MI355X (gfx950) Pfaffian/BCS backflow amplitude and local-energy path.
+ *
+ * Every pointer named "device" is caller-owned device memory; no entry point allocates,
+ * frees or throws.  Each call is asynchronous on `stream` (a hipStream_t passed as void*;
+ * NULL is the default stream) and returns an NH_* status.  Work that needs scratch memory
+ * takes a caller-provided workspace whose size comes from the matching *_workspace_size
+ * query.  Calls on different streams with different workspaces are independent.
+ *
+ * Conventions (bit-exact with the reference, ChenAo-Phys/NeuralHubbard):
+ *   configuration s: int8 [2N], +1 occupied / -1 empty, spin-up block then spin-down block,
+ *   site (x, y) -> x * LY + y (README.md:85-86; code/backflow/models.py:96-97).
+ *   mode m = spin * N + site; occupied modes in ascending order (quantax fermion_idx).
+ *   projection: sum over Flip(axis=0) @ Flip(axis=1) @ SpinInverse (code/states.py:274-279),
+ *   group element g = fx | fy << 1 | si << 2.
+ *   log amplitude: (sign, logabs) pairs, the quantax LogArray (code/backflow/models.py:211).
+ */
+#ifndef NH_H
+#define NH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NH_OK = 0,
+  NH_ERR_ARG = 1,         /* null pointer or invalid scalar argument */
+  NH_ERR_SHAPE = 2,       /* sizes outside what the kernels support */
+  NH_ERR_WORKSPACE = 3,   /* workspace smaller than the query returned */
+  NH_ERR_HIP = 4,         /* a HIP runtime call failed */
+  NH_ERR_UNSUPPORTED = 5  /* option combination not implemented */
+};
+
+enum { NH_KIND_PFAFFIAN = 0, NH_KIND_BCS = 1 };
+enum { NH_PREC_FP32 = 0, NH_PREC_MIXED = 1 };
+
+/* One backflow network and its pairing parameters (all device pointers).
+ * Network: embedding 3x3 periodic conv 2 -> C of the +-1 occupancy, then n_layers residual
+ * layers h <- h + conv3x3(gelu_tanh(h)); features y[spin][site][a] = h[spin * d + a][site],
+ * d = C / 2 (code/backflow/models.py:163-167).
+ * Weights are packed per layer as [tap = (dx+1)*3 + (dy+1)][c_in][c_out] (layer 0: c_in = 2);
+ * biases [(n_layers + 1)][C].
+ * pfaffian: wt = [4][N][d] spin-block table [uu, ud, du, dd] with the distance envelope
+ *           (code/backflow/pf_kernel.py:107-124); base = F0 [2N][2N] antisymmetric or NULL.
+ * bcs:      wt = [N][d] (code/backflow/bcs_kernel.py:63-70); base = phi0 [N] or NULL,
+ *           added as phi0[r(up_p - dn_q)]. */
+typedef struct nh_model {
+  int32_t kind;       /* NH_KIND_* */
+  int32_t LX, LY;     /* periodic lattice, both >= 3 */
+  int32_t n_up, n_dn; /* electrons per spin */
+  int32_t channels;   /* C: 8, 16 or 32 */
+  int32_t n_layers;   /* residual layers after the embedding conv */
+  int32_t precision;  /* NH_PREC_FP32 or NH_PREC_MIXED */
+  int32_t project;    /* 1: 8-element projection, 0: the bare network */
+  int32_t reserved;
+  const float* conv_w;
+  const float* conv_b;
+  const float* wt;
+  const float* base;
+  const void* conv_w_lp; /* NH_PREC_MIXED: conv_w repacked by nh_pack_lowp, else NULL */
+} nh_model;
+
+/* ---- a6: occupied pairing block  (replaces pf_kernel.pairing_occ, code/backflow/pf_kernel.py:335-351)
+ * F[b,p,q] = sum_a Y[b,p,a] Y[b,q,a] Wt[2 s_p + s_q][r(p,q)][a], r = site_p - site_q
+ * periodic, s_p = idx[b,p] / N.  Exactly antisymmetric.  Y [batch][n][d], idx [batch][n],
+ * Wt [4][N][d], F [batch][n][n] (device).  d % 4 == 0, n <= 1024. */
+int nh_pairing_occ(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch, int32_t n,
+                   const float* Y, const int32_t* idx, const float* Wt, float* F);
+
+/* ---- a9: up x down pairing block  (replaces bcs_kernel.pairing_ud, code/backflow/bcs_kernel.py:257-271)
+ * F[b,p,q] = sum_a Yu[b,p,a] Wt[r(iu_p - idn_q)][a] Yd[b,q,a]. */
+int nh_pairing_ud(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch, int32_t n_up,
+                  int32_t n_dn, const float* Yu, const float* Yd, const int32_t* iu,
+                  const int32_t* idn, const float* Wt, float* F);
+
+/* ---- a8: batched Pfaffian  (replaces fermix.slogpf, called at code/backflow/models.py:210)
+ * A [batch][n][n] antisymmetric (only the strict lower triangle is read), n even, n <= 256.
+ * Outputs sign (+1/-1, 0 if singular) and logabs (-inf if singular) per matrix.
+ * Parlett-Reid elimination with partial pivoting, FP32, pivot logs summed in FP64. */
+int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs);
+
+/* ---- a9: batched determinant  (replaces fermix.slogdet, called at code/backflow/models.py:297)
+ * A [batch][n][n], n <= 192.  LU with partial pivoting, FP32, logs summed in FP64. */
+int nh_slogdet(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs);
+
+/* ---- a4: backflow features  (replaces PfaffianBackflow.features, code/backflow/models.py:163-167)
+ * s int8 [batch][2N] -> y float [batch][2][N][d]  (no projection; the identity image). */
+int nh_features(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* y);
+
+/* ---- a12 + model __call__: projected amplitude  (replaces quantax Variational.__call__ over
+ * PfaffianBackflow/BCSBackflow.__call__, code/backflow/models.py:208-211, 294-302,
+ * code/states.py:318-320).  s int8 [batch][2N] -> sign float [batch], logabs double [batch]. */
+size_t nh_log_psi_workspace_size(const nh_model* model, int32_t batch);
+int nh_log_psi(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* sign,
+               double* logabs, void* workspace, size_t workspace_bytes);
+
+/* ---- a13: Hubbard local energy  (replaces quantax Hubbard.Oloc, code/quick_start.py:33,
+ * built at code/states.py:244-247).  E_loc(s) = U sum_i n_iu n_id
+ *   - sum_{<ij>,spin} t_ij JW ψ(s^{i->j}) / ψ(s), t_ij = t (nearest) or tprime (diagonal,
+ * used when tprime != 0).  sign_s / logabs_s are ψ(s) from nh_log_psi.  eloc double [batch].
+ * Connected configurations are evaluated in full (8 images each), in chunks that fit the
+ * workspace; the query returns the size for `chunk` samples per chunk. */
+size_t nh_local_energy_workspace_size(const nh_model* model, double tprime, int32_t chunk);
+int nh_local_energy(void* stream, const nh_model* model, double t, double tprime, double U,
+                    int32_t batch, const int8_t* s, const float* sign_s, const double* logabs_s,
+                    double* eloc, int32_t chunk, void* workspace, size_t workspace_bytes);
+
+/* Hop enumeration used by nh_local_energy, exposed for checking:
+ * for each sample, hops in order (occupied mode ascending, partner in the order
+ * (+x, -x, +y, -y, +x+y, +x-y, -x+y, -x-y)); out_from/out_to/out_coef [batch][max_hops],
+ * count [batch].  max_hops = (n_up + n_dn) * z, z = 4 or 8. */
+int nh_hops(void* stream, int32_t LX, int32_t LY, double t, double tprime, int32_t batch,
+            int32_t n_occ, const int8_t* s, int32_t max_hops, int32_t* count, int32_t* out_from,
+            int32_t* out_to, float* out_coef);
+
+/* ---- measurement hook (no reference counterpart; used by bench.py): while enabled, every
+ * launch of a kernel class is bracketed by HIP events recorded on the launch stream.
+ * nh_profile_query waits for the recorded events and returns the launch count and the summed
+ * kernel time of one class; nh_profile_reset drops the records. */
+enum { NH_PROF_FEATURES = 0, NH_PROF_PAIRING_LINALG = 1, NH_PROF_OTHER = 2, NH_PROF_CLASSES = 3 };
+int nh_profile_enable(int32_t on);
+int nh_profile_query(int32_t kernel_class, int64_t* launches, double* total_ms);
+int nh_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NH_H */

@@ -1,0 +1,354 @@
+r"""Host side of the Pfaffian/BCS backflow path, mirroring the reference's interfaces.
+
+Reference call surface (ChenAo-Phys/NeuralHubbard) and what replaces it here:
+
+=========================================================  ======================================
+reference                                                  here (all device work in libnh.so)
+=========================================================  ======================================
+``pf_kernel.build_wtilde`` (``pf_kernel.py:107-124``)        `build_wtilde` (host, parameters only)
+``bcs_kernel.build_wt_ud`` (``bcs_kernel.py:63-70``)        `build_wt_ud`
+``pf_kernel.pairing_occ`` (``pf_kernel.py:335-351``)        `pairing_occ` -> ``nh_pairing_occ``
+``bcs_kernel.pairing_ud`` (``bcs_kernel.py:257-271``)       `pairing_ud` -> ``nh_pairing_ud``
+``fermix.slogpf`` / ``slogdet`` (``models.py:210, 297``)    `slogpf` / `slogdet` -> ``nh_slogpf``...
+``PfaffianBackflow`` / ``BCSBackflow`` (``models.py``)      `PfaffianBackflow` / `BCSBackflow`
+``quantax.state.Variational`` (``states.py:318-320``)       `Variational` -> ``nh_log_psi``
+``quantax.operator.Hubbard.Oloc`` (``quick_start.py:33``)   `Hubbard.Oloc` -> ``nh_local_energy``
+``quantax.utils.LogArray`` (``models.py:211``)              `LogArray`
+=========================================================  ======================================
+
+Tensors are torch tensors on the current CUDA (HIP) device; kernels run on the current
+stream.  Errors raise `NhError`; non-finite amplitudes propagate as sign 0 / logabs -inf (a
+singular matrix) or NaN (a configuration with the wrong particle number), as the reference
+lets them propagate to its callers (``code/measure_orderparams.py:153-156``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .configs import Params, Workload
+from .lattice import displacement_tables
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+# ------------------------------------------------------------------------ value type
+
+
+@dataclass
+class LogArray:
+    """sign * exp(logabs), the quantax ``LogArray`` (``README.md:90-91``)."""
+
+    sign: object
+    logabs: object
+
+    def __mul__(self, other):
+        if isinstance(other, LogArray):
+            return LogArray(self.sign * other.sign, self.logabs + other.logabs)
+        return LogArray(self.sign * np.sign(other), self.logabs + np.log(np.abs(other)))
+
+    def numpy(self):
+        s = self.sign.detach().cpu().numpy() if hasattr(self.sign, "detach") else np.asarray(self.sign)
+        l = self.logabs.detach().cpu().numpy() if hasattr(self.logabs, "detach") else np.asarray(self.logabs)
+        return s, l
+
+
+# ------------------------------------------------------------------------ parameter tables
+
+
+def softplus(t: float) -> float:
+    return math.log1p(math.exp(-abs(t))) + max(t, 0.0)
+
+
+def build_wtilde(W, theta, LX, LY) -> np.ndarray:
+    """(3, N, d) raw W -> (4, N, d) table [uu, ud, du, dd] with the distance envelope;
+    uu/dd antisymmetrised in r and du(r) = -ud(-r) (``code/backflow/pf_kernel.py:107-124``).
+    Computed in float64, returned as float32."""
+    _, neg, dist = displacement_tables(LX, LY)
+    W = np.asarray(W, np.float64)
+    N = LX * LY
+    blocks = [(W[1] - W[1][neg]) / 2, W[0], -W[0][neg], (W[2] - W[2][neg]) / 2]
+    pref = np.exp(-softplus(float(theta)) * dist)
+    pref /= N * pref.sum()
+    return (np.stack(blocks) * pref[None, :, None]).astype(np.float32)
+
+
+def build_wt_ud(W, theta, LX, LY) -> np.ndarray:
+    """(N, d) -> W * pref (``code/backflow/bcs_kernel.py:63-70``)."""
+    _, _, dist = displacement_tables(LX, LY)
+    pref = np.exp(-softplus(float(theta)) * dist)
+    pref /= LX * LY * pref.sum()
+    return (np.asarray(W, np.float64) * pref[:, None]).astype(np.float32)
+
+
+def pack_conv(params: Params):
+    """Conv weights as [layer][tap = (dx+1)*3 + (dy+1)][c_in][c_out] and biases [layer][C]."""
+    w = [np.ascontiguousarray(np.transpose(np.asarray(cw, np.float32), (2, 3, 1, 0))).reshape(-1)
+         for cw in params.conv_w]
+    b = [np.asarray(cb, np.float32).reshape(-1) for cb in params.conv_b]
+    return np.concatenate(w), np.concatenate(b)
+
+
+# ------------------------------------------------------------------------ kernel wrappers
+
+
+def _dev(x, dtype):
+    torch = _torch()
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device="cuda")
+
+
+def pairing_occ(Y, idx, Wt, LX: int, LY: int):
+    """Occupied pairing block F[b, p, q] (``pf_kernel.pairing_occ``); Y (B, n, d) float32,
+    idx (B, n) int32 mode indices, Wt (4, N, d).  Returns (B, n, n) float32, exactly
+    antisymmetric."""
+    torch = _torch()
+    _lib.require_gpu()
+    Y = Y.contiguous().float()
+    idx = idx.contiguous().to(torch.int32)
+    Wt = Wt.contiguous().float()
+    B, n, d = Y.shape
+    F = torch.empty((B, n, n), dtype=torch.float32, device=Y.device)
+    lib = _lib.load()
+    _lib.check(lib.nh_pairing_occ(_lib.stream_handle(), LX, LY, d, B, n, _lib.ptr(Y), _lib.ptr(idx),
+                                  _lib.ptr(Wt), _lib.ptr(F)), "nh_pairing_occ")
+    return F
+
+
+def pairing_ud(Yu, Yd, iu, idn, Wt, LX: int, LY: int):
+    """Up x down pairing block (``bcs_kernel.pairing_ud``)."""
+    torch = _torch()
+    _lib.require_gpu()
+    Yu, Yd = Yu.contiguous().float(), Yd.contiguous().float()
+    iu, idn = iu.contiguous().to(torch.int32), idn.contiguous().to(torch.int32)
+    Wt = Wt.contiguous().float()
+    B, nu, d = Yu.shape
+    nd = Yd.shape[1]
+    F = torch.empty((B, nu, nd), dtype=torch.float32, device=Yu.device)
+    lib = _lib.load()
+    _lib.check(lib.nh_pairing_ud(_lib.stream_handle(), LX, LY, d, B, nu, nd, _lib.ptr(Yu), _lib.ptr(Yd),
+                                 _lib.ptr(iu), _lib.ptr(idn), _lib.ptr(Wt), _lib.ptr(F)), "nh_pairing_ud")
+    return F
+
+
+def slogpf(F):
+    """Batched (sign, log|Pf|) of antisymmetric (B, n, n) float32 (``fermix.slogpf``)."""
+    torch = _torch()
+    _lib.require_gpu()
+    F = F.contiguous().float()
+    B, n, _ = F.shape
+    sign = torch.empty(B, dtype=torch.float32, device=F.device)
+    logabs = torch.empty(B, dtype=torch.float64, device=F.device)
+    _lib.check(_lib.load().nh_slogpf(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs)),
+               "nh_slogpf")
+    return sign, logabs
+
+
+def slogdet(F):
+    """Batched (sign, log|det|) of (B, n, n) float32 (``fermix.slogdet``)."""
+    torch = _torch()
+    _lib.require_gpu()
+    F = F.contiguous().float()
+    B, n, _ = F.shape
+    sign = torch.empty(B, dtype=torch.float32, device=F.device)
+    logabs = torch.empty(B, dtype=torch.float64, device=F.device)
+    _lib.check(_lib.load().nh_slogdet(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs)),
+               "nh_slogdet")
+    return sign, logabs
+
+
+# ------------------------------------------------------------------------ models
+
+
+class _Backflow:
+    kind = None
+
+    def __init__(self, workload: Workload, params: Params, precision: str = "fp32"):
+        torch = _torch()
+        _lib.require_gpu()
+        self.workload = workload
+        self.params = params
+        self.precision = precision
+        w, b = pack_conv(params)
+        self.conv_w = _dev(w, torch.float32)
+        self.conv_b = _dev(b, torch.float32)
+        LX, LY = workload.LX, workload.LY
+        if self.kind == _lib.NH_KIND_PFAFFIAN:
+            self.Wt = _dev(build_wtilde(params.W, params.theta, LX, LY), torch.float32)
+            self.base = None if params.F0 is None else _dev(params.F0, torch.float32)
+        else:
+            self.Wt = _dev(build_wt_ud(params.W, params.theta, LX, LY), torch.float32)
+            self.base = None if params.phi0 is None else _dev(params.phi0, torch.float32)
+        self.conv_w_lp = None
+        if precision == "mixed":
+            from .mixed import pack_lowp
+
+            self.conv_w_lp = pack_lowp(self)
+
+    @property
+    def d(self):
+        return self.workload.d
+
+    def struct(self, project: bool) -> _lib.NhModel:
+        w = self.workload
+        return _lib.NhModel(
+            self.kind, w.LX, w.LY, w.n_up, w.n_dn, w.channels, w.n_layers,
+            _lib.NH_PREC_MIXED if self.precision == "mixed" else _lib.NH_PREC_FP32,
+            1 if project else 0, 0,
+            self.conv_w.data_ptr(), self.conv_b.data_ptr(), self.Wt.data_ptr(),
+            0 if self.base is None else self.base.data_ptr(),
+            0 if self.conv_w_lp is None else self.conv_w_lp.data_ptr(),
+        )
+
+    def features(self, s):
+        """(B, 2, N, d) features of configurations s (B, 2N) int8 (``models.py:163-167``)."""
+        torch = _torch()
+        s = torch.as_tensor(s, dtype=torch.int8, device="cuda").contiguous()
+        if s.ndim == 1:
+            s = s[None]
+        w = self.workload
+        y = torch.empty((s.shape[0], 2, w.N, w.d), dtype=torch.float32, device="cuda")
+        m = self.struct(False)
+        _lib.check(_lib.load().nh_features(_lib.stream_handle(), ctypes.byref(m), s.shape[0], _lib.ptr(s),
+                                           _lib.ptr(y)), "nh_features")
+        return y
+
+    def __call__(self, s) -> LogArray:
+        """Unprojected amplitude of configurations s (``models.py:208-211``)."""
+        return Variational(self, symm=False)(s)
+
+
+class PfaffianBackflow(_Backflow):
+    """psi(s) = Pf[n * f(n) * n] (``code/backflow/models.py:121-211``)."""
+
+    kind = _lib.NH_KIND_PFAFFIAN
+
+
+class BCSBackflow(_Backflow):
+    """psi(s) = (-1)^{n(n-1)/2} det[n_up * f_ud(n) * n_dn] (``code/backflow/models.py:214-302``)."""
+
+    kind = _lib.NH_KIND_BCS
+
+
+def build_backflow(workload: Workload, params: Params, precision: str = "fp32"):
+    """The backflow of a workload (``build_backflow``, ``code/backflow/models.py:305-332``)."""
+    if workload.kind == "pfaffian":
+        return PfaffianBackflow(workload, params, precision)
+    if workload.kind == "bcs":
+        return BCSBackflow(workload, params, precision)
+    raise ValueError(f"Unknown backflow kind {workload.kind!r}; use 'bcs' or 'pfaffian'.")
+
+
+# ------------------------------------------------------------------------ state and operator
+
+
+class _Workspace:
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes: int):
+        torch = _torch()
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = None
+            self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device="cuda")
+        return self.buf
+
+
+class Variational:
+    """Projected state (``quantax.state.Variational(model, symm=paper_symmetry())``,
+    ``code/states.py:318-320``): ``state(spins)`` returns the LogArray of
+    sum_{g in G} eta_g(s) psi(g.s) over Flip(0) @ Flip(1) @ SpinInverse (``symm=True``), or the
+    bare network (``symm=False``).  ``max_parallel`` bounds the configurations per launch."""
+
+    def __init__(self, model: _Backflow, symm: bool = True, max_parallel: int = 8192):
+        self.model = model
+        self.symm = bool(symm)
+        self.max_parallel = int(max_parallel)
+        self._ws = _Workspace()
+
+    def __call__(self, spins) -> LogArray:
+        torch = _torch()
+        s = torch.as_tensor(spins, dtype=torch.int8, device="cuda").contiguous()
+        if s.ndim == 1:
+            s = s[None]
+        B = s.shape[0]
+        sign = torch.empty(B, dtype=torch.float32, device="cuda")
+        logabs = torch.empty(B, dtype=torch.float64, device="cuda")
+        m = self.model.struct(self.symm)
+        lib = _lib.load()
+        for b0 in range(0, B, self.max_parallel):
+            nb = min(self.max_parallel, B - b0)
+            need = lib.nh_log_psi_workspace_size(ctypes.byref(m), nb)
+            if need == 0:
+                raise _lib.NhError("invalid model description")
+            ws = self._ws.get(need)
+            _lib.check(lib.nh_log_psi(_lib.stream_handle(), ctypes.byref(m), nb, _lib.ptr(s[b0:]),
+                                      _lib.ptr(sign[b0:]), _lib.ptr(logabs[b0:]), _lib.ptr(ws), ws.numel()),
+                       "nh_log_psi")
+        return LogArray(sign, logabs)
+
+
+@dataclass
+class Samples:
+    """Configurations with their amplitudes (``quantax.sampler.Samples``)."""
+
+    spins: object
+    psi: LogArray | None = None
+
+
+class Hubbard:
+    """Hubbard model (``quantax.operator.Hubbard``; built at ``code/states.py:244-247``):
+    H = -t sum_<ij>,s (c+_is c_js + h.c.) - t' sum_<<ij>>,s (...) + U sum_i n_iu n_id."""
+
+    def __init__(self, U: float = 8.0, t: float = 1.0, tprime: float = 0.0, chunk: int = 16):
+        self.U, self.t, self.tprime = float(U), float(t), float(tprime)
+        self.chunk = int(chunk)
+        self._ws = _Workspace()
+
+    def Oloc(self, state: Variational, samples) -> "object":
+        """E_loc(s) for every sample (float64 tensor)."""
+        torch = _torch()
+        if not isinstance(samples, Samples):
+            samples = Samples(samples)
+        s = torch.as_tensor(samples.spins, dtype=torch.int8, device="cuda").contiguous()
+        if s.ndim == 1:
+            s = s[None]
+        psi = samples.psi if samples.psi is not None else state(s)
+        B = s.shape[0]
+        eloc = torch.empty(B, dtype=torch.float64, device="cuda")
+        m = state.model.struct(state.symm)
+        lib = _lib.load()
+        chunk = max(1, min(self.chunk, B))
+        need = lib.nh_local_energy_workspace_size(ctypes.byref(m), self.tprime, chunk)
+        if need == 0:
+            raise _lib.NhError("invalid model description")
+        ws = self._ws.get(need)
+        sign = psi.sign.contiguous().float()
+        logabs = psi.logabs.contiguous().double()
+        _lib.check(lib.nh_local_energy(_lib.stream_handle(), ctypes.byref(m), self.t, self.tprime, self.U, B,
+                                       _lib.ptr(s), _lib.ptr(sign), _lib.ptr(logabs), _lib.ptr(eloc), chunk,
+                                       _lib.ptr(ws), ws.numel()), "nh_local_energy")
+        return eloc
+
+
+def hops(s, LX, LY, t, tprime, n_occ):
+    """Hop enumeration of ``nh_hops``: (count (B,), from (B, H), to (B, H), coef (B, H))."""
+    torch = _torch()
+    s = torch.as_tensor(s, dtype=torch.int8, device="cuda").contiguous()
+    B = s.shape[0]
+    H = n_occ * (8 if tprime != 0 else 4)
+    count = torch.empty(B, dtype=torch.int32, device="cuda")
+    fr = torch.empty((B, H), dtype=torch.int32, device="cuda")
+    to = torch.empty((B, H), dtype=torch.int32, device="cuda")
+    co = torch.empty((B, H), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.load().nh_hops(_lib.stream_handle(), LX, LY, t, tprime, B, n_occ, _lib.ptr(s), H,
+                                   _lib.ptr(count), _lib.ptr(fr), _lib.ptr(to), _lib.ptr(co)), "nh_hops")
+    return count, fr, to, co

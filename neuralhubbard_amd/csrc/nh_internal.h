@@ -1,0 +1,30 @@
+// Internal launchers shared by the translation units of libnh.
+#pragma once
+
+#include "nh_common.h"
+
+namespace nh {
+
+int check_model(const nh_model* m);
+
+// slots (threads) of one feature-network block for an N-site lattice
+int cnn_slots(int N);
+
+// Features of images [img0, img_end) of an item list (ipi images per item) into
+// Y[(img - img0)][2N][d].
+int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
+               const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
+int launch_cnn_mixed(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
+                     const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
+
+// Pairing block + Pfaffian (or determinant) of images [img0, img_end), writing the signed
+// log of eta_g(s) psi(g.s) per image into img_sign / img_logabs (indexed by global image).
+int launch_image_amplitude(hipStream_t st, const nh_model* m, const int8_t* s,
+                           const int32_t* item_b, const int32_t* item_hop, int img0, int img_end,
+                           int ipi, const float* Y, float* img_sign, double* img_logabs);
+
+// Signed log-sum over the ipi images of each item in [item0, item_end).
+int launch_combine(hipStream_t st, const int32_t* item_hop, int item0, int item_end, int ipi,
+                   const float* img_sign, const double* img_logabs, float* sign, double* logabs);
+
+}  // namespace nh

@@ -1,0 +1,319 @@
+/* CPU restatement of the reference's amplitude + local-energy path (C, OpenMP over images).
+ *
+ * TEST AND BASELINE INFRASTRUCTURE ONLY: built twice by oracle/build.py,
+ *   libnhcpu32.so  REAL = float   the CPU baseline timed by bench.py ("port" of the
+ *                                 reference's CPU path: occupied-block pairing as
+ *                                 pairing_occ_reference, code/backflow/pf_kernel.py:371-382,
+ *                                 then a Parlett-Reid Pfaffian in FP32);
+ *   libnhcpu64.so  REAL = double  an FP64 oracle fast enough for 16x16 checks; it is pinned
+ *                                 to the numpy oracle (oracle/oracle.py) by tests/test_cpu_port.py.
+ * The product (neuralhubbard_amd) never loads either library.
+ *
+ * Model layout: the packed weights of include/nh.h (conv [layer][tap][cin][cout], biases,
+ * Wt table, optional base matrix), converted to REAL on use.  Conventions as in the oracle:
+ * image g of s has occupied modes g(idx(s)) in the order of idx(s), so the Pfaffian carries
+ * the fermionic sign of the group element; fermion_inverse_sign = (-1)^{Ne(Ne-1)/2}.
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL float
+#endif
+
+typedef struct {
+  int kind; /* 0 pfaffian, 1 bcs */
+  int LX, LY, n_up, n_dn, C, L, project;
+  const float* w;    /* packed conv weights */
+  const float* b;    /* biases */
+  const float* wt;   /* pf [4][N][d], bcs [N][d] */
+  const float* base; /* pf F0 [2N][2N] or bcs phi0 [N] or NULL */
+} cpu_model;
+
+static int gmap(int m, int g, int N, int LX, int LY) {
+  int spin = m >= N, site = m - spin * N;
+  int x = site / LY, y = site % LY;
+  if (g & 1) x = x ? LX - x : 0;
+  if (g & 2) y = y ? LY - y : 0;
+  spin ^= (g >> 2) & 1;
+  return spin * N + x * LY + y;
+}
+
+static int disp(int sp, int sq, int LX, int LY) {
+  int dx = sp / LY - sq / LY, dy = sp % LY - sq % LY;
+  if (dx < 0) dx += LX;
+  if (dy < 0) dy += LY;
+  return dx * LY + dy;
+}
+
+static REAL gelu(REAL x) {
+  const REAL k0 = (REAL)0.7978845608028654;
+  return (REAL)0.5 * x * ((REAL)1 + tanh(k0 * (x + (REAL)0.044715 * x * x * x)));
+}
+
+typedef struct {
+  REAL *h, *g, *acc, *y, *F, *tau, *v;
+  int* modes;
+} scratch;
+
+static scratch alloc_scratch(const cpu_model* m) {
+  int N = m->LX * m->LY, n = m->n_up + m->n_dn, C = m->C;
+  scratch s;
+  s.h = malloc(sizeof(REAL) * C * N);
+  s.g = malloc(sizeof(REAL) * C * N);
+  s.acc = malloc(sizeof(REAL) * C);
+  s.y = malloc(sizeof(REAL) * 2 * N * (C / 2));
+  s.F = malloc(sizeof(REAL) * (size_t)n * n);
+  s.tau = malloc(sizeof(REAL) * n);
+  s.v = malloc(sizeof(REAL) * n);
+  s.modes = malloc(sizeof(int) * n);
+  return s;
+}
+
+static void free_scratch(scratch* s) {
+  free(s->h); free(s->g); free(s->acc); free(s->y); free(s->F); free(s->tau); free(s->v); free(s->modes);
+}
+
+/* out[co][site] (+)= b[co] + sum_tap sum_ci w[tap][ci][co] in[ci][nbr(site, tap)] */
+static void conv(const cpu_model* m, const float* w, const float* b, int cin, const REAL* in, REAL* out,
+                 REAL* acc, int residual) {
+  const int LX = m->LX, LY = m->LY, N = LX * LY, C = m->C;
+  for (int site = 0; site < N; ++site) {
+    const int x = site / LY, y = site % LY;
+    for (int co = 0; co < C; ++co) acc[co] = b[co];
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int nb = ((x + dx + LX) % LX) * LY + (y + dy + LY) % LY;
+        const float* wt = w + ((dx + 1) * 3 + (dy + 1)) * cin * C;
+        for (int ci = 0; ci < cin; ++ci) {
+          const REAL v = in[ci * N + nb];
+          const float* wr = wt + ci * C;
+          for (int co = 0; co < C; ++co) acc[co] += (REAL)wr[co] * v;
+        }
+      }
+    for (int co = 0; co < C; ++co) out[co * N + site] = residual ? out[co * N + site] + acc[co] : acc[co];
+  }
+}
+
+/* features of configuration x (+-1, 2N) into s->y [2N][d] */
+static void features(const cpu_model* m, const int8_t* x, scratch* s) {
+  const int N = m->LX * m->LY, C = m->C, d = C / 2;
+  for (int i = 0; i < 2 * N; ++i) s->g[i] = (REAL)x[i];
+  conv(m, m->w, m->b, 2, s->g, s->h, s->acc, 0);
+  const float* w = m->w + 9 * 2 * C;
+  for (int l = 1; l <= m->L; ++l, w += 9 * C * C) {
+    for (int i = 0; i < C * N; ++i) s->g[i] = gelu(s->h[i]);
+    conv(m, w, m->b + l * C, C, s->g, s->h, s->acc, 1);
+  }
+  for (int spin = 0; spin < 2; ++spin)
+    for (int site = 0; site < N; ++site)
+      for (int a = 0; a < d; ++a) s->y[(spin * N + site) * d + a] = s->h[(spin * d + a) * N + site];
+}
+
+/* Parlett-Reid on the strict lower triangle of dense A (n x n). */
+static void slogpf(REAL* A, int n, REAL* tau, REAL* v, float* sign, double* logabs) {
+  double la = 0.0;
+  float sg = 1.f;
+  for (int k = 0; k + 1 < n; k += 2) {
+    int kp = k + 1;
+    REAL best = -1;
+    for (int i = k + 1; i < n; ++i) {
+      REAL a = fabs(A[i * n + k]);
+      if (a > best) { best = a; kp = i; }
+    }
+    const int a = k + 1, b = kp;
+    if (b != a) {
+      REAL t = A[a * n + k]; A[a * n + k] = A[b * n + k]; A[b * n + k] = t;
+      for (int i = a + 1; i < b; ++i) {
+        REAL t1 = A[i * n + a], t2 = A[b * n + i];
+        A[i * n + a] = -t2; A[b * n + i] = -t1;
+      }
+      A[b * n + a] = -A[b * n + a];
+      for (int i = b + 1; i < n; ++i) { REAL t1 = A[i * n + a]; A[i * n + a] = A[i * n + b]; A[i * n + b] = t1; }
+      sg = -sg;
+    }
+    const REAL piv = A[a * n + k];
+    if (piv == 0 || !isfinite(piv)) { *sign = 0.f; *logabs = -INFINITY; return; }
+    if (piv > 0) sg = -sg;
+    la += log(fabs((double)piv));
+    for (int i = k + 2; i < n; ++i) { tau[i] = A[i * n + k] / piv; v[i] = A[i * n + a]; }
+    for (int i = k + 3; i < n; ++i) {
+      const REAL ti = tau[i], vi = v[i];
+      REAL* row = A + i * n;
+      for (int j = k + 2; j < i; ++j) row[j] += ti * v[j] - vi * tau[j];
+    }
+  }
+  *sign = sg;
+  *logabs = la;
+}
+
+static void slogdet(REAL* A, int n, REAL* l, float* sign, double* logabs) {
+  double la = 0.0;
+  float sg = 1.f;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    REAL best = -1;
+    for (int i = k; i < n; ++i) { REAL a = fabs(A[i * n + k]); if (a > best) { best = a; p = i; } }
+    if (p != k) {
+      for (int j = k; j < n; ++j) { REAL t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t; }
+      sg = -sg;
+    }
+    const REAL piv = A[k * n + k];
+    if (piv == 0 || !isfinite(piv)) { *sign = 0.f; *logabs = -INFINITY; return; }
+    if (piv < 0) sg = -sg;
+    la += log(fabs((double)piv));
+    for (int i = k + 1; i < n; ++i) l[i] = A[i * n + k] / piv;
+    for (int i = k + 1; i < n; ++i) {
+      const REAL li = l[i];
+      REAL* row = A + i * n;
+      const REAL* uk = A + k * n;
+      for (int j = k + 1; j < n; ++j) row[j] -= li * uk[j];
+    }
+  }
+  *sign = sg;
+  *logabs = la;
+}
+
+static float parity(long long k) { return (k & 1) ? -1.f : 1.f; }
+
+/* eta_g(s) psi(g.s) of configuration x under group element g. */
+static void image_amp(const cpu_model* m, const int8_t* x, int g, scratch* s, int8_t* xg, float* sign,
+                      double* logabs) {
+  const int LX = m->LX, LY = m->LY, N = LX * LY, d = m->C / 2;
+  const int ne = m->n_up + m->n_dn;
+  for (int i = 0; i < 2 * N; ++i) xg[gmap(i, g, N, LX, LY)] = x[i];
+  features(m, xg, s);
+  int c = 0;
+  for (int i = 0; i < 2 * N && c <= ne; ++i)
+    if (x[i] > 0) { if (c < ne) s->modes[c] = gmap(i, g, N, LX, LY); ++c; }
+  if (c != ne) { *sign = 0.f; *logabs = NAN; return; }
+  if (m->kind == 0) {
+    const int n = ne;
+    for (int p = 1; p < n; ++p) {
+      const int mp = s->modes[p], sp = mp >= N;
+      for (int q = 0; q < p; ++q) {
+        const int mq = s->modes[q], sq = mq >= N;
+        const int r = disp(mp - sp * N, mq - sq * N, LX, LY);
+        const float* w = m->wt + ((size_t)(2 * sp + sq) * N + r) * d;
+        REAL acc = 0;
+        for (int a = 0; a < d; ++a) acc += (s->y[mp * d + a] * s->y[mq * d + a]) * (REAL)w[a];
+        if (m->base) acc += (REAL)m->base[(size_t)mp * 2 * N + mq];
+        s->F[p * n + q] = acc;
+      }
+    }
+    slogpf(s->F, n, s->tau, s->v, sign, logabs);
+    *sign *= parity((long long)ne * (ne - 1) / 2);
+  } else {
+    const int n = m->n_up, si = (g >> 2) & 1;
+    const int* rows = si ? s->modes + n : s->modes;
+    const int* cols = si ? s->modes : s->modes + n;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) {
+        const int sq = cols[q] - N;
+        const int r = disp(rows[p], sq, LX, LY);
+        const float* w = m->wt + (size_t)r * d;
+        REAL acc = 0;
+        for (int a = 0; a < d; ++a) acc += (s->y[rows[p] * d + a] * (REAL)w[a]) * s->y[cols[q] * d + a];
+        if (m->base) acc += (REAL)m->base[r];
+        s->F[p * n + q] = acc;
+      }
+    slogdet(s->F, n, s->tau, sign, logabs);
+    *sign *= parity((long long)n * (n - 1) / 2);
+    if (si) *sign *= parity(n);
+    *sign *= parity((long long)ne * (ne - 1) / 2);
+  }
+}
+
+static void combine(const float* sg, const double* la, int k, float* sign, double* logabs) {
+  double mx = -INFINITY;
+  for (int g = 0; g < k; ++g) {
+    if (isnan(la[g])) { *sign = 0.f; *logabs = NAN; return; }
+    if (sg[g] != 0.f && la[g] > mx) mx = la[g];
+  }
+  double tot = 0.0;
+  if (mx > -INFINITY)
+    for (int g = 0; g < k; ++g)
+      if (sg[g] != 0.f) tot += sg[g] * exp(la[g] - mx);
+  if (tot == 0.0) { *sign = 0.f; *logabs = -INFINITY; return; }
+  *sign = tot > 0 ? 1.f : -1.f;
+  *logabs = mx + log(fabs(tot));
+}
+
+/* psi of n_items configurations x [n_items][2N] (projected when m->project). */
+int nhcpu_psi(const cpu_model* m, int n_items, const int8_t* x, float* sign, double* logabs, int threads) {
+  const int N = m->LX * m->LY, G = m->project ? 8 : 1;
+  const long long nimg = (long long)n_items * G;
+  float* isg = malloc(sizeof(float) * nimg);
+  double* ila = malloc(sizeof(double) * nimg);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    scratch s = alloc_scratch(m);
+    int8_t* xg = malloc(2 * N);
+#pragma omp for schedule(dynamic, 1)
+    for (long long i = 0; i < nimg; ++i)
+      image_amp(m, x + (i / G) * 2 * N, (int)(i % G), &s, xg, isg + i, ila + i);
+    free(xg);
+    free_scratch(&s);
+  }
+  for (int i = 0; i < n_items; ++i) combine(isg + (long long)i * G, ila + (long long)i * G, G, sign + i, logabs + i);
+  free(isg);
+  free(ila);
+  return 0;
+}
+
+static const int NDX[8] = {1, -1, 0, 0, 1, 1, -1, -1};
+static const int NDY[8] = {0, 0, 1, -1, 1, -1, 1, -1};
+
+/* E_loc of `batch` configurations, psi(s') of every connected s' evaluated in full. */
+int nhcpu_local_energy(const cpu_model* m, double t, double tp, double U, int batch, const int8_t* x,
+                       const float* sign_s, const double* logabs_s, double* eloc, int threads) {
+  const int LX = m->LX, LY = m->LY, N = LX * LY, z = tp != 0.0 ? 8 : 4;
+  const int ne = m->n_up + m->n_dn, maxh = ne * z;
+  int8_t* xs = malloc((size_t)batch * maxh * 2 * N);
+  float* coef = malloc(sizeof(float) * (size_t)batch * maxh);
+  int* cnt = malloc(sizeof(int) * batch);
+  long long tot = 0;
+  for (int b = 0; b < batch; ++b) {
+    const int8_t* sb = x + (size_t)b * 2 * N;
+    int c = 0;
+    for (int mo = 0; mo < 2 * N; ++mo) {
+      if (sb[mo] <= 0) continue;
+      const int spin = mo >= N, site = mo - spin * N, xx = site / LY, yy = site % LY;
+      for (int k = 0; k < z; ++k) {
+        const int mt = spin * N + ((xx + NDX[k] + LX) % LX) * LY + (yy + NDY[k] + LY) % LY;
+        if (sb[mt] > 0) continue;
+        const int lo = mo < mt ? mo : mt, hi = mo < mt ? mt : mo;
+        int between = 0;
+        for (int q = lo + 1; q < hi; ++q) between += sb[q] > 0;
+        const double tt = k < 4 ? t : tp;
+        int8_t* xc = xs + ((size_t)tot + c) * 2 * N;
+        memcpy(xc, sb, 2 * N);
+        xc[mo] = -1;
+        xc[mt] = 1;
+        coef[tot + c] = (float)((between & 1) ? tt : -tt);
+        ++c;
+      }
+    }
+    cnt[b] = c;
+    tot += c;
+  }
+  float* hs = malloc(sizeof(float) * (tot ? tot : 1));
+  double* hl = malloc(sizeof(double) * (tot ? tot : 1));
+  nhcpu_psi(m, (int)tot, xs, hs, hl, threads);
+  long long off = 0;
+  for (int b = 0; b < batch; ++b) {
+    const int8_t* sb = x + (size_t)b * 2 * N;
+    double e = 0.0;
+    for (int i = 0; i < N; ++i) e += (sb[i] > 0 && sb[N + i] > 0) ? U : 0.0;
+    for (int h = 0; h < cnt[b]; ++h)
+      if (hs[off + h] != 0.f) e += coef[off + h] * (double)(hs[off + h] * sign_s[b]) * exp(hl[off + h] - logabs_s[b]);
+    eloc[b] = e;
+    off += cnt[b];
+  }
+  free(xs); free(coef); free(cnt); free(hs); free(hl);
+  return 0;
+}

@@ -1,0 +1,245 @@
+"""HIP path vs the FP64 oracle, through the C ABI (marked gpu).
+
+Tolerances: log|psi| within 1e-4 absolute on the small lattices and 1e-3 at 16x16 (the
+north-star bound for FP32 against FP64); signs, hop lists and configurations bit-exact;
+E_loc within 1e-6 relative + 1e-5 absolute on small lattices.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests._util import cancellation, log_tol
+from neuralhubbard_amd import lattice as L
+from neuralhubbard_amd.configs import WORKLOADS, make_params, make_configurations, small_workload
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible MI355X")
+    from neuralhubbard_amd import _lib
+
+    _lib.load()
+
+
+def rand_skew(n, rng, scale=1.0):
+    A = rng.standard_normal((n, n)) * scale
+    return np.triu(A, 1) - np.triu(A, 1).T
+
+
+def test_slogpf_matches_oracle():
+    from neuralhubbard_amd.backflow import slogpf
+
+    rng = np.random.default_rng(0)
+    for n, B in [(2, 8), (8, 16), (56, 32), (64, 8), (224, 8), (256, 4)]:
+        print(f"slogpf n={n} B={B}", flush=True)
+        A = np.stack([rand_skew(n, rng, 0.3) for _ in range(B)])
+        sg, la = slogpf(torch.tensor(A, dtype=torch.float32, device="cuda"))
+        sg, la = sg.cpu().numpy(), la.cpu().numpy()
+        A32 = A.astype(np.float32).astype(np.float64)
+        for b in range(B):
+            s_ref, l_ref = O.slogpf(A32[b])
+            assert sg[b] == s_ref, (n, b)
+            assert abs(la[b] - l_ref) < 2e-4 * max(1.0, n / 56), (n, b, la[b], l_ref)
+
+
+def test_slogpf_singular_and_signs():
+    from neuralhubbard_amd.backflow import slogpf
+
+    A = np.zeros((3, 6, 6), np.float32)
+    a = np.array([0.5, -2.0, 3.0])
+    for k in range(3):
+        A[1, 2 * k, 2 * k + 1], A[1, 2 * k + 1, 2 * k] = a[k], -a[k]
+    A[2] = A[1][[1, 0, 2, 3, 4, 5]][:, [1, 0, 2, 3, 4, 5]]
+    sg, la = slogpf(torch.tensor(A, device="cuda"))
+    sg, la = sg.cpu().numpy(), la.cpu().numpy()
+    assert sg[0] == 0 and la[0] == -np.inf
+    assert sg[1] == np.sign(a.prod()) and abs(la[1] - np.log(np.abs(a)).sum()) < 1e-6
+    assert sg[2] == -sg[1]
+
+
+def test_slogdet_matches_oracle():
+    from neuralhubbard_amd.backflow import slogdet
+
+    rng = np.random.default_rng(1)
+    for n, B in [(1, 4), (5, 8), (28, 16), (112, 8)]:
+        A = rng.standard_normal((B, n, n)).astype(np.float32)
+        sg, la = slogdet(torch.tensor(A, device="cuda"))
+        for b in range(B):
+            s_ref, l_ref = O.slogdet(A[b].astype(np.float64))
+            assert sg[b].item() == s_ref
+            assert abs(la[b].item() - l_ref) < 1e-4 * max(1, n / 28)
+
+
+@pytest.mark.parametrize("LX,LY,d", [(8, 8, 8), (16, 16, 16), (5, 7, 4)])
+def test_pairing_occ_matches_oracle(LX, LY, d):
+    from neuralhubbard_amd.backflow import build_wtilde, pairing_occ
+
+    rng = np.random.default_rng(2)
+    N = LX * LY
+    W = (rng.standard_normal((3, N, d)) / np.sqrt(d)).astype(np.float32)
+    Wt = build_wtilde(W, 0.55, LX, LY)
+    B, n = 3, min(2 * N, 40)
+    idx = np.stack([np.sort(rng.choice(2 * N, n, replace=False)) for _ in range(B)]).astype(np.int32)
+    Y = rng.standard_normal((B, n, d)).astype(np.float32)
+    F = pairing_occ(torch.tensor(Y, device="cuda"), torch.tensor(idx, device="cuda"),
+                    torch.tensor(Wt, device="cuda"), LX, LY).cpu().numpy()
+    assert np.array_equal(F, -np.transpose(F, (0, 2, 1)))
+    Wt_ref = O.build_wtilde(W, 0.55, LX, LY)
+    for b in range(B):
+        Yfull = np.zeros((2 * N, d))
+        Yfull[idx[b]] = Y[b]
+        ref = O.pairing_occ(Yfull, idx[b], Wt_ref, LX, LY)
+        np.testing.assert_allclose(F[b], ref, rtol=1e-4, atol=1e-6 * np.abs(ref).max())
+
+
+def test_pairing_ud_matches_oracle():
+    from neuralhubbard_amd.backflow import build_wt_ud, pairing_ud
+
+    rng = np.random.default_rng(3)
+    LX = LY = 8
+    N, d, nu, nd = 64, 8, 10, 12
+    W = rng.standard_normal((N, d)).astype(np.float32)
+    Wt = build_wt_ud(W, 0.55, LX, LY)
+    iu = np.sort(rng.choice(N, nu, replace=False)).astype(np.int32)[None]
+    idn = np.sort(rng.choice(N, nd, replace=False)).astype(np.int32)[None]
+    Yu = rng.standard_normal((1, nu, d)).astype(np.float32)
+    Yd = rng.standard_normal((1, nd, d)).astype(np.float32)
+    F = pairing_ud(*(torch.tensor(x, device="cuda") for x in (Yu, Yd, iu, idn, Wt)), LX, LY).cpu().numpy()
+    ref = O.pairing_ud(Yu[0], Yd[0], iu[0], idn[0], O.build_wt_ud(W, 0.55, LX, LY), LX, LY)
+    np.testing.assert_allclose(F[0], ref, rtol=1e-4, atol=1e-6 * np.abs(ref).max())
+
+
+def _model(wk, precision="fp32"):
+    from neuralhubbard_amd.backflow import build_backflow
+
+    p = make_params(wk)
+    return build_backflow(wk, p, precision), O.Model(wk, p)
+
+
+@pytest.mark.parametrize("name", ["small", "pf8"])
+def test_features_match_oracle(name):
+    wk = small_workload(channels=8, n_layers=2) if name == "small" else WORKLOADS["pf8"]
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 4)
+    y = dev.features(torch.tensor(s, device="cuda")).cpu().numpy()
+    for b in range(4):
+        np.testing.assert_allclose(y[b], ref.features(s[b]), rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("kind,symm", [("pfaffian", True), ("pfaffian", False), ("bcs", True), ("bcs", False)])
+def test_log_psi_small_matches_oracle(kind, symm):
+    from neuralhubbard_amd.backflow import Variational
+
+    wk = small_workload(kind=kind, LX=4, LY=4, n_up=3, n_dn=3 if kind == "bcs" else 1, channels=8,
+                        n_layers=2, base_f0=(kind == "pfaffian"), tprime=-0.3)
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 16)
+    psi = Variational(dev, symm=symm)(torch.tensor(s, device="cuda"))
+    sg, la = psi.numpy()
+    for b in range(16):
+        rs, rl = ref.psi(s[b], project=symm)
+        c = cancellation(ref, s[b]) if symm else 0.0
+        if c > 12:  # exactly cancelling symmetric configuration: psi_P = 0 in FP64
+            continue
+        assert sg[b] == rs, b
+        assert abs(la[b] - rl) < log_tol(1e-4, c), (b, la[b], rl, c)
+
+
+def test_log_psi_pf8_matches_oracle():
+    from neuralhubbard_amd.backflow import Variational
+
+    wk = WORKLOADS["pf8"]
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 64)
+    sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
+    for b in range(0, 64, 8):
+        rs, rl = ref.psi(s[b])
+        c = cancellation(ref, s[b])
+        assert sg[b] == rs and abs(la[b] - rl) < log_tol(1e-4, c), (b, la[b], rl, c)
+
+
+def test_log_psi_pf16_matches_oracle():
+    """16x16 Pfaffian backflow (224 x 224 Pfaffians): |dlog psi| < 1e-3 vs FP64, same sign."""
+    from neuralhubbard_amd.backflow import Variational
+
+    wk = WORKLOADS["pf16"]
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 32)
+    sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
+    for b in range(0, 32, 11):
+        rs, rl = ref.psi(s[b])
+        c = cancellation(ref, s[b])
+        print(f"pf16 b={b} gpu={la[b]:.6f} fp64={rl:.6f} cancellation={c:.2f}", flush=True)
+        assert sg[b] == rs and abs(la[b] - rl) < log_tol(1e-3, c), (b, la[b], rl)
+
+
+def test_log_psi_bcs16_matches_oracle():
+    from neuralhubbard_amd.backflow import Variational
+
+    wk = WORKLOADS["bcs16"]
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 16)
+    sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
+    for b in range(0, 16, 7):
+        rs, rl = ref.psi(s[b])
+        c = cancellation(ref, s[b])
+        assert sg[b] == rs and abs(la[b] - rl) < log_tol(1e-3, c), (b, la[b], rl, c)
+
+
+@pytest.mark.parametrize("LX,LY,tp", [(4, 4, -0.3), (8, 8, 0.0), (16, 16, -0.2), (3, 5, 0.4)])
+def test_hops_bit_exact(LX, LY, tp):
+    from neuralhubbard_amd.backflow import hops
+
+    geom = L.Geometry(LX, LY, (LX * LY) // 3, (LX * LY) // 4)
+    s = L.random_configurations(geom, 8, 5)
+    count, fr, to, co = (x.cpu().numpy() for x in hops(torch.tensor(s, device="cuda"), LX, LY, 1.0, tp,
+                                                        geom.n_occ))
+    for b in range(8):
+        ref = O.hops(s[b], LX, LY, 1.0, tp)
+        assert count[b] == len(ref)
+        assert [(int(f), int(t)) for f, t in zip(fr[b, : count[b]], to[b, : count[b]])] == [(a, c) for a, c, _ in ref]
+        np.testing.assert_array_equal(co[b, : count[b]], np.array([h for _, _, h in ref], np.float32))
+
+
+@pytest.mark.parametrize("kind", ["pfaffian", "bcs"])
+def test_local_energy_small_matches_oracle(kind):
+    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational
+
+    wk = small_workload(kind=kind, LX=4, LY=4, n_up=3, n_dn=3 if kind == "bcs" else 1, channels=8,
+                        n_layers=2, base_f0=(kind == "pfaffian"), tprime=-0.3)
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 8)
+    st = Variational(dev)
+    ts = torch.tensor(s, device="cuda")
+    psi = st(ts)
+    e = Hubbard(8.0, 1.0, -0.3, chunk=3).Oloc(st, Samples(ts, psi)).cpu().numpy()
+    for b in range(8):
+        c = cancellation(ref, s[b])
+        if c > 4:  # strongly cancelling psi_P(s): ratios psi(s')/psi(s) are ill-conditioned
+            continue
+        er = O.eloc(ref, s[b], 8.0, 1.0, -0.3)
+        assert abs(e[b] - er) < log_tol(1e-6, c) * abs(er) + 1e-5, (b, e[b], er)
+
+
+def test_local_energy_pf8_matches_oracle():
+    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational
+
+    wk = WORKLOADS["pf8"]
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 4)
+    st = Variational(dev)
+    ts = torch.tensor(s, device="cuda")
+    e = Hubbard(wk.U, 1.0, wk.tprime).Oloc(st, Samples(ts, st(ts))).cpu().numpy()
+    for b in range(2):
+        assert cancellation(ref, s[b]) < 2
+        er = O.eloc(ref, s[b], wk.U, 1.0, wk.tprime)
+        print(f"pf8 eloc b={b} gpu={e[b]:.8f} fp64={er:.8f}", flush=True)
+        assert abs(e[b] - er) < 1e-5 * abs(er) + 1e-4

@@ -1,0 +1,84 @@
+"""CPU-side checks of the C ABI library: it loads, and it exports every entry point that
+include/nh.h declares (no compute call is made without a GPU)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from neuralhubbard_amd import _lib
+from neuralhubbard_amd import backflow as bf
+from neuralhubbard_amd.configs import WORKLOADS, make_params
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "nh.h")).read()
+    return sorted(set(re.findall(r"\b(nh_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libnh.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with include/nh.h"
+
+
+def test_workspace_queries_need_no_gpu():
+    lib = _lib.load()
+    # a model description with dummy (non-null) pointers: size queries never dereference them
+    m = _lib.NhModel(0, 16, 16, 112, 112, 32, 6, 0, 1, 0, 16, 16, 16, 0, 0)
+    ws = lib.nh_log_psi_workspace_size(ctypes.byref(m), 8192)
+    assert ws > 8192 * 8 * 12
+    assert lib.nh_local_energy_workspace_size(ctypes.byref(m), -0.2, 4) > 0
+    bad = _lib.NhModel(0, 2, 16, 112, 112, 32, 6, 0, 1, 0, 16, 16, 16, 0, 0)
+    assert lib.nh_log_psi_workspace_size(ctypes.byref(bad), 8) == 0
+
+
+def test_argument_errors_without_gpu():
+    lib = _lib.load()
+    m = _lib.NhModel(0, 16, 16, 112, 112, 32, 6, 0, 1, 0, 0, 0, 0, 0, 0)  # null weights
+    assert lib.nh_log_psi(None, ctypes.byref(m), 4, None, None, None, None, 0) == _lib.NH_ERR_ARG
+    assert lib.nh_slogpf(None, 4, 7, None, None, None) == _lib.NH_ERR_ARG
+    assert lib.nh_slogpf(None, 1, 300, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16)) == _lib.NH_ERR_SHAPE
+    assert lib.nh_slogdet(None, 1, 500, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16)) == _lib.NH_ERR_SHAPE
+
+
+def test_host_tables_match_oracle():
+    rng = np.random.default_rng(0)
+    for LX, LY in [(8, 8), (16, 16), (5, 3)]:
+        N, d = LX * LY, 4
+        W = rng.standard_normal((3, N, d))
+        np.testing.assert_allclose(bf.build_wtilde(W, 0.55, LX, LY), O.build_wtilde(W, 0.55, LX, LY), rtol=2e-7,
+                                   atol=1e-12)
+        W2 = rng.standard_normal((N, d))
+        np.testing.assert_allclose(bf.build_wt_ud(W2, 0.55, LX, LY), O.build_wt_ud(W2, 0.55, LX, LY), rtol=2e-7,
+                                   atol=1e-12)
+
+
+def test_pack_conv_layout():
+    wk = WORKLOADS["pf8"]
+    p = make_params(wk)
+    w, b = bf.pack_conv(p)
+    C = wk.channels
+    assert w.size == 9 * 2 * C + wk.n_layers * 9 * C * C and b.size == (wk.n_layers + 1) * C
+    # layer 1, tap (dx, dy) = (+1, -1), c_in 3, c_out 5
+    off = 9 * 2 * C + ((2 * 3 + 0) * C + 3) * C + 5
+    assert w[off] == p.conv_w[1][5, 3, 2, 0]
+
+
+def test_gpu_entry_points_refuse_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(_lib.NhError):
+        bf.slogpf(torch.zeros((1, 4, 4)))
