@@ -23,114 +23,408 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ inline void wave_argmax(float& best, int& bi) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    float ob = __shfl_xor(best, off);
-    int oi = __shfl_xor(bi, off);
-    if (ob > best || (ob == best && oi < bi)) {
-      best = ob;
-      bi = oi;
-    }
-  }
+// Wave-wide max / min of a 32-bit value: DPP within rows of 16 lanes, readlane across rows.
+#define NH_DPP(v, ctrl) ((uint32_t)__builtin_amdgcn_update_dpp(0, (int)(v), ctrl, 0xf, 0xf, false))
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, NH_DPP(v, 0xB1));   // quad_perm [1,0,3,2]
+  v = max(v, NH_DPP(v, 0x4E));   // quad_perm [2,3,0,1]
+  v = max(v, NH_DPP(v, 0x124));  // row_ror:4
+  v = max(v, NH_DPP(v, 0x128));  // row_ror:8
+  uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  return max(max(a, b), max(c, d));
+}
+__device__ inline int wave_min_i32(int v) {
+  v = min(v, (int)NH_DPP(v, 0xB1));
+  v = min(v, (int)NH_DPP(v, 0x4E));
+  v = min(v, (int)NH_DPP(v, 0x124));
+  v = min(v, (int)NH_DPP(v, 0x128));
+  int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  return min(min(a, b), min(c, d));
 }
 
-// Parlett-Reid on the packed lower triangle `tri` (n even). Result valid in thread 0.
+// argmax_i |col(i)| over i in [lo, n), smallest index on ties; wave-uniform result (n if empty).
+template <class ColFn>
+__device__ inline int wave_pivot(int lo, int n, ColFn col) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t best = 0;
+  int bi = n;
+  for (int i = lo + lane; i < n; i += kWave) {
+    const uint32_t u = __float_as_uint(fabsf(col(i)));
+    if (bi == n || u > best) {
+      best = u;
+      bi = i;
+    }
+  }
+  const uint32_t m = wave_max_u32(bi < n ? best : 0u);
+  return wave_min_i32((bi < n && best == m) ? bi : n);
+}
+
+// Running product of |pivots| as an FP64 mantissa in [0.5, 1) and an integer exponent.
+struct LogAcc {
+  double mant = 1.0;
+  int expo = 0;
+  __device__ void mul(float x) {
+    int e;
+    mant = frexp(mant * (double)fabsf(x), &e);
+    expo += e;
+  }
+  __device__ double log_value() const { return log(mant) + (double)expo * 0.6931471805599453; }
+};
+
+constexpr int kMaxColChunks = 4;  // n <= 256 = 4 x 64 columns per lane
+constexpr int kRowBatch = 8;      // rows whose loads are in flight together
+
+// Parlett-Reid on the packed lower triangle `tri` (n even, n <= 256). Result valid in thread 0.
+// Parlett-Reid on the packed lower triangle `tri` (n even, n <= 256).  `tri` must be followed
+// by kWave dummy floats, tau / vv must hold n + kWave floats, and `part` 2 * (NT / kWave) words.
+// Addresses of lanes with no element to touch are redirected to the dummy / slack entries,
+// so the step code has no divergent branches (each divergent branch costs an LDS drain).
+// Result valid in thread 0.
+//
+// Step k (pivot column k, a = k + 1), two barrier-separated phases on all waves:
+//   T: every wave reduces the per-wave partial maxima of |A[i][k]| left by the previous
+//      phase U to the same pivot row b; one lane per row i >= k + 2 reads the column pair of
+//      the permuted matrix A' = P A P (P the transposition (a b)) straight from the
+//      unpermuted triangle, writes tau_i = A'[i][k] / A'[a][k], v_i = A'[i][a] and stores the
+//      exchanged entry of row/column b.
+//   U: A[i][j] += tau_i v_j - v_i tau_j for i > j >= k + 2; wave w owns rows i = w (mod NW),
+//      a lane the columns k + 2 + lane + 64 c.  The owner of row i also holds the new A[i][k+2]
+//      and keeps the running argmax of the next pivot column.
 template <int NT>
-__device__ void pf_eliminate(float* tri, int n, float* tau, float* vv, int* flag, float& sign_out,
+__device__ void pf_eliminate(float* tri, int n, float* tau, float* vv, uint32_t* part, float& sign_out,
                              double& log_out) {
   constexpr int NW = NT / kWave;
-  const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
-  double logabs = 0.0;
+  constexpr int RB = 4;  // rows per group in phase U
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int ntri = (n * (n - 1)) / 2;
+  const int dummy = ntri + lane;
+  {  // partial maxima of column 0 over rows >= 1
+    uint32_t best = 0;
+    int bi = n;
+    for (int i = 1 + wave; i < n; i += NW) {
+      const uint32_t u = __float_as_uint(fabsf(tri[toff(i, 0)]));
+      if (bi == n || u > best) {
+        best = u;
+        bi = i;
+      }
+    }
+    if (lane == 0) {
+      part[2 * wave] = best;
+      part[2 * wave + 1] = (uint32_t)bi;
+    }
+  }
+  __syncthreads();
+  LogAcc acc;
   float sign = 1.f;
+  bool singular = false;
   for (int k = 0; k + 1 < n; k += 2) {
-    if (wave == 0) {
-      float best = -1.f;
-      int bi = n;
-      for (int i = k + 1 + lane; i < n; i += kWave) {
-        float v = fabsf(tri[toff(i, k)]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
+    const int a = k + 1;
+    // ---- phase T
+    const uint32_t pv = lane < NW ? part[2 * lane] : 0u;
+    const int pidx = lane < NW ? (int)part[2 * lane + 1] : n;
+    const uint32_t m = wave_max_u32(pidx < n ? pv : 0u);
+    const int b0 = wave_min_i32((pidx < n && pv == m) ? pidx : n);
+    const int bb = b0 < n ? b0 : a;
+    const float piv = tri[toff(bb, k)];  // A'[a][k] = A[b][k]
+    if (!(piv != 0.f) || !isfinite(piv)) {
+      singular = true;
+      break;
+    }
+    if (threadIdx.x == 0) {
+      if (bb != a) sign = -sign;
+      sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
+      acc.mul(piv);
+    }
+    const float rp = 1.0f / piv;
+    for (int t = threadIdx.x; t < n - k - 2; t += NT) {
+      const int i = k + 2 + t;                    // i >= k + 2 > a
+      const int pi = i == bb ? a : i;            // P(i)
+      const int hi = pi > bb ? pi : bb, lo = pi > bb ? bb : pi;
+      const float ck = tri[toff(pi, k)];          // A'[i][k] = A[P i][k]
+      const float cav = tri[toff(hi, lo)];        // A'[i][a] = A[P i][b]
+      const float ra = tri[toff(i, a)];           // old A[i][a]
+      tau[i] = ck * rp;
+      vv[i] = pi > bb ? cav : -cav;
+      const bool live = bb != a && i != bb;       // new A[i][b] = old A[i][a]
+      const int ad = live ? (i > bb ? toff(i, bb) : toff(bb, i)) : dummy;
+      tri[ad] = i > bb ? ra : -ra;
+    }
+    __syncthreads();
+    // ---- phase U
+    float tj[kMaxColChunks], vj[kMaxColChunks];
+#pragma unroll
+    for (int c = 0; c < kMaxColChunks; ++c) {
+      const int j = k + 2 + lane + c * kWave;
+      const int jc = j < n ? j : n + lane;
+      tj[c] = tau[jc];
+      vj[c] = vv[jc];
+    }
+    uint32_t best = 0;
+    int bi = n;
+    for (int i0 = k + 3 + wave; i0 < n; i0 += NW * RB) {
+      float ti[RB], vi[RB];
+      int base[RB], len[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int i = i0 + r * NW;
+        const int ic = i < n ? i : n;  // row n: past the end, len 0 -> dummy only
+        ti[r] = tau[ic < n ? ic : n + lane];
+        vi[r] = vv[ic < n ? ic : n + lane];
+        base[r] = toff(ic < n ? ic : 0, k + 2) + lane;
+        len[r] = ic < n ? ic - (k + 2) : 0;
       }
-      wave_argmax(best, bi);
-      const int a = k + 1, b = bi;
-      if (b != a) {
-        if (lane == 0) {
-          float t0 = tri[toff(a, k)];
-          tri[toff(a, k)] = tri[toff(b, k)];
-          tri[toff(b, k)] = t0;
-          tri[toff(b, a)] = -tri[toff(b, a)];
+      int lmax = len[0];
+#pragma unroll
+      for (int r = 1; r < RB; ++r) lmax = len[r] > lmax ? len[r] : lmax;
+#pragma unroll
+      for (int c = 0; c < kMaxColChunks; ++c) {
+        if (c * kWave >= lmax) break;  // uniform
+        int ad[RB];
+        float x[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          ad[r] = (c * kWave + lane < len[r]) ? base[r] + c * kWave : dummy;
+          x[r] = tri[ad[r]];
         }
-        for (int i = a + 1 + lane; i < b; i += kWave) {
-          float t1 = tri[toff(i, a)], t2 = tri[toff(b, i)];
-          tri[toff(i, a)] = -t2;
-          tri[toff(b, i)] = -t1;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          x[r] = fmaf(ti[r], vj[c], fmaf(-vi[r], tj[c], x[r]));
+          tri[ad[r]] = x[r];
         }
-        for (int i = b + 1 + lane; i < n; i += kWave) {
-          float t1 = tri[toff(i, a)];
-          tri[toff(i, a)] = tri[toff(i, b)];
-          tri[toff(i, b)] = t1;
-        }
-      }
-      wave_sync();
-      const float piv = tri[toff(a, k)];
-      const bool bad = !(piv != 0.f) || !isfinite(piv);
-      if (!bad)
-        for (int i = k + 2 + lane; i < n; i += kWave) {
-          tau[i] = tri[toff(i, k)] / piv;
-          vv[i] = tri[toff(i, a)];
-        }
-      if (lane == 0) {
-        if (b != a) sign = -sign;
-        if (bad) {
-          flag[0] = 1;
-        } else {
-          sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
-          logabs += log((double)fabsf(piv));
+        if (c == 0) {
+          // lane 0 of chunk 0 holds the new A[i][k+2]: next pivot column
+#pragma unroll
+          for (int r = 0; r < RB; ++r) {
+            const int i = i0 + r * NW;
+            const uint32_t u = __float_as_uint(fabsf(x[r]));
+            if (lane == 0 && i < n && (bi == n || u > best)) {
+              best = u;
+              bi = i;
+            }
+          }
         }
       }
     }
-    __syncthreads();
-    if (flag[0]) break;
-    for (int i = k + 3 + wave; i < n; i += NW) {
-      const float ti = tau[i], vi = vv[i];
-      float* row = tri + toff(i, 0);
-      for (int j = k + 2 + lane; j < i; j += kWave) row[j] = fmaf(ti, vv[j], fmaf(-vi, tau[j], row[j]));
+    if (lane == 0) {
+      part[2 * wave] = best;
+      part[2 * wave + 1] = (uint32_t)bi;
     }
     __syncthreads();
   }
-  if (flag[0]) {
+  if (singular) {
     sign_out = 0.f;
     log_out = -INFINITY;
   } else {
     sign_out = sign;
-    log_out = logabs;
+    log_out = acc.log_value();
   }
 }
 
-// LU with partial pivoting on the dense row-major M (leading dimension ld). Thread 0 result.
+typedef float nh_f32x4 __attribute__((ext_vector_type(4)));
+
+// Blocked Parlett-Reid for 64 < n <= 256 (n even) on the packed lower triangle `tri` (followed
+// by kWave dummy floats).  Panels of up to P elimination steps (2P columns):
+//   panel step s (pivot column k, a = k + 1): one thread per row i computes the column pair
+//     it needs from the panel-start triangle plus the panel's earlier rank-2 terms
+//     (left-looking), c_i = A[i][k] + sum_{s'<s} (U_s'[i] W_s'[k] - W_s'[i] U_s'[k]);
+//     every wave reduces the per-wave maxima to the same pivot row b; the column pair of the
+//     permuted matrix A' = P A P (P the transposition (a b)) gives U_s = A'[:, k] / A'[a][k] and
+//     W_s = A'[:, a]; row/column b of the triangle and of U, W take the old row/column a.
+//   after the panel: A[i][j] += sum_s U_s[i] W_s[j] - W_s[i] U_s[j] for i > j >= k0 + 2P, as
+//     16 x 16 tiles of FP32 MFMA products [U | -W] [W | U]^T, each trailing element read and
+//     written once per panel.
+// U and W hold P rows of ldu >= n + 16 floats, cbuf n floats, part 2 * NW words.
+template <int NT, int P>
+__device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float* cbuf, uint32_t* part,
+                           float& sign_out, double& log_out) {
+  constexpr int NW = NT / kWave;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(t / kWave);
+  const int ntri = (n * (n - 1)) / 2;
+  const int dummy = ntri + lane;
+  LogAcc acc;
+  float sign = 1.f;
+  bool singular = false;
+  for (int k0 = 0; k0 < n && !singular; k0 += 2 * P) {
+    const int steps = (n - k0) / 2 < P ? (n - k0) / 2 : P;
+    // this thread's row t of the panel vectors, kept in registers; entries of later steps are 0,
+    // so every sum below runs over all P terms without branches
+    float pu[P], pw[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      pu[q] = 0.f;
+      pw[q] = 0.f;
+    }
+    for (int e = t; e < ldu; e += NT)
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        U[q * ldu + e] = 0.f;
+        W[q * ldu + e] = 0.f;
+      }
+    __syncthreads();
+    for (int s = 0; s < steps; ++s) {
+      const int k = k0 + 2 * s, a = k + 1;
+      // (1) effective column k on rows i > k
+      const bool row = t > k && t < n;
+      float c = tri[toff(row ? t : k + 1, k)];
+      {
+        float uk[P], wk[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          uk[q] = U[q * ldu + k];
+          wk[q] = W[q * ldu + k];
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q) c = fmaf(pu[q], wk[q], fmaf(-pw[q], uk[q], c));
+      }
+      if (row) cbuf[t] = c;
+      {
+        const uint32_t u = row ? __float_as_uint(fabsf(c)) : 0u;
+        const uint32_t m = wave_max_u32(u);
+        const int bi = wave_min_i32((row && u == m) ? t : n);
+        if (lane == 0) {
+          part[2 * wave] = m;
+          part[2 * wave + 1] = (uint32_t)bi;
+        }
+      }
+      __syncthreads();
+      // (2) pivot row b (identical in every wave)
+      const uint32_t pv = lane < NW ? part[2 * lane] : 0u;
+      const int pidx = lane < NW ? (int)part[2 * lane + 1] : n;
+      const uint32_t m = wave_max_u32(pidx < n ? pv : 0u);
+      const int b0 = wave_min_i32((pidx < n && pv == m) ? pidx : n);
+      const int b = b0 < n ? b0 : a;
+      const float piv = cbuf[b];  // A'[a][k] = A[b][k]
+      if (!(piv != 0.f) || !isfinite(piv)) {
+        singular = true;
+        break;
+      }
+      if (t == 0) {
+        if (b != a) sign = -sign;
+        sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
+        acc.mul(piv);
+      }
+      const float rp = 1.0f / piv;
+      // (3) new panel column on rows i >= k + 2; row b takes row a's earlier entries
+      const bool live = t >= k + 2 && t < n;
+      const bool isb = t == b && b != a;
+      const int pi = live ? (t == b ? a : t) : a + 1;  // P(i) (a + 1: in-bounds stand-in)
+      const int hi = pi > b ? pi : b, lo = pi > b ? b : pi;
+      const float raw = tri[(live && hi != lo) ? toff(hi, lo) : dummy];
+      const float ra = tri[live ? toff(t, a) : dummy];  // old A[i][a]
+      const float cp = cbuf[live ? pi : k + 1];
+      float v = pi > b ? raw : -raw;  // A[P i][b] from the triangle
+      {
+        float ub[P], wb[P], ua[P], wa[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          ub[q] = U[q * ldu + b];
+          wb[q] = W[q * ldu + b];
+          ua[q] = U[q * ldu + a];
+          wa[q] = W[q * ldu + a];
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          const float mu = isb ? ua[q] : pu[q];
+          const float mw = isb ? wa[q] : pw[q];
+          v = fmaf(mu, wb[q], fmaf(-mw, ub[q], v));
+          pu[q] = mu;
+          pw[q] = mw;
+        }
+      }
+      const float tau = cp * rp;
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (q == s) {  // uniform
+          pu[q] = live ? tau : pu[q];
+          pw[q] = live ? v : pw[q];
+        }
+      }
+      __syncthreads();
+      if (live) {
+        if (isb) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            U[q * ldu + t] = pu[q];
+            W[q * ldu + t] = pw[q];
+          }
+        } else {
+          U[s * ldu + t] = tau;
+          W[s * ldu + t] = v;
+        }
+        if (b != a && t != b) {  // new A[i][b] = old A[i][a]
+          if (t > b) tri[toff(t, b)] = ra;
+          else tri[toff(b, t)] = -ra;
+        }
+      }
+      __syncthreads();
+    }
+    if (singular) break;
+    // trailing rank-2P update of rows/columns >= K
+    const int K = k0 + 2 * steps;
+    const int mtr = n - K;
+    if (mtr <= 1) continue;
+    const int nt = (mtr + 15) / 16;
+    const int ntiles = nt * (nt + 1) / 2;
+    for (int tile = wave; tile < ntiles; tile += NW) {
+      // tile -> (ti, tj), tj <= ti, row-major over the lower triangle of tiles
+      int ti = (int)((sqrtf(8.f * tile + 1.f) - 1.f) * 0.5f);
+      while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+      while (ti * (ti + 1) / 2 > tile) --ti;
+      const int tj = tile - ti * (ti + 1) / 2;
+      const int i0 = K + 16 * ti, j0 = K + 16 * tj;
+      const int col = j0 + (lane & 15);
+      int ad[4];
+      nh_f32x4 cacc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + 4 * (lane >> 4) + r;
+        ad[r] = (i < n && col < i) ? toff(i, col) : dummy;
+        cacc[r] = tri[ad[r]];
+      }
+      const int ia = i0 + (lane & 15);  // A operand row
+      const bool iok = ia < n, jok = col < n;
+#pragma unroll
+      for (int kk = 0; kk < (2 * P) / 4; ++kk) {
+        const int q = kk * 4 + (lane >> 4);  // 0 .. 2P-1
+        // q < P: U_q[i] W_q[j];  q >= P: -W_q[i] U_q[j]  (entries of unused steps are 0)
+        const float* L = q < P ? U + q * ldu : W + (q - P) * ldu;
+        const float* R = q < P ? W + q * ldu : U + (q - P) * ldu;
+        const float av = iok ? (q < P ? L[ia] : -L[ia]) : 0.f;
+        const float bv = jok ? R[col] : 0.f;
+        cacc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cacc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tri[ad[r]] = cacc[r];
+    }
+    __syncthreads();
+  }
+  if (singular) {
+    sign_out = 0.f;
+    log_out = -INFINITY;
+  } else {
+    sign_out = sign;
+    log_out = acc.log_value();
+  }
+}
+
+// LU with partial pivoting on the dense row-major M (leading dimension ld, n <= 256); M must be
+// followed by kWave dummy floats at n * ld.
 template <int NT>
 __device__ void lu_eliminate(float* M, int n, int ld, float* lcol, float* urow, int* flag,
                              float& sign_out, double& log_out) {
   constexpr int NW = NT / kWave;
   const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
-  double logabs = 0.0;
+  LogAcc acc;
   float sign = 1.f;
   for (int k = 0; k < n; ++k) {
     if (wave == 0) {
-      float best = -1.f;
-      int bi = n;
-      for (int i = k + lane; i < n; i += kWave) {
-        float v = fabsf(M[i * ld + k]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      wave_argmax(best, bi);
-      if (bi != k)
+      const int bi = wave_pivot(k, n, [&](int i) { return M[i * ld + k]; });
+      if (bi != k && bi < n)
         for (int j = k + lane; j < n; j += kWave) {
           float t1 = M[k * ld + j];
           M[k * ld + j] = M[bi * ld + j];
@@ -139,8 +433,9 @@ __device__ void lu_eliminate(float* M, int n, int ld, float* lcol, float* urow, 
       wave_sync();
       const float piv = M[k * ld + k];
       const bool bad = !(piv != 0.f) || !isfinite(piv);
+      const float rp = 1.0f / piv;
       if (!bad) {
-        for (int i = k + 1 + lane; i < n; i += kWave) lcol[i] = M[i * ld + k] / piv;
+        for (int i = k + 1 + lane; i < n; i += kWave) lcol[i] = M[i * ld + k] * rp;
         for (int j = k + 1 + lane; j < n; j += kWave) urow[j] = M[k * ld + j];
       }
       if (lane == 0) {
@@ -149,16 +444,37 @@ __device__ void lu_eliminate(float* M, int n, int ld, float* lcol, float* urow, 
           flag[0] = 1;
         } else {
           sign = (piv < 0.f) ? -sign : sign;
-          logabs += log((double)fabsf(piv));
+          acc.mul(piv);
         }
       }
     }
     __syncthreads();
     if (flag[0]) break;
-    for (int i = k + 1 + wave; i < n; i += NW) {
-      const float li = lcol[i];
-      float* row = M + i * ld;
-      for (int j = k + 1 + lane; j < n; j += kWave) row[j] = fmaf(-li, urow[j], row[j]);
+    float uj[kMaxColChunks];
+#pragma unroll
+    for (int c = 0; c < kMaxColChunks; ++c) {
+      const int j = k + 1 + lane + c * kWave;
+      uj[c] = (j < n) ? urow[j] : 0.f;
+    }
+    const int dummy = n * ld + lane;  // branch-free stores, as in pf_eliminate
+    for (int i0 = k + 1 + wave; i0 < n; i0 += NW * kRowBatch) {
+      float x[kRowBatch][kMaxColChunks], li[kRowBatch];
+      int ad[kRowBatch][kMaxColChunks];
+#pragma unroll
+      for (int r = 0; r < kRowBatch; ++r) {
+        const int i = i0 + r * NW;
+        li[r] = lcol[i < n ? i : n - 1];
+#pragma unroll
+        for (int c = 0; c < kMaxColChunks; ++c) {
+          const int j = k + 1 + lane + c * kWave;
+          ad[r][c] = (i < n && j < n) ? i * ld + j : dummy;
+          x[r][c] = M[ad[r][c]];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kRowBatch; ++r)
+#pragma unroll
+        for (int c = 0; c < kMaxColChunks; ++c) M[ad[r][c]] = fmaf(-li[r], uj[c], x[r][c]);
     }
     __syncthreads();
   }
@@ -167,7 +483,7 @@ __device__ void lu_eliminate(float* M, int n, int ld, float* lcol, float* urow, 
     log_out = -INFINITY;
   } else {
     sign_out = sign;
-    log_out = logabs;
+    log_out = acc.log_value();
   }
 }
 
@@ -192,8 +508,19 @@ __device__ inline float parity_sign(long long k) { return (k & 1) ? -1.f : 1.f; 
 
 // ------------------------------------------------------------------ Pfaffian image kernel
 
+constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffian
+
+// floats of the region used first by the pairing build (features of the occupied modes, one
+// Wt spin block) and then by the blocked elimination (U, W, column buffer)
+__host__ __device__ inline int pf_scratch_floats(int n, int N, int D) {
+  const int build = n * (D + 4) + N * D;
+  const int elim = 2 * kPanel * (n + 16) + n;
+  return ((build > elim ? build : elim) + 3) & ~3;
+}
+
 template <int D, int NT>
-__global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, const int8_t* __restrict__ s,
+__global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, int n_up,
+                                                 const int8_t* __restrict__ s,
                                                  const int32_t* __restrict__ item_b,
                                                  const int32_t* __restrict__ item_hop, int img0,
                                                  int img_end, int ipi, const float* __restrict__ Y,
@@ -202,6 +529,7 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, c
                                                  float* __restrict__ img_sign,
                                                  double* __restrict__ img_logabs) {
   extern __shared__ float lds[];
+  constexpr int YS = D + 4;  // padded row stride of the occupied features
   const int img = img0 + blockIdx.x;
   if (img >= img_end) return;
   const int item = img / ipi;
@@ -213,11 +541,15 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, c
 
   const int ntri = (n * (n - 1)) / 2;
   float* tri = lds;
-  float* yocc = tri + ((ntri + 3) & ~3);
-  float* tau = yocc + n * D;
-  float* vv = tau + n;
-  int* modes = reinterpret_cast<int*>(vv + n);
-  int* flag = modes + n;
+  float* scr = tri + ((ntri + kWave + 3) & ~3);  // pairing build, then blocked elimination
+  float* yocc = scr;
+  float* wtl = yocc + n * YS;
+  float* tau = scr + pf_scratch_floats(n, N, D);
+  float* vv = tau + n + kWave;
+  int* modes = reinterpret_cast<int*>(vv + n + kWave);
+  int* coord = modes + n;  // (x << 16) | y of each mode's site
+  int* flag = coord + n;
+  uint32_t* part = reinterpret_cast<uint32_t*>(flag + 4);
 
   const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
   if (wave == 0) {
@@ -235,45 +567,80 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, c
   const float* Yi = Y + (size_t)(img - img0) * (two_n * D);
   for (int e = t; e < n * (D / 4); e += NT) {
     const int p = e / (D / 4), a4 = e - p * (D / 4);
-    reinterpret_cast<float4*>(yocc)[e] = reinterpret_cast<const float4*>(Yi + (size_t)modes[p] * D)[a4];
+    *reinterpret_cast<float4*>(yocc + p * YS + 4 * a4) =
+        reinterpret_cast<const float4*>(Yi + (size_t)modes[p] * D)[a4];
   }
-  __syncthreads();
+  for (int p = t; p < n; p += NT) {
+    const int m = modes[p];
+    const int site = m - (m >= N ? N : 0);
+    coord[p] = ((site / LY) << 16) | (site % LY);
+  }
 
+  // Three regions of the lower triangle, each with one spin block of Wt staged in LDS:
+  // the first n_up modes carry spin s0 (flipped by SpinInverse), the rest spin s1.
+  const int s0 = (g >> 2) & 1, s1 = s0 ^ 1;
   constexpr int NW = NT / kWave;
-  for (int p = 1 + wave; p < n; p += NW) {
-    float yp[D];
+  for (int region = 0; region < 3; ++region) {
+    const int blk = region == 0 ? 3 * s0 : (region == 1 ? 2 * s1 + s0 : 3 * s1);
+    const int p0 = region == 0 ? 1 : (region == 1 ? n_up : n_up + 1);
+    const int p1 = region == 0 ? n_up : n;
+    const int q0 = region == 2 ? n_up : 0;
+    __syncthreads();  // previous region done with wtl
+    const float4* src = reinterpret_cast<const float4*>(wt + (size_t)blk * N * D);
+    for (int e = t; e < N * D / 4; e += NT) reinterpret_cast<float4*>(wtl)[e] = src[e];
+    __syncthreads();
+    for (int p = p0 + wave; p < p1; p += NW) {
+      float yp[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) yp[a] = yocc[p * D + a];
-    const int mp = modes[p];
-    const int sp = mp >= N;
-    const int sitep = mp - sp * N;
-    float* row = tri + toff(p, 0);
-    for (int q = lane; q < p; q += kWave) {
-      const int mq = modes[q];
-      const int sq = mq >= N;
-      const int r = disp_index(sitep, mq - sq * N, LX, LY);
-      const float4* w4 = reinterpret_cast<const float4*>(wt + (size_t)((2 * sp + sq) * N + r) * D);
-      const float4* y4 = reinterpret_cast<const float4*>(yocc + q * D);
-      float acc = 0.f;
+      for (int a = 0; a < D; ++a) yp[a] = yocc[p * YS + a];
+      const int mp = modes[p];
+      const int sxp = coord[p] >> 16, syp = coord[p] & 0xffff;
+      float* row = tri + toff(p, 0);
+      const int qend = region == 1 ? n_up : p;
+      // all column chunks of the row are computed before the first store (loads overlap)
+      float accs[kMaxColChunks];
 #pragma unroll
-      for (int a4 = 0; a4 < D / 4; ++a4) {
-        const float4 w = w4[a4];
-        const float4 yq = y4[a4];
-        acc = fmaf(yp[4 * a4] * yq.x, w.x, acc);
-        acc = fmaf(yp[4 * a4 + 1] * yq.y, w.y, acc);
-        acc = fmaf(yp[4 * a4 + 2] * yq.z, w.z, acc);
-        acc = fmaf(yp[4 * a4 + 3] * yq.w, w.w, acc);
+      for (int c = 0; c < kMaxColChunks; ++c) {
+        const int q = q0 + lane + c * kWave;
+        float acc = 0.f;
+        if (q < qend) {
+          const int mq = modes[q];
+          int dx = sxp - (coord[q] >> 16), dy = syp - (coord[q] & 0xffff);
+          dx += dx < 0 ? LX : 0;
+          dy += dy < 0 ? LY : 0;
+          const int r = dx * LY + dy;
+          const float4* w4 = reinterpret_cast<const float4*>(wtl + r * D);
+          const float4* y4 = reinterpret_cast<const float4*>(yocc + q * YS);
+#pragma unroll
+          for (int a4 = 0; a4 < D / 4; ++a4) {
+            const float4 w = w4[a4];
+            const float4 yq = y4[a4];
+            acc = fmaf(yp[4 * a4] * yq.x, w.x, acc);
+            acc = fmaf(yp[4 * a4 + 1] * yq.y, w.y, acc);
+            acc = fmaf(yp[4 * a4 + 2] * yq.z, w.z, acc);
+            acc = fmaf(yp[4 * a4 + 3] * yq.w, w.w, acc);
+          }
+          if (F0) acc += F0[(size_t)mp * two_n + mq];
+        }
+        accs[c] = acc;
       }
-      if (F0) acc += F0[(size_t)mp * two_n + mq];
-      row[q] = acc;
+#pragma unroll
+      for (int c = 0; c < kMaxColChunks; ++c) {
+        const int q = q0 + lane + c * kWave;
+        if (q < qend) row[q] = accs[c];
+      }
     }
   }
-  if (t == 0) flag[0] = 0;
   __syncthreads();
 
   float sg;
   double la;
-  pf_eliminate<NT>(tri, n, tau, vv, flag, sg, la);
+  if (NT > kWave) {
+    const int ldu = n + 16;
+    pf_blocked<NT, kPanel>(tri, n, scr, scr + kPanel * ldu, ldu, scr + 2 * kPanel * ldu, part, sg, la);
+  } else {
+    pf_eliminate<NT>(tri, n, tau, vv, part, sg, la);
+  }
   if (t == 0) {
     // fermion_inverse_sign: reversal of the Ne creation operators
     sg *= parity_sign((long long)n * (n - 1) / 2);
@@ -308,7 +675,7 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   const int ld = n + 1;
 
   float* M = lds;
-  float* yu = M + ((n * ld + 3) & ~3);
+  float* yu = M + ((n * ld + kWave + 3) & ~3);
   float* yd = yu + n * D;
   float* lcol = yd + n * D;
   float* urow = lcol + n;
@@ -378,13 +745,14 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   }
 }
 
-static size_t pf_lds_bytes(int n, int D) {
+static size_t pf_lds_bytes(int n, int N, int D) {
   const int ntri = (n * (n - 1)) / 2;
-  return sizeof(float) * (size_t)(((ntri + 3) & ~3) + n * D + 2 * n) + sizeof(int) * (size_t)(n + 4);
+  return sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + pf_scratch_floats(n, N, D) + 2 * (n + kWave)) +
+         sizeof(int) * (size_t)(2 * n + 4 + 2 * 16);
 }
 
 static size_t bcs_lds_bytes(int n, int ne, int D) {
-  return sizeof(float) * (size_t)(((n * (n + 1) + 3) & ~3) + 2 * n * D + 2 * n) + sizeof(int) * (size_t)(ne + 4);
+  return sizeof(float) * (size_t)(((n * (n + 1) + kWave + 3) & ~3) + 2 * n * D + 2 * n) + sizeof(int) * (size_t)(ne + 4);
 }
 
 constexpr size_t kMaxLds = 160 * 1024;
@@ -398,13 +766,13 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
   if (nimg <= 0) return NH_OK;
   if (m->kind == NH_KIND_PFAFFIAN) {
     const int n = m->n_up + m->n_dn;
-    const size_t lds = pf_lds_bytes(n, D);
+    const size_t lds = pf_lds_bytes(n, N, D);
     if (lds > kMaxLds) return NH_ERR_SHAPE;
     if (n <= 64) {
-      hipLaunchKernelGGL((k_image_pf<D, 64>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, n, s,
+      hipLaunchKernelGGL((k_image_pf<D, 64>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, n, m->n_up, s,
                          item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs);
     } else {
-      hipLaunchKernelGGL((k_image_pf<D, 512>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, n, s,
+      hipLaunchKernelGGL((k_image_pf<D, 512>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, n, m->n_up, s,
                          item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs);
     }
   } else {
@@ -416,7 +784,7 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
                          m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
                          img_logabs);
     } else {
-      hipLaunchKernelGGL((k_image_bcs<D, 512>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, m->n_up,
+      hipLaunchKernelGGL((k_image_bcs<D, 1024>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, m->n_up,
                          m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
                          img_logabs);
     }
@@ -444,17 +812,22 @@ __global__ void __launch_bounds__(NT) k_slogpf(int n, const float* __restrict__ 
   extern __shared__ float lds[];
   const int ntri = (n * (n - 1)) / 2;
   float* tri = lds;
-  float* tau = tri + ((ntri + 3) & ~3);
-  float* vv = tau + n;
-  int* flag = reinterpret_cast<int*>(vv + n);
+  float* tau = tri + ((ntri + kWave + 3) & ~3);
+  float* vv = tau + n + kWave;
+  uint32_t* part = reinterpret_cast<uint32_t*>(vv + n + kWave);
+  float* scr = reinterpret_cast<float*>(part + 2 * 16);
   const float* Ab = A + (size_t)blockIdx.x * n * n;
   for (int i = 1 + threadIdx.x / kWave; i < n; i += NT / kWave)
     for (int j = threadIdx.x & (kWave - 1); j < i; j += kWave) tri[toff(i, j)] = Ab[(size_t)i * n + j];
-  if (threadIdx.x == 0) flag[0] = 0;
   __syncthreads();
   float sg;
   double la;
-  pf_eliminate<NT>(tri, n, tau, vv, flag, sg, la);
+  if (NT > kWave) {
+    const int ldu = n + 16;
+    pf_blocked<NT, kPanel>(tri, n, scr, scr + kPanel * ldu, ldu, scr + 2 * kPanel * ldu, part, sg, la);
+  } else {
+    pf_eliminate<NT>(tri, n, tau, vv, part, sg, la);
+  }
   if (threadIdx.x == 0) {
     sign[blockIdx.x] = sg;
     logabs[blockIdx.x] = la;
@@ -467,7 +840,7 @@ __global__ void __launch_bounds__(NT) k_slogdet(int n, const float* __restrict__
   extern __shared__ float lds[];
   const int ld = n + 1;
   float* M = lds;
-  float* lcol = M + ((n * ld + 3) & ~3);
+  float* lcol = M + ((n * ld + kWave + 3) & ~3);
   float* urow = lcol + n;
   int* flag = reinterpret_cast<int*>(urow + n);
   const float* Ab = A + (size_t)blockIdx.x * n * n;
@@ -538,11 +911,12 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) return NH_ERR_SHAPE;
   const int ntri = (n * (n - 1)) / 2;
-  const size_t lds = sizeof(float) * (size_t)(((ntri + 3) & ~3) + 2 * n) + 16;
+  const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + 2 * (n + kWave) + 2 * kPanel * (n + 16) + n) +
+                     4 * 2 * 16;
   if (n <= 64)
     hipLaunchKernelGGL(k_slogpf<64>, dim3(batch), dim3(64), lds, st, n, A, sign, logabs);
   else
-    hipLaunchKernelGGL(k_slogpf<512>, dim3(batch), dim3(512), lds, st, n, A, sign, logabs);
+    hipLaunchKernelGGL(k_slogpf<1024>, dim3(batch), dim3(1024), lds, st, n, A, sign, logabs);
   NH_LAUNCH_CHECK();
   return NH_OK;
 }
@@ -553,7 +927,7 @@ extern "C" int nh_slogdet(void* stream, int32_t batch, int32_t n, const float* A
   if (n > 192) return NH_ERR_SHAPE;
   if (batch == 0) return NH_OK;
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = sizeof(float) * (size_t)(((n * (n + 1) + 3) & ~3) + 2 * n) + 16;
+  const size_t lds = sizeof(float) * (size_t)(((n * (n + 1) + kWave + 3) & ~3) + 2 * n) + 16;
   if (n <= 64)
     hipLaunchKernelGGL(k_slogdet<64>, dim3(batch), dim3(64), lds, st, n, A, sign, logabs);
   else
