@@ -227,16 +227,18 @@ typedef float nh_f32x4 __attribute__((ext_vector_type(4)));
 
 // Blocked Parlett-Reid for 64 < n <= 256 (n even) on the packed lower triangle `tri` (followed
 // by kWave dummy floats).  Panels of up to P elimination steps (2P columns):
-//   panel step s (pivot column k, a = k + 1): one thread per row i computes the column pair
-//     it needs from the panel-start triangle plus the panel's earlier rank-2 terms
-//     (left-looking), c_i = A[i][k] + sum_{s'<s} (U_s'[i] W_s'[k] - W_s'[i] U_s'[k]);
-//     every wave reduces the per-wave maxima to the same pivot row b; the column pair of the
-//     permuted matrix A' = P A P (P the transposition (a b)) gives U_s = A'[:, k] / A'[a][k] and
+//   panel step s (pivot column k, a = k + 1): one thread per row i (waves with no row of the
+//     step skip it: every instruction a wave issues costs its SIMD four cycles) forms the
+//     effective column from the panel-start triangle plus the panel's earlier rank-2 terms
+//     (left-looking), c_i = A[i][k] + sum_{s'<s} (U_s'[i] W_s'[k] - W_s'[i] U_s'[k]); the
+//     pivot row b is the argmax over per-wave DPP maxima; the column pair of the permuted
+//     matrix A' = P A P (P the transposition (a b)) gives U_s = A'[:, k] / A'[a][k] and
 //     W_s = A'[:, a]; row/column b of the triangle and of U, W take the old row/column a.
-//   after the panel: A[i][j] += sum_s U_s[i] W_s[j] - W_s[i] U_s[j] for i > j >= k0 + 2P, as
-//     16 x 16 tiles of FP32 MFMA products [U | -W] [W | U]^T, each trailing element read and
-//     written once per panel.
-// U and W hold P rows of ldu >= n + 16 floats, cbuf n floats, part 2 * NW words.
+//   trailing: A[i][j] += sum_s U_s[i] W_s[j] - W_s[i] U_s[j] for i > j >= k0 + 2P, as 16 x 16
+//     tiles of FP32 MFMA products [U | -W] [W | U]^T over all waves, each trailing element read
+//     and written once per panel.
+// U and W hold P rows of ldu >= n + 16 floats (zeroed per panel, so rows of later steps are 0
+// and every sum runs over all P terms), cbuf n + 1 floats, part 2 * NW + 2 words.
 template <int NT, int P>
 __device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float* cbuf, uint32_t* part,
                            float& sign_out, double& log_out) {
@@ -246,32 +248,32 @@ __device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float
   const int wave = __builtin_amdgcn_readfirstlane(t / kWave);
   const int ntri = (n * (n - 1)) / 2;
   const int dummy = ntri + lane;
+  uint32_t* flag = part + 2 * NW;
   LogAcc acc;
   float sign = 1.f;
   bool singular = false;
-  for (int k0 = 0; k0 < n && !singular; k0 += 2 * P) {
+  for (int k0 = 0; k0 < n; k0 += 2 * P) {
     const int steps = (n - k0) / 2 < P ? (n - k0) / 2 : P;
-    // this thread's row t of the panel vectors, kept in registers; entries of later steps are 0,
-    // so every sum below runs over all P terms without branches
-    float pu[P], pw[P];
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-      pu[q] = 0.f;
-      pw[q] = 0.f;
-    }
     for (int e = t; e < ldu; e += NT)
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         U[q * ldu + e] = 0.f;
         W[q * ldu + e] = 0.f;
       }
+    float pu[P], pw[P];  // this thread's row of the panel vectors
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      pu[q] = 0.f;
+      pw[q] = 0.f;
+    }
     __syncthreads();
     for (int s = 0; s < steps; ++s) {
       const int k = k0 + 2 * s, a = k + 1;
-      // (1) effective column k on rows i > k
-      const bool row = t > k && t < n;
-      float c = tri[toff(row ? t : k + 1, k)];
-      {
+      // (1) effective column k on rows i > k, per-wave argmax
+      const bool wrow = wave * kWave + kWave - 1 > k && wave * kWave < n;  // uniform
+      if (wrow) {
+        const bool row = t > k && t < n;
+        float c = tri[row ? toff(t, k) : dummy];
         float uk[P], wk[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) {
@@ -280,9 +282,7 @@ __device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float
         }
 #pragma unroll
         for (int q = 0; q < P; ++q) c = fmaf(pu[q], wk[q], fmaf(-pw[q], uk[q], c));
-      }
-      if (row) cbuf[t] = c;
-      {
+        cbuf[row ? t : n] = c;
         const uint32_t u = row ? __float_as_uint(fabsf(c)) : 0u;
         const uint32_t m = wave_max_u32(u);
         const int bi = wave_min_i32((row && u == m) ? t : n);
@@ -290,76 +290,86 @@ __device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float
           part[2 * wave] = m;
           part[2 * wave + 1] = (uint32_t)bi;
         }
+      } else if (lane == 0) {
+        part[2 * wave] = 0u;
+        part[2 * wave + 1] = (uint32_t)n;
       }
       __syncthreads();
-      // (2) pivot row b (identical in every wave)
-      const uint32_t pv = lane < NW ? part[2 * lane] : 0u;
-      const int pidx = lane < NW ? (int)part[2 * lane + 1] : n;
-      const uint32_t m = wave_max_u32(pidx < n ? pv : 0u);
-      const int b0 = wave_min_i32((pidx < n && pv == m) ? pidx : n);
-      const int b = b0 < n ? b0 : a;
-      const float piv = cbuf[b];  // A'[a][k] = A[b][k]
-      if (!(piv != 0.f) || !isfinite(piv)) {
-        singular = true;
-        break;
-      }
-      if (t == 0) {
-        if (b != a) sign = -sign;
-        sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
-        acc.mul(piv);
-      }
-      const float rp = 1.0f / piv;
-      // (3) new panel column on rows i >= k + 2; row b takes row a's earlier entries
-      const bool live = t >= k + 2 && t < n;
-      const bool isb = t == b && b != a;
-      const int pi = live ? (t == b ? a : t) : a + 1;  // P(i) (a + 1: in-bounds stand-in)
-      const int hi = pi > b ? pi : b, lo = pi > b ? b : pi;
-      const float raw = tri[(live && hi != lo) ? toff(hi, lo) : dummy];
-      const float ra = tri[live ? toff(t, a) : dummy];  // old A[i][a]
-      const float cp = cbuf[live ? pi : k + 1];
-      float v = pi > b ? raw : -raw;  // A[P i][b] from the triangle
-      {
-        float ub[P], wb[P], ua[P], wa[P];
+      // (2)-(3) pivot and new panel column on rows i >= k + 2
+      const bool wlive = (wave * kWave + kWave - 1 >= k + 2 && wave * kWave < n) || wave == 0;  // uniform
+      float tau = 0.f, v = 0.f, ra = 0.f;
+      int b = a;
+      if (wlive) {
+        const uint32_t pv = lane < NW ? part[2 * lane] : 0u;
+        const int pidx = lane < NW ? (int)part[2 * lane + 1] : n;
+        const uint32_t m = wave_max_u32(pidx < n ? pv : 0u);
+        const int b0 = wave_min_i32((pidx < n && pv == m) ? pidx : n);
+        b = b0 < n ? b0 : a;
+        const float piv = cbuf[b];  // A'[a][k] = A[b][k]
+        const bool bad = !(piv != 0.f) || !isfinite(piv);
+        if (t == 0) {
+          flag[0] = bad ? 1u : 0u;
+          if (!bad) {
+            if (b != a) sign = -sign;
+            sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
+            acc.mul(piv);
+          }
+        }
+        const float rp = 1.0f / piv;
+        const bool live = t >= k + 2 && t < n;
+        const bool isb = t == b && b != a;
+        const int pi = live ? (isb ? a : t) : a + 1;  // P(i); a + 1: in-bounds stand-in
+        const int hi = pi > b ? pi : b, lo = pi > b ? b : pi;
+        const float raw = tri[(live && hi != lo) ? toff(hi, lo) : dummy];
+        ra = tri[live ? toff(t, a) : dummy];  // old A[i][a]
+        const float cp = cbuf[live ? pi : k + 1];
+        float ub[P], wb[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) {
           ub[q] = U[q * ldu + b];
           wb[q] = W[q * ldu + b];
-          ua[q] = U[q * ldu + a];
-          wa[q] = W[q * ldu + a];
         }
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-          const float mu = isb ? ua[q] : pu[q];
-          const float mw = isb ? wa[q] : pw[q];
-          v = fmaf(mu, wb[q], fmaf(-mw, ub[q], v));
-          pu[q] = mu;
-          pw[q] = mw;
-        }
-      }
-      const float tau = cp * rp;
-#pragma unroll
-      for (int q = 0; q < P; ++q) {
-        if (q == s) {  // uniform
-          pu[q] = live ? tau : pu[q];
-          pw[q] = live ? v : pw[q];
-        }
-      }
-      __syncthreads();
-      if (live) {
-        if (isb) {
+        if (isb) {  // the pivot row takes row a's earlier entries
 #pragma unroll
           for (int q = 0; q < P; ++q) {
-            U[q * ldu + t] = pu[q];
-            W[q * ldu + t] = pw[q];
+            pu[q] = U[q * ldu + a];
+            pw[q] = W[q * ldu + a];
           }
-        } else {
+        }
+        v = pi > b ? raw : -raw;  // A[P i][b] from the triangle
+#pragma unroll
+        for (int q = 0; q < P; ++q) v = fmaf(pu[q], wb[q], fmaf(-pw[q], ub[q], v));
+        tau = cp * rp;
+      }
+      __syncthreads();
+      if (flag[0]) {
+        singular = true;
+        break;
+      }
+      if (wlive) {
+        const bool live = t >= k + 2 && t < n;
+        const bool isb = t == b && b != a;
+        if (live) {
+          if (isb) {
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+              if (q < s) {  // uniform
+                U[q * ldu + t] = pu[q];
+                W[q * ldu + t] = pw[q];
+              }
+          }
           U[s * ldu + t] = tau;
           W[s * ldu + t] = v;
         }
-        if (b != a && t != b) {  // new A[i][b] = old A[i][a]
-          if (t > b) tri[toff(t, b)] = ra;
-          else tri[toff(b, t)] = -ra;
-        }
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          if (q == s) {  // uniform
+            pu[q] = live ? tau : pu[q];
+            pw[q] = live ? v : pw[q];
+          }
+        // new A[i][b] = old A[i][a]
+        const bool ex = live && b != a && t != b;
+        tri[ex ? (t > b ? toff(t, b) : toff(b, t)) : dummy] = t > b ? ra : -ra;
       }
       __syncthreads();
     }
@@ -514,7 +524,7 @@ constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffia
 // Wt spin block) and then by the blocked elimination (U, W, column buffer)
 __host__ __device__ inline int pf_scratch_floats(int n, int N, int D) {
   const int build = n * (D + 4) + N * D;
-  const int elim = 2 * kPanel * (n + 16) + n;
+  const int elim = 2 * kPanel * (n + 16) + n + kWave;
   return ((build > elim ? build : elim) + 3) & ~3;
 }
 
@@ -549,7 +559,7 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
   int* modes = reinterpret_cast<int*>(vv + n + kWave);
   int* coord = modes + n;  // (x << 16) | y of each mode's site
   int* flag = coord + n;
-  uint32_t* part = reinterpret_cast<uint32_t*>(flag + 4);
+  uint32_t* part = reinterpret_cast<uint32_t*>(flag + 4);  // 2 * 16 + 2 words
 
   const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
   if (wave == 0) {
@@ -748,7 +758,7 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
 static size_t pf_lds_bytes(int n, int N, int D) {
   const int ntri = (n * (n - 1)) / 2;
   return sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + pf_scratch_floats(n, N, D) + 2 * (n + kWave)) +
-         sizeof(int) * (size_t)(2 * n + 4 + 2 * 16);
+         sizeof(int) * (size_t)(2 * n + 4 + 2 * 16 + 2);
 }
 
 static size_t bcs_lds_bytes(int n, int ne, int D) {
@@ -815,7 +825,7 @@ __global__ void __launch_bounds__(NT) k_slogpf(int n, const float* __restrict__ 
   float* tau = tri + ((ntri + kWave + 3) & ~3);
   float* vv = tau + n + kWave;
   uint32_t* part = reinterpret_cast<uint32_t*>(vv + n + kWave);
-  float* scr = reinterpret_cast<float*>(part + 2 * 16);
+  float* scr = reinterpret_cast<float*>(part + 2 * 16 + 4);
   const float* Ab = A + (size_t)blockIdx.x * n * n;
   for (int i = 1 + threadIdx.x / kWave; i < n; i += NT / kWave)
     for (int j = threadIdx.x & (kWave - 1); j < i; j += kWave) tri[toff(i, j)] = Ab[(size_t)i * n + j];
@@ -911,8 +921,8 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) return NH_ERR_SHAPE;
   const int ntri = (n * (n - 1)) / 2;
-  const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + 2 * (n + kWave) + 2 * kPanel * (n + 16) + n) +
-                     4 * 2 * 16;
+  const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + 2 * (n + kWave) + 2 * kPanel * (n + 16) + n + kWave) +
+                     4 * (2 * 16 + 4);
   if (n <= 64)
     hipLaunchKernelGGL(k_slogpf<64>, dim3(batch), dim3(64), lds, st, n, A, sign, logabs);
   else
