@@ -41,7 +41,9 @@ enum { NH_PREC_FP32 = 0, NH_PREC_MIXED = 1 };
  * layers h <- h + conv3x3(gelu_tanh(h)); features y[spin][site][a] = h[spin * d + a][site],
  * d = C / 2 (code/backflow/models.py:163-167).
  * Weights are packed per layer as [tap = (dx+1)*3 + (dy+1)][c_in][c_out] (layer 0: c_in = 2);
- * biases [(n_layers + 1)][C].
+ * biases [(n_layers + 1)][C].  For LY == 16 and C == 32 the network runs on MFMA when the
+ * fragment-ordered copies are given (layout in neuralhubbard_amd/mixed.py): conv_w_mfma for
+ * NH_PREC_FP32, conv_w_lp (BF16 operands, FP32 accumulation) for NH_PREC_MIXED.
  * pfaffian: wt = [4][N][d] spin-block table [uu, ud, du, dd] with the distance envelope
  *           (code/backflow/pf_kernel.py:107-124); base = F0 [2N][2N] antisymmetric or NULL.
  * bcs:      wt = [N][d] (code/backflow/bcs_kernel.py:63-70); base = phi0 [N] or NULL,
@@ -59,7 +61,8 @@ typedef struct nh_model {
   const float* conv_b;
   const float* wt;
   const float* base;
-  const void* conv_w_lp; /* NH_PREC_MIXED: conv_w repacked by nh_pack_lowp, else NULL */
+  const void* conv_w_lp;     /* NH_PREC_MIXED: BF16 conv weights in MFMA fragment order, else NULL */
+  const float* conv_w_mfma;  /* FP32 conv weights in MFMA fragment order, or NULL (VALU kernel) */
 } nh_model;
 
 /* ---- a6: occupied pairing block  (replaces pf_kernel.pairing_occ, code/backflow/pf_kernel.py:335-351)

@@ -39,6 +39,7 @@ class NhModel(ctypes.Structure):
         ("wt", ctypes.c_void_p),
         ("base", ctypes.c_void_p),
         ("conv_w_lp", ctypes.c_void_p),
+        ("conv_w_mfma", ctypes.c_void_p),
     ]
 
 

@@ -188,6 +188,11 @@ class _Backflow:
             self.Wt = _dev(build_wt_ud(params.W, params.theta, LX, LY), torch.float32)
             self.base = None if params.phi0 is None else _dev(params.phi0, torch.float32)
         self.conv_w_lp = None
+        self.conv_w_mfma = None
+        if workload.LY == 16 and workload.channels == 32:
+            from .mixed import pack_mfma_f32
+
+            self.conv_w_mfma = _dev(pack_mfma_f32(params), torch.float32)
         if precision == "mixed":
             from .mixed import pack_lowp
 
@@ -206,6 +211,7 @@ class _Backflow:
             self.conv_w.data_ptr(), self.conv_b.data_ptr(), self.Wt.data_ptr(),
             0 if self.base is None else self.base.data_ptr(),
             0 if self.conv_w_lp is None else self.conv_w_lp.data_ptr(),
+            0 if self.conv_w_mfma is None else self.conv_w_mfma.data_ptr(),
         )
 
     def features(self, s):

@@ -58,7 +58,7 @@ int check_model(const nh_model* m) {
   if (m->kind == NH_KIND_PFAFFIAN && ((m->n_up + m->n_dn) % 2 || m->n_up + m->n_dn > 256)) return NH_ERR_SHAPE;
   if (m->kind == NH_KIND_BCS && (m->n_up != m->n_dn || m->n_up > 192)) return NH_ERR_SHAPE;
   if (m->precision != NH_PREC_FP32 && m->precision != NH_PREC_MIXED) return NH_ERR_ARG;
-  if (m->precision == NH_PREC_MIXED && !m->conv_w_lp) return NH_ERR_ARG;
+  if (m->precision == NH_PREC_MIXED && (!m->conv_w_lp || !cnn_mfma_supported(m))) return NH_ERR_UNSUPPORTED;
   return NH_OK;
 }
 
@@ -107,9 +107,7 @@ static int run_items(hipStream_t st, const nh_model* m, const int8_t* s, const i
     int rc;
     {
       ProfScope ps(st, NH_PROF_FEATURES);
-      rc = (m->precision == NH_PREC_MIXED)
-               ? launch_cnn_mixed(st, m, s, item_b, item_hop, (int)i0, (int)i1, ipi, b.Y)
-               : launch_cnn(st, m, s, item_b, item_hop, (int)i0, (int)i1, ipi, b.Y);
+      rc = launch_cnn(st, m, s, item_b, item_hop, (int)i0, (int)i1, ipi, b.Y);
     }
     if (rc != NH_OK) return rc;
     {

@@ -143,17 +143,16 @@ static int launch_c(hipStream_t st, const nh_model* m, const int8_t* s, const in
 
 int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
                const int32_t* item_hop, int img0, int img_end, int ipi, float* Y) {
+  if (m->precision == NH_PREC_MIXED)
+    return launch_cnn_mfma(st, m, true, s, item_b, item_hop, img0, img_end, ipi, Y);
+  if (m->conv_w_mfma && cnn_mfma_supported(m))
+    return launch_cnn_mfma(st, m, false, s, item_b, item_hop, img0, img_end, ipi, Y);
   switch (m->channels) {
     case 8: return launch_c<8>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);
     case 16: return launch_c<16>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);
     case 32: return launch_c<32>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);
     default: return NH_ERR_SHAPE;
   }
-}
-
-int launch_cnn_mixed(hipStream_t, const nh_model*, const int8_t*, const int32_t*, const int32_t*, int, int,
-                     int, float*) {
-  return NH_ERR_UNSUPPORTED;
 }
 
 }  // namespace nh

@@ -1,0 +1,68 @@
+"""Fragment-ordered conv weights for the MFMA feature network (LY == 16, C == 32).
+
+FP32 (v_mfma_f32_16x16x4_f32): for layer l, tap, output half hf, K-step kk and lane:
+    W[tap][ci = 4 kk + lane // 16][co = 16 hf + lane % 16]
+BF16 (v_mfma_f32_16x16x32_bf16): for layer l, tap, half hf, lane, j < 8:
+    W[tap][ci = 8 (lane // 16) + j][co = 16 hf + lane % 16]
+  stored as split BF16: the hi parts of all layers, then the lo parts, W - hi rounded.
+Layer 0 has 2 input channels; the missing ones are zero.  Tap = (dx + 1) * 3 + (dy + 1).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+C = 32
+
+
+def _tap_major(params):
+    """[layer][tap][ci (padded to C)][co] float32."""
+    out = []
+    for w in params.conv_w:
+        w = np.asarray(w, np.float32)  # (co, ci, 3, 3)
+        co, ci = w.shape[:2]
+        t = np.zeros((9, C, C), np.float32)
+        t[:, :ci, :co] = np.transpose(w, (2, 3, 1, 0)).reshape(9, ci, co)
+        out.append(t)
+    return np.stack(out)
+
+
+def pack_mfma_f32(params) -> np.ndarray:
+    W = _tap_major(params)  # (L, 9, ci, co)
+    lane = np.arange(64)
+    kk = np.arange(8)
+    ci = 4 * kk[:, None] + lane[None, :] // 16  # (8, 64)
+    out = np.empty((W.shape[0], 9, 2, 8, 64), np.float32)
+    for hf in range(2):
+        co = 16 * hf + lane % 16  # (64,)
+        out[:, :, hf] = W[:, :, ci, co[None, :]]
+    return out.reshape(-1)
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """float32 -> bf16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def pack_bf16(params) -> np.ndarray:
+    """Split-BF16 fragments: hi parts of every layer, then lo parts (W = hi + lo + O(2^-16 W))."""
+    W = _tap_major(params)
+    lane = np.arange(64)
+    j = np.arange(8)
+    ci = 8 * (lane[:, None] // 16) + j[None, :]  # (64, 8)
+    out = np.empty((W.shape[0], 9, 2, 64, 8), np.float32)
+    for hf in range(2):
+        co = (16 * hf + lane % 16)[:, None]  # (64, 1)
+        out[:, :, hf] = W[:, :, ci, co]
+    hi = bf16_bits(out.reshape(-1))
+    hif = (hi.astype(np.uint32) << 16).view(np.float32)
+    lo = bf16_bits(out.reshape(-1) - hif)
+    return np.concatenate([hi, lo])
+
+
+def pack_lowp(model):
+    import torch
+
+    return torch.as_tensor(pack_bf16(model.params).view(np.int16), device="cuda")
