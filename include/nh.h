@@ -118,6 +118,42 @@ int nh_hops(void* stream, int32_t LX, int32_t LY, double t, double tprime, int32
             int32_t n_occ, const int8_t* s, int32_t max_hops, int32_t* count, int32_t* out_from,
             int32_t* out_to, float* out_coef);
 
+/* ---- surrogate log-amplitude network (no reference counterpart: the learned stage one of
+ * delayed-acceptance Metropolis).  z1 = conv3x3(x0) + b1, z2 = conv3x3(gelu(z1)) + b2,
+ * z3 = conv3x3(gelu(z2)) + b3 (1 channel), f(s) = sum over sites of z3; x0 = the 2-channel +-1
+ * occupancy, periodic 3x3 convolutions, channels 16 or 32, N <= 256.  params (device, FP32):
+ * w1 [9][2][C], b1 [C], w2 [9][C][C], b2 [C], w3 [9][C], b3 [1]; nh_sur_nparams gives the count. */
+typedef struct nh_surrogate {
+  int32_t LX, LY, channels, reserved;
+  float* params;
+} nh_surrogate;
+int32_t nh_sur_nparams(int32_t channels);
+/* f(s) for s int8 [batch][2N] -> out float [batch]. */
+int nh_sur_forward(void* stream, const nh_surrogate* sur, int32_t batch, const int8_t* s, float* out);
+/* Gradient of sum_b 0.5 * weight * (f(s_b) - target_b)^2 w.r.t. params -> grad [nparams];
+ * loss [1] (may be NULL).  Per-configuration gradients are summed in a fixed order. */
+size_t nh_sur_grad_workspace_size(const nh_surrogate* sur, int32_t batch);
+int nh_sur_grad(void* stream, const nh_surrogate* sur, int32_t batch, const int8_t* s, const float* target,
+                float weight, float* grad, float* loss, void* workspace, size_t workspace_bytes);
+/* Adam step (Kingma & Ba) on n parameters with bias correction at step >= 1. */
+int nh_adam(void* stream, int32_t n, float* w, const float* g, float* m, float* v, float lr, float beta1,
+            float beta2, float eps, int32_t step);
+
+/* ---- a14 (+ a16): Metropolis exchange sweep  (replaces quantax sampler.sweep of the
+ * MixSampler built at code/states.py:355-408; the delayed-acceptance stage is new).
+ * n_steps proposals per chain; step t of chain c uses Philox4x32-10 at counter
+ * (c, step0 + t, 0) and key seed.  Proposal: site i = r0 mod N, direction r1 & 3 of
+ * (+x, -x, +y, -y), spin r1 >> 2 & 1: the occupations of modes (spin, i) and (spin, j) are
+ * exchanged (a spin-preserving nearest-neighbour hop).  sur == NULL: accept with
+ * min(1, |psi'/psi|^2).  sur != NULL: delayed acceptance (stage 1 with the surrogate, stage 2
+ * exact), f_cur [n_chains] the surrogate value of the current configurations.
+ * s / sign / logabs (/ f_cur) are updated in place; counters int32 [4][n_chains] are
+ * incremented: changing proposals, stage-1 survivors, accepted, exact evaluations. */
+size_t nh_metropolis_workspace_size(const nh_model* model, int32_t n_chains);
+int nh_metropolis_sweep(void* stream, const nh_model* model, const nh_surrogate* sur, int32_t n_chains, int8_t* s,
+                        float* sign, double* logabs, float* f_cur, uint64_t seed, int64_t step0, int32_t n_steps,
+                        int32_t* counters, void* workspace, size_t workspace_bytes);
+
 /* ---- measurement hook (no reference counterpart; used by bench.py): while enabled, every
  * launch of a kernel class is bracketed by HIP events recorded on the launch stream.
  * nh_profile_query waits for the recorded events and returns the launch count and the summed

@@ -22,6 +22,11 @@ class NhError(RuntimeError):
     pass
 
 
+class NhSurrogate(ctypes.Structure):
+    _fields_ = [("LX", ctypes.c_int32), ("LY", ctypes.c_int32), ("channels", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("params", ctypes.c_void_p)]
+
+
 class NhModel(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
@@ -61,6 +66,14 @@ SIGNATURES = {
     "nh_local_energy_workspace_size": (_S, [_MP, _D, _I]),
     "nh_local_energy": (_I, [_P, _MP, _D, _D, _D, _I, _P, _P, _P, _P, _I, _P, _S]),
     "nh_hops": (_I, [_P, _I, _I, _D, _D, _I, _I, _P, _I, _P, _P, _P, _P]),
+    "nh_sur_nparams": (_I, [_I]),
+    "nh_sur_forward": (_I, [_P, ctypes.POINTER(NhSurrogate), _I, _P, _P]),
+    "nh_sur_grad_workspace_size": (_S, [ctypes.POINTER(NhSurrogate), _I]),
+    "nh_sur_grad": (_I, [_P, ctypes.POINTER(NhSurrogate), _I, _P, _P, ctypes.c_float, _P, _P, _P, _S]),
+    "nh_adam": (_I, [_P, _I, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I]),
+    "nh_metropolis_workspace_size": (_S, [_MP, _I]),
+    "nh_metropolis_sweep": (_I, [_P, _MP, ctypes.POINTER(NhSurrogate), _I, _P, _P, _P, _P, ctypes.c_uint64,
+                                 ctypes.c_int64, _I, _P, _P, _S]),
     "nh_profile_enable": (_I, [_I]),
     "nh_profile_query": (_I, [_I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
     "nh_profile_reset": (_I, []),

@@ -148,6 +148,14 @@ extern "C" int nh_log_psi(void* stream, const nh_model* model, int32_t batch, co
   return run_items((hipStream_t)stream, model, s, nullptr, nullptr, batch, b, sign, logabs);
 }
 
+int nh_log_psi_items(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
+                     const int32_t* item_hop, int n_items, float* sign, double* logabs, void* ws, size_t wsb) {
+  if (wsb < nh_log_psi_workspace_size(m, n_items)) return NH_ERR_WORKSPACE;
+  Carver c((void*)align_up((size_t)ws));
+  ItemBuffers b = carve_items(c, m, n_items);
+  return run_items(st, m, s, item_b, item_hop, n_items, b, sign, logabs);
+}
+
 static size_t eloc_carve(Carver& c, const nh_model* m, double tprime, int32_t chunk, int32_t** count,
                          int32_t** item_b, int32_t** item_hop, float** coef, float** hsign, double** hlog,
                          ItemBuffers* ib) {

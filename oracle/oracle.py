@@ -414,3 +414,100 @@ def philox4x32(ctr, key, rounds=10):
         ]
         k = [(k[0] + PHILOX_W0) & 0xFFFFFFFF, (k[1] + PHILOX_W1) & 0xFFFFFFFF]
     return c
+
+
+# ------------------------------------------------------------------------ surrogate network
+
+
+def sur_unpack(p, C):
+    p = np.asarray(p, np.float64)
+    o = 0
+    out = []
+    for size in (9 * 2 * C, C, 9 * C * C, C, 9 * C, 1):
+        out.append(p[o:o + size])
+        o += size
+    w1, b1, w2, b2, w3, b3 = out
+    # [tap][ci][co] -> (co, ci, 3, 3) for conv3x3
+    W1 = w1.reshape(3, 3, 2, C).transpose(3, 2, 0, 1)
+    W2 = w2.reshape(3, 3, C, C).transpose(3, 2, 0, 1)
+    W3 = w3.reshape(3, 3, C, 1).transpose(3, 2, 0, 1)
+    return W1, b1, W2, b2, W3, b3
+
+
+def sur_forward(p, s, LX, LY, C):
+    """f(s) = sum_sites conv3(gelu(conv2(gelu(conv1(x0))))) (layout of include/nh.h)."""
+    W1, b1, W2, b2, W3, b3 = sur_unpack(p, C)
+    x0 = np.asarray(s, np.float64).reshape(2, LX, LY)
+    z1 = conv3x3(x0, W1, b1)
+    z2 = conv3x3(gelu(z1), W2, b2)
+    z3 = conv3x3(gelu(z2), W3, b3)
+    return float(z3.sum())
+
+
+def sur_loss(p, S, T, LX, LY, C, weight):
+    return sum(0.5 * weight * (sur_forward(p, s, LX, LY, C) - t) ** 2 for s, t in zip(S, T))
+
+
+def sur_grad_fd(p, S, T, LX, LY, C, weight, idx, h=1e-6):
+    """Central finite differences of the loss at parameter indices idx."""
+    p = np.asarray(p, np.float64)
+    out = []
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        out.append((sur_loss(p + e, S, T, LX, LY, C, weight) - sur_loss(p - e, S, T, LX, LY, C, weight)) / (2 * h))
+    return np.array(out)
+
+
+# ------------------------------------------------------------------------ Metropolis chains
+
+
+def philox_draw(chain, step, seed):
+    return philox4x32([chain, step & 0xFFFFFFFF, (step >> 32) & 0xFFFFFFFF, 0], [seed & 0xFFFFFFFF, seed >> 32])
+
+
+def metropolis_chain(psi, s0, chain, seed, step0, n_steps, LX, LY, sur=None, margin=1e-6):
+    """Replays chain `chain` of the GPU sweep: proposals from Philox4x32-10 at counter
+    (chain, step), exchange of two nearest-neighbour modes of one spin, acceptance with
+    min(1, |psi'/psi|^2) or, with `sur` (a function s -> f(s)), delayed acceptance.  psi is a
+    function s -> (sign, logabs).  Returns (final configuration, accepted count, near_tie),
+    near_tie True when some acceptance test fell within `margin` of its threshold (then a
+    rounding-level difference can legitimately change the trajectory)."""
+    N = LX * LY
+    s = np.array(s0, dtype=np.int8, copy=True)
+    cur = psi(s)
+    f_cur = sur(s) if sur is not None else 0.0
+    dirs = [(1, 0), (-1, 0), (0, 1), (0, -1)]
+    acc = 0
+    tie = False
+    for t in range(step0, step0 + n_steps):
+        r = philox_draw(chain, t, seed)
+        i = r[0] % N
+        dx, dy = dirs[r[1] & 3]
+        spin = (r[1] >> 2) & 1
+        x, y = divmod(i, LY)
+        j = ((x + dx) % LX) * LY + (y + dy) % LY
+        mi, mj = spin * N + i, spin * N + j
+        if s[mi] == s[mj]:
+            continue
+        u1 = (r[2] >> 8) / 16777216.0
+        u2 = (r[3] >> 8) / 16777216.0
+        sp = s.copy()
+        sp[mi], sp[mj] = s[mj], s[mi]
+        f_new = 0.0
+        if sur is not None:
+            f_new = sur(sp)
+            d1 = 2.0 * (f_new - f_cur)
+            if d1 < 0 and u1 > 0 and abs(math.log(u1) - d1) < margin:
+                tie = True
+            if not (d1 >= 0 or u1 < math.exp(d1)):
+                continue
+        new = psi(sp)
+        d = 2.0 * (new[1] - cur[1]) - (2.0 * (f_new - f_cur) if sur is not None else 0.0)
+        u = u2 if sur is not None else u1
+        if d < 0 and u > 0 and abs(math.log(u) - d) < margin:
+            tie = True
+        if new[0] != 0 and (d >= 0 or u < math.exp(d)):
+            s, cur, f_cur = sp, new, f_new
+            acc += 1
+    return s, acc, tie
