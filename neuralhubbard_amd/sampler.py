@@ -134,7 +134,9 @@ class MetropolisSampler:
         self.sweep_steps = int(sweep_steps)
         self.seed = int(seed)
         self.surrogate = surrogate
-        self.s = torch.as_tensor(np.asarray(initial_spins, np.int8), device="cuda").contiguous().clone()
+        if not isinstance(initial_spins, torch.Tensor):
+            initial_spins = np.asarray(initial_spins, np.int8)
+        self.s = torch.as_tensor(initial_spins, dtype=torch.int8, device="cuda").contiguous().clone()
         n = self.s.shape[0]
         psi = state(self.s)
         self.sign = psi.sign.contiguous().clone()
@@ -142,6 +144,7 @@ class MetropolisSampler:
         self.f_cur = surrogate(self.s).contiguous().clone() if surrogate is not None else None
         self.counters = torch.zeros((4, n), dtype=torch.int32, device="cuda")
         self.step = 0
+        self._stats_from = 0
         self._ws = _Workspace()
         if thermal_steps:
             self.run(thermal_steps)
@@ -179,9 +182,14 @@ class MetropolisSampler:
         if self.surrogate is not None:
             self.f_cur = self.surrogate(self.s).contiguous().clone()
 
+    def reset_stats(self):
+        """Start the acceptance counters afresh (for example after thermalisation)."""
+        self.counters.zero_()
+        self._stats_from = self.step
+
     def stats(self) -> dict:
         c = self.counters.sum(dim=1).cpu().numpy().astype(np.int64)
-        total = self.step * self.n_chains
+        total = (self.step - self._stats_from) * self.n_chains
         return {"proposals": int(total), "changing": int(c[0]), "stage1_passed": int(c[1]),
                 "accepted": int(c[2]), "exact_evaluations": int(c[3]),
                 "acceptance": float(c[2] / max(1, c[0])),

@@ -382,3 +382,22 @@ def test_oracle_jackknife():
     xb = x.reshape(8, 8).mean(1)
     assert abs(errs[8] - xb.std(ddof=1) / math.sqrt(8)) < 1e-12
     assert e == max(errs.values())
+
+
+def test_product_energy_statistics_match_oracle():
+    """neuralhubbard_amd.measure (host statistics) against the oracle's jackknife, and the
+    closed form for i.i.d. sweeps: error with blocks of 1 = std(ddof=1)/sqrt(n)."""
+    from neuralhubbard_amd import measure as M
+
+    rng = np.random.default_rng(3)
+    Em = rng.normal(-0.7 * 256, 2.0, 64)
+    Eq = Em**2 + rng.uniform(50, 60, 64)
+    r = M.energy(Em, Eq, 256)
+    v, e, errs = O.jackknife(lambda m: m / 256, Em)
+    assert abs(r["E_per_site"] - v) < 1e-12 and abs(r["E_err"] - e) < 1e-12
+    assert set(r["E_err_blocks"]) == {1, 2, 4, 8}
+    assert abs(r["E_err_blocks"][1] - Em.std(ddof=1) / np.sqrt(64) / 256) < 1e-12
+    vv, ve, _ = O.jackknife(lambda m, q: (q - m * m) / 256, Em, Eq)
+    assert abs(r["var_per_site"] - vv) < 1e-9 and abs(r["var_err"] - ve) < 1e-9
+    few = M.energy(Em[:7], Eq[:7], 256)
+    assert np.isnan(few["E_err"])
