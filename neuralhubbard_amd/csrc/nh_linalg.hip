@@ -225,155 +225,190 @@ __device__ void pf_eliminate(float* tri, int n, float* tau, float* vv, uint32_t*
 
 typedef float nh_f32x4 __attribute__((ext_vector_type(4)));
 
-// Blocked Parlett-Reid for 64 < n <= 256 (n even) on the packed lower triangle `tri` (followed
-// by kWave dummy floats).  Panels of up to P elimination steps (2P columns):
-//   panel step s (pivot column k, a = k + 1): one thread per row i (waves with no row of the
-//     step skip it: every instruction a wave issues costs its SIMD four cycles) forms the
-//     effective column from the panel-start triangle plus the panel's earlier rank-2 terms
-//     (left-looking), c_i = A[i][k] + sum_{s'<s} (U_s'[i] W_s'[k] - W_s'[i] U_s'[k]); the
-//     pivot row b is the argmax over per-wave DPP maxima; the column pair of the permuted
-//     matrix A' = P A P (P the transposition (a b)) gives U_s = A'[:, k] / A'[a][k] and
-//     W_s = A'[:, a]; row/column b of the triangle and of U, W take the old row/column a.
-//   trailing: A[i][j] += sum_s U_s[i] W_s[j] - W_s[i] U_s[j] for i > j >= k0 + 2P, as 16 x 16
-//     tiles of FP32 MFMA products [U | -W] [W | U]^T over all waves, each trailing element read
-//     and written once per panel.
-// U and W hold P rows of ldu >= n + 16 floats (zeroed per panel, so rows of later steps are 0
-// and every sum runs over all P terms), cbuf n + 1 floats, part 2 * NW + 2 words.
+// Scratch floats of pf_blocked2: panel vectors row-interleaved UW (n + 16) x 2P and q-major
+// UT, WT P x (n + 16) each, column buffer n + 4, pivot keys 16 words, pivots n / 2 + 4.
+template <int P>
+__host__ __device__ inline int pf2_scratch_floats(int n) {
+  return (n + 16) * 4 * P + ((n + 7) & ~3) + 16 + ((n / 2 + 7) & ~3);
+}
+
+__device__ inline double wave_sum_f64(double x) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+// Blocked Parlett-Reid with a short per-step chain (64 < n <= 256, n even), on the packed
+// lower triangle `tri` (followed by kWave dummy floats).  Panels of up to P elimination steps:
+//   (1) rows i > k: effective column c_i = A[i][k] + sum_{q<s} (U_q[i] W_q[k] - W_q[i] U_q[k])
+//       (left-looking over the panel's earlier steps), one 32-bit key per row, |c_i| with its
+//       low 8 mantissa bits replaced by 255 - i, and a wave max of the keys;
+//   (2) every thread reduces the wave keys to the pivot row b (largest |c| to 2^-15 relative,
+//       smallest index among those), reads the pivot, and rows i >= k + 2 form the new panel
+//       column U_s[i] = A'[i][k] / A'[a][k], W_s[i] = A'[i][a] of A' = P A P (P the
+//       transposition (a b)), and move row/column a's triangle entries to b;
+//   trailing: A[i][j] += sum_q U_q[i] W_q[j] - W_q[i] U_q[j] for i > j >= k0 + 2P as 16 x 16
+//       FP32 MFMA tiles [U | -W] [W | U]^T, each trailing element read and written once.
+// Two barriers per step.  The panel vectors are kept twice: row-interleaved (UW[i][2q], UW[i][2q+1],
+// one 16-byte load per two steps for the step reads) and q-major (UT[q][i], WT[q][i], conflict-free
+// MFMA operands).  The pivot row takes row a's panel entries; others may still read row b's old
+// ones in phase (2), so that store waits for phase (1) of the next step, whose readers of the row
+// use row a's slot instead (row a is dead by then and also receives the new column entry).
+// |pivots| go to `pivs` and their logs are summed in FP64 at the end; the sign is uniform.
 template <int NT, int P>
-__device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float* cbuf, uint32_t* part,
-                           float& sign_out, double& log_out) {
+__device__ void pf_blocked2(float* tri, int n, float* scr, float& sign_out, double& log_out) {
   constexpr int NW = NT / kWave;
+  constexpr int LW = 2 * P;
+  static_assert(NW <= 16 && NW % 4 == 0, "pivot keys: 4, 8 or 16 waves");
+  static_assert(P % 2 == 0, "two steps per 16-byte load");
+  const int ldt = n + 16;
+  float* UW = scr;
+  float* UT = UW + ldt * LW;
+  float* WT = UT + P * ldt;
+  float* cbuf = WT + P * ldt;
+  uint32_t* keys = reinterpret_cast<uint32_t*>(cbuf + ((n + 7) & ~3));
+  float* pivs = reinterpret_cast<float*>(keys + 16);
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(t / kWave);
   const int ntri = (n * (n - 1)) / 2;
   const int dummy = ntri + lane;
-  uint32_t* flag = part + 2 * NW;
-  LogAcc acc;
+  const int row_dummy = n + (lane & 15);  // padding rows of UW / UT / WT, never read
   float sign = 1.f;
   bool singular = false;
   for (int k0 = 0; k0 < n; k0 += 2 * P) {
     const int steps = (n - k0) / 2 < P ? (n - k0) / 2 : P;
-    for (int e = t; e < ldu; e += NT)
-#pragma unroll
-      for (int q = 0; q < P; ++q) {
-        U[q * ldu + e] = 0.f;
-        W[q * ldu + e] = 0.f;
-      }
     float pu[P], pw[P];  // this thread's row of the panel vectors
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       pu[q] = 0.f;
       pw[q] = 0.f;
     }
-    __syncthreads();
-    for (int s = 0; s < steps; ++s) {
+    int bprev = -1, aprev = -1;  // uniform: exchange of the previous step of this panel
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      if (s >= steps) break;  // uniform
       const int k = k0 + 2 * s, a = k + 1;
-      // (1) effective column k on rows i > k, per-wave argmax
-      const bool wrow = wave * kWave + kWave - 1 > k && wave * kWave < n;  // uniform
-      if (wrow) {
-        const bool row = t > k && t < n;
-        float c = tri[row ? toff(t, k) : dummy];
-        float uk[P], wk[P];
+      // (0) delayed store of the previous pivot row's adopted entries (q < s - 1)
+      if (s > 1 && t == bprev && bprev != aprev) {
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-          uk[q] = U[q * ldu + k];
-          wk[q] = W[q * ldu + k];
+        for (int q = 0; q + 1 < s; ++q) {
+          UW[t * LW + 2 * q] = pu[q];
+          UW[t * LW + 2 * q + 1] = pw[q];
+          UT[q * ldt + t] = pu[q];
+          WT[q * ldt + t] = pw[q];
         }
-#pragma unroll
-        for (int q = 0; q < P; ++q) c = fmaf(pu[q], wk[q], fmaf(-pw[q], uk[q], c));
-        cbuf[row ? t : n] = c;
-        const uint32_t u = row ? __float_as_uint(fabsf(c)) : 0u;
-        const uint32_t m = wave_max_u32(u);
-        const int bi = wave_min_i32((row && u == m) ? t : n);
-        if (lane == 0) {
-          part[2 * wave] = m;
-          part[2 * wave + 1] = (uint32_t)bi;
-        }
-      } else if (lane == 0) {
-        part[2 * wave] = 0u;
-        part[2 * wave + 1] = (uint32_t)n;
       }
-      __syncthreads();
-      // (2)-(3) pivot and new panel column on rows i >= k + 2
-      const bool wlive = (wave * kWave + kWave - 1 >= k + 2 && wave * kWave < n) || wave == 0;  // uniform
-      float tau = 0.f, v = 0.f, ra = 0.f;
-      int b = a;
-      if (wlive) {
-        const uint32_t pv = lane < NW ? part[2 * lane] : 0u;
-        const int pidx = lane < NW ? (int)part[2 * lane + 1] : n;
-        const uint32_t m = wave_max_u32(pidx < n ? pv : 0u);
-        const int b0 = wave_min_i32((pidx < n && pv == m) ? pidx : n);
-        b = b0 < n ? b0 : a;
-        const float piv = cbuf[b];  // A'[a][k] = A[b][k]
-        const bool bad = !(piv != 0.f) || !isfinite(piv);
-        if (t == 0) {
-          flag[0] = bad ? 1u : 0u;
-          if (!bad) {
-            if (b != a) sign = -sign;
-            sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
-            acc.mul(piv);
-          }
+      // (1) effective column k and the wave's pivot key
+      const bool row = t > k && t < n;
+      float c = 0.f;
+      const bool wrow = wave * kWave + kWave - 1 > k && wave * kWave < n;  // uniform
+      uint32_t key = 0u;
+      if (wrow) {
+        c = tri[row ? toff(t, k) : dummy];
+        const int kk = (k == bprev && bprev != aprev) ? aprev : k;
+        const float4* uwk = reinterpret_cast<const float4*>(UW + kk * LW);
+#pragma unroll
+        for (int q2 = 0; 2 * q2 < s; ++q2) {
+          const float4 x = uwk[q2];  // U_2q2, W_2q2, U_2q2+1, W_2q2+1 of row k
+          c = fmaf(pu[2 * q2], x.y, fmaf(-pw[2 * q2], x.x, c));
+          if (2 * q2 + 1 < s) c = fmaf(pu[2 * q2 + 1], x.w, fmaf(-pw[2 * q2 + 1], x.z, c));
         }
-        const float rp = 1.0f / piv;
+        cbuf[row ? t : n] = c;
+        key = row ? ((__float_as_uint(fabsf(c)) & 0xFFFFFF00u) | (uint32_t)(255 - t)) : 0u;
+        key = wave_max_u32(key);
+      }
+      if (lane == 0) keys[wave] = key;
+      __syncthreads();
+      // (2) pivot row, new panel column, exchange of row/column a into b
+      uint32_t km = 0u;
+#pragma unroll
+      for (int w = 0; w < NW; w += 4) {
+        const uint4 kv = *reinterpret_cast<const uint4*>(keys + w);
+        km = max(km, max(max(kv.x, kv.y), max(kv.z, kv.w)));
+      }
+      km = __builtin_amdgcn_readfirstlane(km);
+      const int b = 255 - (int)(km & 0xFFu);
+      const bool wlive = wave * kWave + kWave - 1 >= k + 2 && wave * kWave < n;  // uniform
+      float piv;
+      if (wlive) {
+        // every load that depends on b is issued before the pivot is examined
         const bool live = t >= k + 2 && t < n;
-        const bool isb = t == b && b != a;
-        const int pi = live ? (isb ? a : t) : a + 1;  // P(i); a + 1: in-bounds stand-in
+        const bool isb = live && t == b && b != a;
+        const int pi = isb ? a : t;  // P(i)
         const int hi = pi > b ? pi : b, lo = pi > b ? b : pi;
-        const float raw = tri[(live && hi != lo) ? toff(hi, lo) : dummy];
-        ra = tri[live ? toff(t, a) : dummy];  // old A[i][a]
-        const float cp = cbuf[live ? pi : k + 1];
+        piv = cbuf[b];                                        // A'[a][k] = A[b][k]
+        const float raw = tri[live ? toff(hi, lo) : dummy];  // A[P i][b]
+        const float ra = tri[live ? toff(t, a) : dummy];     // old A[i][a]
+        const float ca = cbuf[a];
+        const float4* uwb = reinterpret_cast<const float4*>(UW + b * LW);
+        const float4* uwa = reinterpret_cast<const float4*>(UW + a * LW);
         float ub[P], wb[P];
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-          ub[q] = U[q * ldu + b];
-          wb[q] = W[q * ldu + b];
+        for (int q2 = 0; 2 * q2 < s; ++q2) {
+          const float4 x = uwb[q2];
+          ub[2 * q2] = x.x;
+          wb[2 * q2] = x.y;
+          ub[2 * q2 + 1] = x.z;
+          wb[2 * q2 + 1] = x.w;
         }
         if (isb) {  // the pivot row takes row a's earlier entries
 #pragma unroll
-          for (int q = 0; q < P; ++q) {
-            pu[q] = U[q * ldu + a];
-            pw[q] = W[q * ldu + a];
+          for (int q2 = 0; 2 * q2 < s; ++q2) {
+            const float4 x = uwa[q2];
+            pu[2 * q2] = x.x;
+            pw[2 * q2] = x.y;
+            if (2 * q2 + 1 < s) {
+              pu[2 * q2 + 1] = x.z;
+              pw[2 * q2 + 1] = x.w;
+            }
           }
         }
-        v = pi > b ? raw : -raw;  // A[P i][b] from the triangle
+        float rp = __builtin_amdgcn_rcpf(piv);
+        rp = fmaf(rp, fmaf(-piv, rp, 1.f), rp);  // one Newton step
+        float v = pi > b ? raw : -raw;
 #pragma unroll
-        for (int q = 0; q < P; ++q) v = fmaf(pu[q], wb[q], fmaf(-pw[q], ub[q], v));
-        tau = cp * rp;
+        for (int q = 0; q < s; ++q) v = fmaf(pu[q], wb[q], fmaf(-pw[q], ub[q], v));
+        const float tau = (isb ? ca : c) * rp;
+        if (piv != 0.f && isfinite(piv)) {  // uniform
+          const int r1 = live ? t : row_dummy;
+          const int r2 = isb ? a : row_dummy;
+          *reinterpret_cast<float2*>(UW + r1 * LW + 2 * s) = make_float2(tau, v);
+          *reinterpret_cast<float2*>(UW + r2 * LW + 2 * s) = make_float2(tau, v);
+          UT[s * ldt + r1] = tau;
+          WT[s * ldt + r1] = v;
+          pu[s] = live ? tau : 0.f;
+          pw[s] = live ? v : 0.f;
+          const bool ex = live && b != a && t != b;  // new A[i][b] = old A[i][a]
+          tri[ex ? (t > b ? toff(t, b) : toff(b, t)) : dummy] = t > b ? ra : -ra;
+        }
+      } else {
+        piv = cbuf[b];
       }
-      __syncthreads();
-      if (flag[0]) {
+      if (!(piv != 0.f) || !isfinite(piv)) {
         singular = true;
         break;
       }
-      if (wlive) {
-        const bool live = t >= k + 2 && t < n;
-        const bool isb = t == b && b != a;
-        if (live) {
-          if (isb) {
-#pragma unroll
-            for (int q = 0; q < P; ++q)
-              if (q < s) {  // uniform
-                U[q * ldu + t] = pu[q];
-                W[q * ldu + t] = pw[q];
-              }
-          }
-          U[s * ldu + t] = tau;
-          W[s * ldu + t] = v;
-        }
-#pragma unroll
-        for (int q = 0; q < P; ++q)
-          if (q == s) {  // uniform
-            pu[q] = live ? tau : pu[q];
-            pw[q] = live ? v : pw[q];
-          }
-        // new A[i][b] = old A[i][a]
-        const bool ex = live && b != a && t != b;
-        tri[ex ? (t > b ? toff(t, b) : toff(b, t)) : dummy] = t > b ? ra : -ra;
-      }
+      if (b != a) sign = -sign;
+      sign = (piv < 0.f) ? sign : -sign;  // Pf *= A[k][k+1] = -piv
+      if (t == 0) pivs[k >> 1] = piv;
+      bprev = b;
+      aprev = a;
       __syncthreads();
     }
     if (singular) break;
+    // the last step's delayed store
+    if (t == bprev && bprev != aprev) {
+#pragma unroll
+      for (int q = 0; q + 1 < P; ++q)
+        if (q + 1 < steps) {
+          UW[t * LW + 2 * q] = pu[q];
+          UW[t * LW + 2 * q + 1] = pw[q];
+          UT[q * ldt + t] = pu[q];
+          WT[q * ldt + t] = pw[q];
+        }
+    }
+    __syncthreads();
     // trailing rank-2P update of rows/columns >= K
     const int K = k0 + 2 * steps;
     const int mtr = n - K;
@@ -396,14 +431,14 @@ __device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float
         ad[r] = (i < n && col < i) ? toff(i, col) : dummy;
         cacc[r] = tri[ad[r]];
       }
-      const int ia = i0 + (lane & 15);  // A operand row
+      const int ia = i0 + (lane & 15);
       const bool iok = ia < n, jok = col < n;
 #pragma unroll
       for (int kk = 0; kk < (2 * P) / 4; ++kk) {
         const int q = kk * 4 + (lane >> 4);  // 0 .. 2P-1
-        // q < P: U_q[i] W_q[j];  q >= P: -W_q[i] U_q[j]  (entries of unused steps are 0)
-        const float* L = q < P ? U + q * ldu : W + (q - P) * ldu;
-        const float* R = q < P ? W + q * ldu : U + (q - P) * ldu;
+        // q < P: U_q[i] W_q[j];  q >= P: -W_q[i] U_q[j]
+        const float* L = q < P ? UT + q * ldt : WT + (q - P) * ldt;
+        const float* R = q < P ? WT + q * ldt : UT + (q - P) * ldt;
         const float av = iok ? (q < P ? L[ia] : -L[ia]) : 0.f;
         const float bv = jok ? R[col] : 0.f;
         cacc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cacc, 0, 0, 0);
@@ -416,9 +451,14 @@ __device__ void pf_blocked(float* tri, int n, float* U, float* W, int ldu, float
   if (singular) {
     sign_out = 0.f;
     log_out = -INFINITY;
-  } else {
-    sign_out = sign;
-    log_out = acc.log_value();
+    return;
+  }
+  sign_out = sign;
+  log_out = 0.0;
+  if (wave == 0) {
+    double l = 0.0;
+    for (int i = lane; i < n / 2; i += kWave) l += log((double)fabsf(pivs[i]));
+    log_out = wave_sum_f64(l);
   }
 }
 
@@ -518,13 +558,19 @@ __device__ inline float parity_sign(long long k) { return (k & 1) ? -1.f : 1.f; 
 
 // ------------------------------------------------------------------ Pfaffian image kernel
 
-constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffian
+constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffian (n <= 240)
+
+// steps per panel: 8, or 4 above n = 240 where the triangle leaves less LDS for panel vectors
+__host__ __device__ inline int pf_panel(int n) { return n > 240 ? 4 : kPanel; }
+__host__ __device__ inline int pf_elim_floats(int n) {
+  return pf_panel(n) == kPanel ? pf2_scratch_floats<kPanel>(n) : pf2_scratch_floats<4>(n);
+}
 
 // floats of the region used first by the pairing build (features of the occupied modes, one
-// Wt spin block) and then by the blocked elimination (U, W, column buffer)
+// Wt spin block) and then by the blocked elimination (panel vectors, column buffer, pivots)
 __host__ __device__ inline int pf_scratch_floats(int n, int N, int D) {
   const int build = n * (D + 4) + N * D;
-  const int elim = 2 * kPanel * (n + 16) + n + kWave;
+  const int elim = pf_elim_floats(n);
   return ((build > elim ? build : elim) + 3) & ~3;
 }
 
@@ -645,9 +691,11 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 
   float sg;
   double la;
-  if (NT > kWave) {
-    const int ldu = n + 16;
-    pf_blocked<NT, kPanel>(tri, n, scr, scr + kPanel * ldu, ldu, scr + 2 * kPanel * ldu, part, sg, la);
+  if constexpr (NT > kWave) {
+    if (pf_panel(n) == kPanel)
+      pf_blocked2<NT, kPanel>(tri, n, scr, sg, la);
+    else
+      pf_blocked2<NT, 4>(tri, n, scr, sg, la);
   } else {
     pf_eliminate<NT>(tri, n, tau, vv, part, sg, la);
   }
@@ -782,7 +830,7 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
       hipLaunchKernelGGL((k_image_pf<D, 64>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, n, m->n_up, s,
                          item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs);
     } else {
-      hipLaunchKernelGGL((k_image_pf<D, 512>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, n, m->n_up, s,
+      hipLaunchKernelGGL((k_image_pf<D, 1024>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n, m->n_up, s,
                          item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs);
     }
   } else {
@@ -825,16 +873,17 @@ __global__ void __launch_bounds__(NT) k_slogpf(int n, const float* __restrict__ 
   float* tau = tri + ((ntri + kWave + 3) & ~3);
   float* vv = tau + n + kWave;
   uint32_t* part = reinterpret_cast<uint32_t*>(vv + n + kWave);
-  float* scr = reinterpret_cast<float*>(part + 2 * 16 + 4);
   const float* Ab = A + (size_t)blockIdx.x * n * n;
   for (int i = 1 + threadIdx.x / kWave; i < n; i += NT / kWave)
     for (int j = threadIdx.x & (kWave - 1); j < i; j += kWave) tri[toff(i, j)] = Ab[(size_t)i * n + j];
   __syncthreads();
   float sg;
   double la;
-  if (NT > kWave) {
-    const int ldu = n + 16;
-    pf_blocked<NT, kPanel>(tri, n, scr, scr + kPanel * ldu, ldu, scr + 2 * kPanel * ldu, part, sg, la);
+  if constexpr (NT > kWave) {
+    if (pf_panel(n) == kPanel)
+      pf_blocked2<NT, kPanel>(tri, n, tau, sg, la);
+    else
+      pf_blocked2<NT, 4>(tri, n, tau, sg, la);
   } else {
     pf_eliminate<NT>(tri, n, tau, vv, part, sg, la);
   }
@@ -921,8 +970,9 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) return NH_ERR_SHAPE;
   const int ntri = (n * (n - 1)) / 2;
-  const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + 2 * (n + kWave) + 2 * kPanel * (n + 16) + n + kWave) +
-                     4 * (2 * 16 + 4);
+  const int small = 2 * (n + kWave) + 2 * 16 + 4;  // pf_eliminate: tau, vv, part
+  const int big = pf_elim_floats(n);
+  const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + (n <= 64 ? small : big));
   if (n <= 64)
     hipLaunchKernelGGL(k_slogpf<64>, dim3(batch), dim3(64), lds, st, n, A, sign, logabs);
   else
