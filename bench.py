@@ -81,7 +81,7 @@ def read_traffic(images_per_launch):
         return None
 
 
-def cpu_leg(wk, params, spins, gpu_psi):
+def cpu_leg(wk, params, spins, gpu_psi, gpu_eloc):
     """CPU baseline (FP32 C port, timed) and the FP64 oracle as checker of the GPU amplitudes."""
     import numpy as np
 
@@ -108,11 +108,15 @@ def cpu_leg(wk, params, spins, gpu_psi):
     ref = CpuPath(wk, params, bits=64, threads=cores)
     sg64, la64 = ref.psi(x)
     acc = {}
+    ne = len(next(iter(gpu_eloc.values())))
+    e64 = ref.local_energy(x[:ne], sg64[:ne], la64[:ne])
     for form, (sg, la) in gpu_psi.items():
         err = np.abs(np.asarray(la) - la64)
+        eerr = np.abs(np.asarray(gpu_eloc[form]) - e64) / wk.N
         acc[form] = {"configs": int(len(x)), "max_abs_logpsi_err": float(err.max()),
                      "median_abs_logpsi_err": float(np.median(err)),
-                     "sign_agreement": float(np.mean(np.asarray(sg) == sg64))}
+                     "sign_agreement": float(np.mean(np.asarray(sg) == sg64)),
+                     "eloc_configs": int(ne), "max_abs_eloc_per_site_err": float(eerr.max())}
     out["accuracy_vs_fp64"] = acc
     return out
 
@@ -136,7 +140,7 @@ def main():
         return
 
     from neuralhubbard_amd import _lib
-    from neuralhubbard_amd.backflow import Hubbard, Variational, build_backflow, hops
+    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow, hops
     from neuralhubbard_amd.configs import WORKLOADS, make_configurations, make_params
     from neuralhubbard_amd.measure import energy, within_error_bars
     from neuralhubbard_amd.sampler import MetropolisSampler, Surrogate
@@ -298,11 +302,12 @@ def main():
     if not args.no_cpu:
         n = max(1, min(args.cpu_samples, C))
         x = results[head]["sampler"].s[:n]
-        gpu_psi = {}
+        gpu_psi, gpu_eloc = {}, {}
         for f in forms:
             p = states[f](x)
             gpu_psi[f] = (p.sign.cpu().numpy(), p.logabs.cpu().numpy())
-        out.update(cpu_leg(wk, params, x.cpu().numpy(), gpu_psi))
+            gpu_eloc[f] = H.Oloc(states[f], Samples(x[:2])).cpu().numpy()
+        out.update(cpu_leg(wk, params, x.cpu().numpy(), gpu_psi, gpu_eloc))
     print(json.dumps(out))
 
 

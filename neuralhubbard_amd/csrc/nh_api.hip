@@ -18,7 +18,8 @@ int launch_eloc_reduce(hipStream_t st, int N, double U, int batch, int b0, const
                        const float* sign_s, const double* logabs_s, int max_hops, const int32_t* count,
                        const float* coef, const float* hop_sign, const double* hop_logabs, double* eloc);
 
-constexpr int64_t kImageChunk = 32768;  // images whose features are resident at once
+constexpr int64_t kImageChunk = 32768;    // images whose features are resident at once
+constexpr int64_t kImageChunkBig = 1024;  // ... when each image also keeps a global Pfaffian triangle
 
 // ---- measurement hook: HIP events around every launch of a kernel class, on its stream
 struct ProfRec {
@@ -55,7 +56,7 @@ int check_model(const nh_model* m) {
   const int N = m->LX * m->LY;
   if (m->n_up > N || m->n_dn > N) return NH_ERR_SHAPE;
   if (!m->conv_w || !m->conv_b || !m->wt) return NH_ERR_ARG;
-  if (m->kind == NH_KIND_PFAFFIAN && ((m->n_up + m->n_dn) % 2 || m->n_up + m->n_dn > 256)) return NH_ERR_SHAPE;
+  if (m->kind == NH_KIND_PFAFFIAN && ((m->n_up + m->n_dn) % 2 || m->n_up + m->n_dn > 1024)) return NH_ERR_SHAPE;
   if (m->kind == NH_KIND_BCS && (m->n_up != m->n_dn || m->n_up > 192)) return NH_ERR_SHAPE;
   if (m->precision != NH_PREC_FP32 && m->precision != NH_PREC_MIXED) return NH_ERR_ARG;
   if (m->precision == NH_PREC_MIXED && (!m->conv_w_lp || !cnn_mfma_supported(m))) return NH_ERR_UNSUPPORTED;
@@ -82,6 +83,7 @@ struct ItemBuffers {
   float* Y;
   float* img_sign;
   double* img_logabs;
+  float* gtri;  // Pfaffian triangles above n = 256, one per image of a chunk
   int64_t chunk;
 };
 
@@ -89,9 +91,11 @@ static ItemBuffers carve_items(Carver& c, const nh_model* m, int64_t n_items) {
   const int N = m->LX * m->LY;
   const int D = m->channels / 2;
   const int64_t images = n_items * ipi_of(m);
+  const size_t tri = pf_global_tri_floats(m);
   ItemBuffers b;
-  b.chunk = std::min<int64_t>(images, kImageChunk);
+  b.chunk = std::min<int64_t>(images, tri ? kImageChunkBig : kImageChunk);
   b.Y = c.take<float>((size_t)b.chunk * 2 * N * D);
+  b.gtri = tri ? c.take<float>((size_t)b.chunk * tri) : nullptr;
   b.img_sign = c.take<float>((size_t)images);
   b.img_logabs = c.take<double>((size_t)images);
   return b;
@@ -113,7 +117,7 @@ static int run_items(hipStream_t st, const nh_model* m, const int8_t* s, const i
     {
       ProfScope ps(st, NH_PROF_PAIRING_LINALG);
       rc = launch_image_amplitude(st, m, s, item_b, item_hop, (int)i0, (int)i1, ipi, b.Y, b.img_sign,
-                                  b.img_logabs);
+                                  b.img_logabs, b.gtri);
     }
     if (rc != NH_OK) return rc;
   }

@@ -255,8 +255,9 @@ __device__ inline double wave_sum_f64(double x) {
 // ones in phase (2), so that store waits for phase (1) of the next step, whose readers of the row
 // use row a's slot instead (row a is dead by then and also receives the new column entry).
 // |pivots| go to `pivs` and their logs are summed in FP64 at the end; the sign is uniform.
-template <int NT, int P>
+template <int NT, int P, int KB = 8>
 __device__ void pf_blocked2(float* tri, int n, float* scr, float& sign_out, double& log_out) {
+  constexpr uint32_t kIdx = (1u << KB) - 1u;  // low key bits: kIdx - row (n <= 2^KB)
   constexpr int NW = NT / kWave;
   constexpr int LW = 2 * P;
   static_assert(NW <= 16 && NW % 4 == 0, "pivot keys: 4, 8 or 16 waves");
@@ -315,7 +316,7 @@ __device__ void pf_blocked2(float* tri, int n, float* scr, float& sign_out, doub
           if (2 * q2 + 1 < s) c = fmaf(pu[2 * q2 + 1], x.w, fmaf(-pw[2 * q2 + 1], x.z, c));
         }
         cbuf[row ? t : n] = c;
-        key = row ? ((__float_as_uint(fabsf(c)) & 0xFFFFFF00u) | (uint32_t)(255 - t)) : 0u;
+        key = row ? ((__float_as_uint(fabsf(c)) & ~kIdx) | (kIdx - (uint32_t)t)) : 0u;
         key = wave_max_u32(key);
       }
       if (lane == 0) keys[wave] = key;
@@ -328,7 +329,7 @@ __device__ void pf_blocked2(float* tri, int n, float* scr, float& sign_out, doub
         km = max(km, max(max(kv.x, kv.y), max(kv.z, kv.w)));
       }
       km = __builtin_amdgcn_readfirstlane(km);
-      const int b = 255 - (int)(km & 0xFFu);
+      const int b = (int)(kIdx - (km & kIdx));
       const bool wlive = wave * kWave + kWave - 1 >= k + 2 && wave * kWave < n;  // uniform
       float piv;
       if (wlive) {
@@ -561,7 +562,8 @@ __device__ inline float parity_sign(long long k) { return (k & 1) ? -1.f : 1.f; 
 constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffian (n <= 240)
 
 // steps per panel: 8, or 4 above n = 240 where the triangle leaves less LDS for panel vectors
-__host__ __device__ inline int pf_panel(int n) { return n > 240 ? 4 : kPanel; }
+constexpr int kBigN = 256;  // above this the triangle lives in a global workspace, not in LDS
+__host__ __device__ inline int pf_panel(int n) { return (n > 240 && n <= kBigN) ? 4 : kPanel; }
 __host__ __device__ inline int pf_elim_floats(int n) {
   return pf_panel(n) == kPanel ? pf2_scratch_floats<kPanel>(n) : pf2_scratch_floats<4>(n);
 }
@@ -574,7 +576,10 @@ __host__ __device__ inline int pf_scratch_floats(int n, int N, int D) {
   return ((build > elim ? build : elim) + 3) & ~3;
 }
 
-template <int D, int NT>
+// BIG (n > kBigN): the triangle of image img sits in the global workspace gtri at
+// (img - img0) * gstride floats (followed by kWave dummy floats); LDS holds only the pairing
+// build, the panel vectors and the mode lists.
+template <int D, int NT, bool BIG>
 __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, int n_up,
                                                  const int8_t* __restrict__ s,
                                                  const int32_t* __restrict__ item_b,
@@ -583,7 +588,8 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
                                                  const float* __restrict__ wt,
                                                  const float* __restrict__ F0,
                                                  float* __restrict__ img_sign,
-                                                 double* __restrict__ img_logabs) {
+                                                 double* __restrict__ img_logabs,
+                                                 float* __restrict__ gtri, size_t gstride) {
   extern __shared__ float lds[];
   constexpr int YS = D + 4;  // padded row stride of the occupied features
   const int img = img0 + blockIdx.x;
@@ -596,13 +602,20 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
   const int two_n = 2 * N;
 
   const int ntri = (n * (n - 1)) / 2;
-  float* tri = lds;
-  float* scr = tri + ((ntri + kWave + 3) & ~3);  // pairing build, then blocked elimination
+  float* tri;
+  float* scr;  // pairing build, then blocked elimination
+  if constexpr (BIG) {
+    tri = gtri + (size_t)(img - img0) * gstride;
+    scr = lds;
+  } else {
+    tri = lds;
+    scr = tri + ((ntri + kWave + 3) & ~3);
+  }
   float* yocc = scr;
   float* wtl = yocc + n * YS;
   float* tau = scr + pf_scratch_floats(n, N, D);
-  float* vv = tau + n + kWave;
-  int* modes = reinterpret_cast<int*>(vv + n + kWave);
+  float* vv = tau + (BIG ? 0 : n + kWave);  // pf_eliminate only (n <= 64)
+  int* modes = reinterpret_cast<int*>(vv + (BIG ? 0 : n + kWave));
   int* coord = modes + n;  // (x << 16) | y of each mode's site
   int* flag = coord + n;
   uint32_t* part = reinterpret_cast<uint32_t*>(flag + 4);  // 2 * 16 + 2 words
@@ -653,37 +666,40 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
       const int sxp = coord[p] >> 16, syp = coord[p] & 0xffff;
       float* row = tri + toff(p, 0);
       const int qend = region == 1 ? n_up : p;
-      // all column chunks of the row are computed before the first store (loads overlap)
-      float accs[kMaxColChunks];
+      // groups of up to 256 columns; within a group all chunks are computed before the first
+      // store (loads overlap)
+      for (int cq = q0; cq < qend; cq += kMaxColChunks * kWave) {
+        float accs[kMaxColChunks];
 #pragma unroll
-      for (int c = 0; c < kMaxColChunks; ++c) {
-        const int q = q0 + lane + c * kWave;
-        float acc = 0.f;
-        if (q < qend) {
-          const int mq = modes[q];
-          int dx = sxp - (coord[q] >> 16), dy = syp - (coord[q] & 0xffff);
-          dx += dx < 0 ? LX : 0;
-          dy += dy < 0 ? LY : 0;
-          const int r = dx * LY + dy;
-          const float4* w4 = reinterpret_cast<const float4*>(wtl + r * D);
-          const float4* y4 = reinterpret_cast<const float4*>(yocc + q * YS);
+        for (int c = 0; c < kMaxColChunks; ++c) {
+          const int q = cq + lane + c * kWave;
+          float acc = 0.f;
+          if (q < qend) {
+            const int mq = modes[q];
+            int dx = sxp - (coord[q] >> 16), dy = syp - (coord[q] & 0xffff);
+            dx += dx < 0 ? LX : 0;
+            dy += dy < 0 ? LY : 0;
+            const int r = dx * LY + dy;
+            const float4* w4 = reinterpret_cast<const float4*>(wtl + r * D);
+            const float4* y4 = reinterpret_cast<const float4*>(yocc + q * YS);
 #pragma unroll
-          for (int a4 = 0; a4 < D / 4; ++a4) {
-            const float4 w = w4[a4];
-            const float4 yq = y4[a4];
-            acc = fmaf(yp[4 * a4] * yq.x, w.x, acc);
-            acc = fmaf(yp[4 * a4 + 1] * yq.y, w.y, acc);
-            acc = fmaf(yp[4 * a4 + 2] * yq.z, w.z, acc);
-            acc = fmaf(yp[4 * a4 + 3] * yq.w, w.w, acc);
+            for (int a4 = 0; a4 < D / 4; ++a4) {
+              const float4 w = w4[a4];
+              const float4 yq = y4[a4];
+              acc = fmaf(yp[4 * a4] * yq.x, w.x, acc);
+              acc = fmaf(yp[4 * a4 + 1] * yq.y, w.y, acc);
+              acc = fmaf(yp[4 * a4 + 2] * yq.z, w.z, acc);
+              acc = fmaf(yp[4 * a4 + 3] * yq.w, w.w, acc);
+            }
+            if (F0) acc += F0[(size_t)mp * two_n + mq];
           }
-          if (F0) acc += F0[(size_t)mp * two_n + mq];
+          accs[c] = acc;
         }
-        accs[c] = acc;
-      }
 #pragma unroll
-      for (int c = 0; c < kMaxColChunks; ++c) {
-        const int q = q0 + lane + c * kWave;
-        if (q < qend) row[q] = accs[c];
+        for (int c = 0; c < kMaxColChunks; ++c) {
+          const int q = cq + lane + c * kWave;
+          if (q < qend) row[q] = accs[c];
+        }
       }
     }
   }
@@ -691,7 +707,9 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 
   float sg;
   double la;
-  if constexpr (NT > kWave) {
+  if constexpr (BIG) {
+    pf_blocked2<NT, kPanel, 10>(tri, n, scr, sg, la);
+  } else if constexpr (NT > kWave) {
     if (pf_panel(n) == kPanel)
       pf_blocked2<NT, kPanel>(tri, n, scr, sg, la);
     else
@@ -805,8 +823,16 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
 
 static size_t pf_lds_bytes(int n, int N, int D) {
   const int ntri = (n * (n - 1)) / 2;
+  if (n > kBigN)
+    return sizeof(float) * (size_t)pf_scratch_floats(n, N, D) + sizeof(int) * (size_t)(2 * n + 4 + 2 * 16 + 2);
   return sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + pf_scratch_floats(n, N, D) + 2 * (n + kWave)) +
          sizeof(int) * (size_t)(2 * n + 4 + 2 * 16 + 2);
+}
+
+size_t pf_global_tri_floats(const nh_model* m) {
+  const int n = m->n_up + m->n_dn;
+  if (m->kind != NH_KIND_PFAFFIAN || n <= kBigN) return 0;
+  return (size_t)((((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63);
 }
 
 static size_t bcs_lds_bytes(int n, int ne, int D) {
@@ -818,7 +844,7 @@ constexpr size_t kMaxLds = 160 * 1024;
 template <int D>
 static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
                           const int32_t* item_hop, int img0, int img_end, int ipi, const float* Y,
-                          float* img_sign, double* img_logabs) {
+                          float* img_sign, double* img_logabs, float* gtri) {
   const int N = m->LX * m->LY;
   const int nimg = img_end - img0;
   if (nimg <= 0) return NH_OK;
@@ -827,11 +853,18 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
     const size_t lds = pf_lds_bytes(n, N, D);
     if (lds > kMaxLds) return NH_ERR_SHAPE;
     if (n <= 64) {
-      hipLaunchKernelGGL((k_image_pf<D, 64>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, n, m->n_up, s,
-                         item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs);
+      hipLaunchKernelGGL((k_image_pf<D, 64, false>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, n, m->n_up,
+                         s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs,
+                         nullptr, (size_t)0);
+    } else if (n <= kBigN) {
+      hipLaunchKernelGGL((k_image_pf<D, 1024, false>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n,
+                         m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
+                         img_logabs, nullptr, (size_t)0);
     } else {
-      hipLaunchKernelGGL((k_image_pf<D, 1024>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n, m->n_up, s,
-                         item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs);
+      if (!gtri) return NH_ERR_ARG;
+      hipLaunchKernelGGL((k_image_pf<D, 1024, true>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n,
+                         m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
+                         img_logabs, gtri, pf_global_tri_floats(m));
     }
   } else {
     const int n = m->n_up;
@@ -853,11 +886,12 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
 
 int launch_image_amplitude(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
                            const int32_t* item_hop, int img0, int img_end, int ipi, const float* Y,
-                           float* img_sign, double* img_logabs) {
+                           float* img_sign, double* img_logabs, float* gtri) {
   switch (m->channels / 2) {
-    case 4: return launch_image_d<4>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs);
-    case 8: return launch_image_d<8>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs);
-    case 16: return launch_image_d<16>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs);
+    case 4: return launch_image_d<4>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
+    case 8: return launch_image_d<8>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
+    case 16:
+      return launch_image_d<16>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     default: return NH_ERR_SHAPE;
   }
 }

@@ -181,6 +181,22 @@ def test_log_psi_pf16_matches_oracle():
         assert sg[b] == rs and abs(la[b] - rl) < log_tol(1e-3, c), (b, la[b], rl)
 
 
+def test_log_psi_pf32_matches_oracle():
+    """32x32 Pfaffian backflow (896 x 896 Pfaffians, triangles in the global workspace, VALU
+    feature network): same sign as FP64 and |d log psi| < 1e-2 (the measured error is
+    printed; FP32 elimination over 448 pivot steps)."""
+    from neuralhubbard_amd.backflow import Variational
+
+    wk = WORKLOADS["pf32"]
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 2)
+    sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
+    for b in range(2):
+        rs, rl = ref.psi(s[b])
+        print(f"pf32 b={b} gpu={la[b]:.6f} fp64={rl:.6f} |d|={abs(la[b] - rl):.2e}", flush=True)
+        assert sg[b] == rs and abs(la[b] - rl) < 1e-2, (b, la[b], rl)
+
+
 def test_log_psi_bcs16_matches_oracle():
     from neuralhubbard_amd.backflow import Variational
 
