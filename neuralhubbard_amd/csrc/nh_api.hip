@@ -3,6 +3,7 @@
 // caller's workspace.
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -28,22 +29,35 @@ struct ProfRec {
 };
 static std::mutex g_prof_mu;
 static std::vector<ProfRec> g_prof;
-static bool g_prof_on = false;
+static std::atomic<bool> g_prof_on{false};
 
+// Brackets the launches of one kernel class with two events on the launch stream while the
+// profiling hook is enabled; a record whose events could not be created or recorded is dropped.
 struct ProfScope {
   hipStream_t st;
   int kid;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(hipStream_t s, int k) : st(s), kid(k) {
-    if (!g_prof_on) return;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
-    hipEventRecord(a, st);
+    if (!g_prof_on.load(std::memory_order_relaxed)) return;
+    if (hipEventCreate(&a) != hipSuccess) {
+      a = nullptr;
+      return;
+    }
+    if (hipEventCreate(&b) != hipSuccess || hipEventRecord(a, st) != hipSuccess) drop();
   }
   ~ProfScope() {
     if (!a || !b) return;
-    hipEventRecord(b, st);
+    if (hipEventRecord(b, st) != hipSuccess) {
+      drop();
+      return;
+    }
     std::lock_guard<std::mutex> lk(g_prof_mu);
     g_prof.push_back({a, b, kid});
+  }
+  void drop() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    a = b = nullptr;
   }
 };
 
@@ -259,11 +273,12 @@ extern "C" int nh_profile_query(int32_t kernel_class, int64_t* launches, double*
 
 extern "C" int nh_profile_reset(void) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
+  int rc = NH_OK;
   for (auto& r : g_prof) {
-    hipEventSynchronize(r.b);
-    hipEventDestroy(r.a);
-    hipEventDestroy(r.b);
+    if (hipEventSynchronize(r.b) != hipSuccess) rc = NH_ERR_HIP;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
   }
   g_prof.clear();
-  return NH_OK;
+  return rc;
 }
