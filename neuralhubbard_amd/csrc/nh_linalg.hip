@@ -1,12 +1,12 @@
 // Pairing blocks, skew Pfaffian and determinant on gfx950 (FP32 arithmetic, FP64 log-sum).
 //
-// One workgroup owns one matrix.  A Pfaffian matrix lives in LDS as its packed strict lower
-// triangle (row i at i(i-1)/2), so n = 224 takes 98 KiB; a determinant matrix lives as a
-// dense row-major square.  Elimination step k:
-//   wave 0 : pivot search on column k (wave argmax, smallest index on ties), the symmetric
-//            row/column swap, the pivot's log and sign, and the two update vectors;
-//   all    : the rank-2 (Pfaffian) or rank-1 (LU) update of the trailing part, rows spread
-//            over waves, columns over lanes, so LDS accesses are unit-stride.
+// One workgroup owns one matrix.  A Pfaffian matrix lives as its packed strict lower triangle
+// (row i at i(i-1)/2): in LDS up to n = 256 (n = 224 takes 98 KiB), in a global workspace above
+// (pf_blocked2 works on either).  A determinant matrix lives in LDS as a dense row-major square.
+//   n <= 64:  pf_eliminate / lu_eliminate, one wave, rank-2 (rank-1) updates per step.
+//   n > 64:   pf_blocked2 (blocked Parlett-Reid: left-looking panels, packed pivot keys, two
+//             barriers per step, FP32 MFMA trailing updates) and lu_blocked (blocked LU with
+//             rows in place, one barrier per step, MFMA trailing updates).
 // Parlett-Reid for the Pfaffian follows Wimmer, ACM TOMS 38 (2012) 30 (pfaffian_LTL): with
 // pivot p = A[k+1][k], Pf *= -p, tau_i = A[i][k] / p, v_i = A[i][k+1] and
 // A[i][j] += tau_i v_j - v_i tau_j for i > j >= k + 2.
