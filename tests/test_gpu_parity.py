@@ -50,6 +50,50 @@ def test_slogpf_matches_oracle():
             assert abs(la[b] - l_ref) < 2e-4 * max(1.0, n / 56), (n, b, la[b], l_ref)
 
 
+def _cond_tol(A64, base):
+    """base + 1e-6 x condition number: FP32 elimination loses about eps * cond in log|det|."""
+    sv = np.linalg.svd(A64, compute_uv=False)
+    return base + 1e-6 * sv[0] / max(sv[-1], 1e-300)
+
+
+@pytest.mark.parametrize("n", [66, 70, 96, 130, 192, 222, 240, 242, 250, 256])
+def test_slogpf_blocked_sizes(n):
+    """Blocked Pfaffian at sizes around its panel boundaries (8-step panels up to n = 240, 4-step
+    panels above, partial last panels): sign against the FP64 oracle, log|Pf| against
+    LAPACK's log|det| / 2 within a condition-scaled tolerance."""
+    from neuralhubbard_amd.backflow import slogpf
+
+    rng = np.random.default_rng(n)
+    B = 12
+    A = np.stack([rand_skew(n, rng, 0.3) for _ in range(B)]).astype(np.float32)
+    sg, la = slogpf(torch.tensor(A, device="cuda"))
+    sg, la = sg.cpu().numpy(), la.cpu().numpy()
+    for b in range(B):
+        A64 = A[b].astype(np.float64)
+        s_ref, l_ref = O.slogpf(A64)
+        _, ld = np.linalg.slogdet(A64)
+        assert abs(l_ref - ld / 2) < 1e-8
+        assert sg[b] == s_ref, (n, b)
+        assert abs(la[b] - l_ref) < _cond_tol(A64, 2e-4 * n / 56), (n, b, la[b], l_ref)
+
+
+@pytest.mark.parametrize("n", [65, 100, 112, 127, 128, 129, 160, 192])
+def test_slogdet_blocked_sizes(n):
+    """Blocked LU at sizes around its 16-column panels and thread-count switch (n = 128)."""
+    from neuralhubbard_amd.backflow import slogdet
+
+    rng = np.random.default_rng(n)
+    B = 12
+    A = rng.standard_normal((B, n, n)).astype(np.float32)
+    sg, la = slogdet(torch.tensor(A, device="cuda"))
+    sg, la = sg.cpu().numpy(), la.cpu().numpy()
+    for b in range(B):
+        A64 = A[b].astype(np.float64)
+        s_ref, l_ref = np.linalg.slogdet(A64)
+        assert sg[b] == s_ref, (n, b)
+        assert abs(la[b] - l_ref) < _cond_tol(A64, 1e-4 * n / 28), (n, b, la[b], l_ref)
+
+
 def test_slogpf_singular_and_signs():
     from neuralhubbard_amd.backflow import slogpf
 
