@@ -19,8 +19,6 @@ int launch_eloc_reduce(hipStream_t st, int N, double U, int batch, int b0, const
                        const float* sign_s, const double* logabs_s, int max_hops, const int32_t* count,
                        const float* coef, const float* hop_sign, const double* hop_logabs, double* eloc);
 
-constexpr int64_t kImageChunk = 32768;    // images whose features are resident at once
-constexpr int64_t kImageChunkBig = 1024;  // ... when each image also keeps a global Pfaffian triangle
 
 // ---- measurement hook: HIP events around every launch of a kernel class, on its stream
 struct ProfRec {
@@ -107,7 +105,7 @@ static ItemBuffers carve_items(Carver& c, const nh_model* m, int64_t n_items) {
   const int64_t images = n_items * ipi_of(m);
   const size_t tri = pf_global_tri_floats(m);
   ItemBuffers b;
-  b.chunk = std::min<int64_t>(images, tri ? kImageChunkBig : kImageChunk);
+  b.chunk = std::min<int64_t>(images, image_chunk(m));
   b.Y = c.take<float>((size_t)b.chunk * 2 * N * D);
   b.gtri = tri ? c.take<float>((size_t)b.chunk * tri) : nullptr;
   b.img_sign = c.take<float>((size_t)images);

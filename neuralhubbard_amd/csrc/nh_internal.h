@@ -20,12 +20,14 @@ bool cnn_mfma_supported(const nh_model* m);
 
 // Pairing block + Pfaffian (or determinant) of images [img0, img_end), writing the signed
 // log of eta_g(s) psi(g.s) per image into img_sign / img_logabs (indexed by global image).
-// Pfaffians above n = 256 keep their triangle in gtri, pf_global_tri_floats(m) floats per
+// Pfaffians above n = 64 keep their triangle in gtri, pf_global_tri_floats(m) floats per
 // image of the chunk (NULL otherwise).
 int launch_image_amplitude(hipStream_t st, const nh_model* m, const int8_t* s,
                            const int32_t* item_b, const int32_t* item_hop, int img0, int img_end,
                            int ipi, const float* Y, float* img_sign, double* img_logabs, float* gtri);
 size_t pf_global_tri_floats(const nh_model* m);
+// images whose features (and triangles) are resident at once
+int64_t image_chunk(const nh_model* m);
 
 // Signed log-sum over the ipi images of each item in [item0, item_end).
 int launch_combine(hipStream_t st, const int32_t* item_hop, int item0, int item_end, int ipi,

@@ -740,27 +740,32 @@ __device__ inline float parity_sign(long long k) { return (k & 1) ? -1.f : 1.f; 
 
 // ------------------------------------------------------------------ Pfaffian image kernel
 
-constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffian (n <= 240)
+constexpr int kPanel = 8;  // elimination steps per panel of the blocked Pfaffian
 
-// steps per panel: 8, or 4 above n = 240 where the triangle leaves less LDS for panel vectors
-constexpr int kBigN = 256;  // above this the triangle lives in a global workspace, not in LDS
-__host__ __device__ inline int pf_panel(int n) { return (n > 240 && n <= kBigN) ? 4 : kPanel; }
-__host__ __device__ inline int pf_elim_floats(int n) {
-  return pf_panel(n) == kPanel ? pf2_scratch_floats<kPanel>(n) : pf2_scratch_floats<4>(n);
+// Standalone nh_slogpf keeps the triangle in LDS: 8-step panels, or 4 above n = 240 where the
+// triangle leaves less LDS for the panel vectors.
+__host__ __device__ inline int pf_panel_lds(int n) { return n > 240 ? 4 : kPanel; }
+__host__ __device__ inline int pf_elim_floats_lds(int n) {
+  return pf_panel_lds(n) == kPanel ? pf2_scratch_floats<kPanel>(n) : pf2_scratch_floats<4>(n);
 }
+
+// Image kernels above n = 64 keep the triangle in a global workspace: LDS then holds only the
+// pairing build, the panel vectors and the mode lists, so four 256-thread workgroups share a CU
+// and hide each other's pivot-step latency (measured 0.44 us per 224 x 224 image against
+// 0.58 us with the triangle in LDS and one 1024-thread workgroup per CU).
+constexpr int kLdsTriMaxN = 64;
 
 // floats of the region used first by the pairing build (features of the occupied modes, one
 // Wt spin block) and then by the blocked elimination (panel vectors, column buffer, pivots)
 __host__ __device__ inline int pf_scratch_floats(int n, int N, int D) {
   const int build = n * (D + 4) + N * D;
-  const int elim = pf_elim_floats(n);
+  const int elim = pf2_scratch_floats<kPanel>(n);
   return ((build > elim ? build : elim) + 3) & ~3;
 }
 
-// BIG (n > kBigN): the triangle of image img sits in the global workspace gtri at
-// (img - img0) * gstride floats (followed by kWave dummy floats); LDS holds only the pairing
-// build, the panel vectors and the mode lists.
-template <int D, int NT, bool BIG>
+// GTRI (n > kLdsTriMaxN): the triangle of image img sits in the global workspace gtri at
+// (img - img0) * gstride floats (followed by kWave dummy floats).
+template <int D, int NT, bool GTRI>
 __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, int n_up,
                                                  const int8_t* __restrict__ s,
                                                  const int32_t* __restrict__ item_b,
@@ -785,7 +790,7 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
   const int ntri = (n * (n - 1)) / 2;
   float* tri;
   float* scr;  // pairing build, then blocked elimination
-  if constexpr (BIG) {
+  if constexpr (GTRI) {
     tri = gtri + (size_t)(img - img0) * gstride;
     scr = lds;
   } else {
@@ -795,8 +800,8 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
   float* yocc = scr;
   float* wtl = yocc + n * YS;
   float* tau = scr + pf_scratch_floats(n, N, D);
-  float* vv = tau + (BIG ? 0 : n + kWave);  // pf_eliminate only (n <= 64)
-  int* modes = reinterpret_cast<int*>(vv + (BIG ? 0 : n + kWave));
+  float* vv = tau + (GTRI ? 0 : n + kWave);  // pf_eliminate only (n <= 64)
+  int* modes = reinterpret_cast<int*>(vv + (GTRI ? 0 : n + kWave));
   int* coord = modes + n;  // (x << 16) | y of each mode's site
   int* flag = coord + n;
   uint32_t* part = reinterpret_cast<uint32_t*>(flag + 4);  // 2 * 16 + 2 words
@@ -888,16 +893,11 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 
   float sg;
   double la;
-  if constexpr (BIG) {
+  static_assert(GTRI || NT == kWave, "the LDS triangle is for one-wave eliminations (n <= 64)");
+  if constexpr (GTRI)
     pf_blocked2<NT, kPanel, 10>(tri, n, scr, sg, la);
-  } else if constexpr (NT > kWave) {
-    if (pf_panel(n) == kPanel)
-      pf_blocked2<NT, kPanel>(tri, n, scr, sg, la);
-    else
-      pf_blocked2<NT, 4>(tri, n, scr, sg, la);
-  } else {
+  else
     pf_eliminate<NT>(tri, n, tau, vv, part, sg, la);
-  }
   if (t == 0) {
     // fermion_inverse_sign: reversal of the Ne creation operators
     sg *= parity_sign((long long)n * (n - 1) / 2);
@@ -1007,15 +1007,21 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
 
 static size_t pf_lds_bytes(int n, int N, int D) {
   const int ntri = (n * (n - 1)) / 2;
-  if (n > kBigN)
+  if (n > kLdsTriMaxN)
     return sizeof(float) * (size_t)pf_scratch_floats(n, N, D) + sizeof(int) * (size_t)(2 * n + 4 + 2 * 16 + 2);
   return sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + pf_scratch_floats(n, N, D) + 2 * (n + kWave)) +
          sizeof(int) * (size_t)(2 * n + 4 + 2 * 16 + 2);
 }
 
+int64_t image_chunk(const nh_model* m) {
+  const int n = m->n_up + m->n_dn;
+  if (m->kind != NH_KIND_PFAFFIAN || n <= kLdsTriMaxN) return 32768;
+  return n <= 256 ? 8192 : 1024;  // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB)
+}
+
 size_t pf_global_tri_floats(const nh_model* m) {
   const int n = m->n_up + m->n_dn;
-  if (m->kind != NH_KIND_PFAFFIAN || n <= kBigN) return 0;
+  if (m->kind != NH_KIND_PFAFFIAN || n <= kLdsTriMaxN) return 0;
   return (size_t)((((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63);
 }
 
@@ -1040,12 +1046,13 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
       hipLaunchKernelGGL((k_image_pf<D, 64, false>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, n, m->n_up,
                          s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign, img_logabs,
                          nullptr, (size_t)0);
-    } else if (n <= kBigN) {
-      hipLaunchKernelGGL((k_image_pf<D, 1024, false>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n,
+    } else if (!gtri) {
+      return NH_ERR_ARG;
+    } else if (n <= 256) {
+      hipLaunchKernelGGL((k_image_pf<D, 256, true>), dim3(nimg), dim3(256), lds, st, N, m->LX, m->LY, n,
                          m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
-                         img_logabs, nullptr, (size_t)0);
+                         img_logabs, gtri, pf_global_tri_floats(m));
     } else {
-      if (!gtri) return NH_ERR_ARG;
       hipLaunchKernelGGL((k_image_pf<D, 1024, true>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n,
                          m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
                          img_logabs, gtri, pf_global_tri_floats(m));
@@ -1098,7 +1105,7 @@ __global__ void __launch_bounds__(NT) k_slogpf(int n, const float* __restrict__ 
   float sg;
   double la;
   if constexpr (NT > kWave) {
-    if (pf_panel(n) == kPanel)
+    if (pf_panel_lds(n) == kPanel)
       pf_blocked2<NT, kPanel>(tri, n, tau, sg, la);
     else
       pf_blocked2<NT, 4>(tri, n, tau, sg, la);
@@ -1192,7 +1199,7 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   if (n == 0) return NH_ERR_SHAPE;
   const int ntri = (n * (n - 1)) / 2;
   const int small = 2 * (n + kWave) + 2 * 16 + 4;  // pf_eliminate: tau, vv, part
-  const int big = pf_elim_floats(n);
+  const int big = pf_elim_floats_lds(n);
   const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + (n <= 64 ? small : big));
   if (n <= 64)
     hipLaunchKernelGGL(k_slogpf<64>, dim3(batch), dim3(64), lds, st, n, A, sign, logabs);
