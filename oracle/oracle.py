@@ -26,6 +26,13 @@ Pinning status per function (see the tests named in brackets):
                             strings over the full Hilbert space       [test_oracle_eloc]
   philox4x32                published known-answer vectors           [test_oracle_philox]
   jackknife / energy        closed forms for i.i.d. blocks           [test_oracle_jackknife]
+  sur_forward / sur_loss    the same network in torch conv2d (circular
+                            padding), translation invariance, f = N b3
+                            for zero weights                         [test_oracle_surrogate_forward]
+  sur_grad_fd               torch autograd                           [test_oracle_surrogate_gradient]
+  philox_draw /             chi-square stationarity of |psi|^2 over an
+  metropolis_chain          enumerated 3x3 space, plain and delayed;
+                            a |psi|-targeting chain fails it          [test_oracle_metropolis_*]
 """
 
 from __future__ import annotations

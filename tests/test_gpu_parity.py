@@ -1,7 +1,7 @@
 """HIP path vs the FP64 oracle, through the C ABI (marked gpu).
 
 Tolerances: log|psi| within 1e-4 absolute on the small lattices and 1e-3 at 16x16 (the
-north-star bound for FP32 against FP64); signs, hop lists and configurations bit-exact;
+target bound for FP32 against FP64); signs, hop lists and configurations bit-exact;
 E_loc within 1e-6 relative + 1e-5 absolute on small lattices.
 """
 
