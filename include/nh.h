@@ -35,15 +35,25 @@ enum {
 
 enum { NH_KIND_PFAFFIAN = 0, NH_KIND_BCS = 1 };
 enum { NH_PREC_FP32 = 0, NH_PREC_MIXED = 1 };
+enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1 };
 
 /* One backflow network and its pairing parameters (all device pointers).
- * Network: embedding 3x3 periodic conv 2 -> C of the +-1 occupancy, then n_layers residual
- * layers h <- h + conv3x3(gelu_tanh(h)); features y[spin][site][a] = h[spin * d + a][site],
- * d = C / 2 (code/backflow/models.py:163-167).
- * Weights are packed per layer as [tap = (dx+1)*3 + (dy+1)][c_in][c_out] (layer 0: c_in = 2);
- * biases [(n_layers + 1)][C].  For LY == 16 and C == 32 the network runs on MFMA when the
- * fragment-ordered copies are given (layout in neuralhubbard_amd/mixed.py): conv_w_mfma for
- * NH_PREC_FP32, conv_w_lp (BF16 operands, FP32 accumulation) for NH_PREC_MIXED.
+ * Features y[spin][site][a] = h[spin * d + a][site], d = C / 2 (code/backflow/models.py:163-167).
+ * Conv weights are packed per conv as [tap = (dx+1)*3 + (dy+1)][c_in][c_out].
+ * network NH_NET_CONV: embedding 3x3 periodic conv 2 -> C of the +-1 occupancy, then n_layers
+ *   residual layers h <- h + conv3x3(gelu_tanh(h)); conv_w = the n_layers + 1 convs (the first
+ *   with c_in = 2), conv_b [(n_layers + 1)][C].  For LY == 16 and C == 32 the network runs on
+ *   MFMA when the fragment-ordered copies are given (layout in neuralhubbard_amd/mixed.py):
+ *   conv_w_mfma for NH_PREC_FP32, conv_w_lp (BF16 operands, FP32 accumulation) for
+ *   NH_PREC_MIXED.
+ * network NH_NET_RESCONV: the residual CNN of the released states (quantax ResConv as built at
+ *   code/backflow/models.py:319-329; data/supplement/tableS1_network/network_settings.json:21-56):
+ *   h = emb[occupation class of each site] (class = n_up + 2 n_dn: empty, up, down, double),
+ *   n_layers blocks h <- h + conv3x3(gelu_tanh(conv3x3(gelu_tanh(rmsnorm(h))))) with
+ *   rmsnorm(h) = h / sqrt(mean over all channels and sites of h^2 + 1e-5), then
+ *   h / sqrt(n_layers + 1).  conv_w = emb [4][C] followed by the 2 n_layers convs,
+ *   conv_b [2 n_layers][C].  C == 64, LY == 16, LX in {4, 8, 12, 16}, NH_PREC_FP32 (FP32 MFMA);
+ *   conv_w_lp and conv_w_mfma are unused.
  * pfaffian: wt = [4][N][d] spin-block table [uu, ud, du, dd] with the distance envelope
  *           (code/backflow/pf_kernel.py:107-124); base = F0 [2N][2N] antisymmetric or NULL.
  * bcs:      wt = [N][d] (code/backflow/bcs_kernel.py:63-70); base = phi0 [N] or NULL,
@@ -52,11 +62,11 @@ typedef struct nh_model {
   int32_t kind;       /* NH_KIND_* */
   int32_t LX, LY;     /* periodic lattice, both >= 3 */
   int32_t n_up, n_dn; /* electrons per spin */
-  int32_t channels;   /* C: 8, 16 or 32 */
-  int32_t n_layers;   /* residual layers after the embedding conv */
+  int32_t channels;   /* C: 8, 16 or 32 (NH_NET_CONV), 64 (NH_NET_RESCONV) */
+  int32_t n_layers;   /* residual layers after the embedding conv, or residual blocks */
   int32_t precision;  /* NH_PREC_FP32 or NH_PREC_MIXED */
   int32_t project;    /* 1: 8-element projection, 0: the bare network */
-  int32_t reserved;
+  int32_t network;    /* NH_NET_* */
   const float* conv_w;
   const float* conv_b;
   const float* wt;

@@ -91,9 +91,12 @@ def build_wt_ud(W, theta, LX, LY) -> np.ndarray:
 
 
 def pack_conv(params: Params):
-    """Conv weights as [layer][tap = (dx+1)*3 + (dy+1)][c_in][c_out] and biases [layer][C]."""
+    """Conv weights as [conv][tap = (dx+1)*3 + (dy+1)][c_in][c_out] and biases [conv][C]; a
+    residual CNN's occupation embedding goes first as [class][C]."""
     w = [np.ascontiguousarray(np.transpose(np.asarray(cw, np.float32), (2, 3, 1, 0))).reshape(-1)
          for cw in params.conv_w]
+    if params.emb is not None:
+        w.insert(0, np.ascontiguousarray(np.asarray(params.emb, np.float32).T).reshape(-1))
     b = [np.asarray(cb, np.float32).reshape(-1) for cb in params.conv_b]
     return np.concatenate(w), np.concatenate(b)
 
@@ -189,7 +192,10 @@ class _Backflow:
             self.base = None if params.phi0 is None else _dev(params.phi0, torch.float32)
         self.conv_w_lp = None
         self.conv_w_mfma = None
-        if workload.LY == 16 and workload.channels == 32:
+        self.network = _lib.NH_NET_RESCONV if workload.network == "resconv" else _lib.NH_NET_CONV
+        if precision == "mixed" and self.network == _lib.NH_NET_RESCONV:
+            raise _lib.NhError("the residual CNN runs in FP32 only")
+        if self.network == _lib.NH_NET_CONV and workload.LY == 16 and workload.channels == 32:
             from .mixed import pack_mfma_f32
 
             self.conv_w_mfma = _dev(pack_mfma_f32(params), torch.float32)
@@ -207,7 +213,7 @@ class _Backflow:
         return _lib.NhModel(
             self.kind, w.LX, w.LY, w.n_up, w.n_dn, w.channels, w.n_layers,
             _lib.NH_PREC_MIXED if self.precision == "mixed" else _lib.NH_PREC_FP32,
-            1 if project else 0, 0,
+            1 if project else 0, self.network,
             self.conv_w.data_ptr(), self.conv_b.data_ptr(), self.Wt.data_ptr(),
             0 if self.base is None else self.base.data_ptr(),
             0 if self.conv_w_lp is None else self.conv_w_lp.data_ptr(),

@@ -3,7 +3,11 @@
 Each workload is a periodic LX x LY Hubbard lattice (t = 1, t', U), a backflow network
 (a 3x3 periodic conv embedding of the 2-channel +-1 occupancy, then ``n_layers`` residual
 3x3 conv layers ``h <- h + conv(gelu(h))`` of ``channels`` channels, no final activation),
-and a pairing form:
+and a pairing form.  The released states use instead the reference's residual CNN
+(``network="resconv"``, ``data/supplement/tableS1_network/network_settings.json:21-56``): an
+embedding vector per site occupation (empty, up, down, double), ``n_layers`` blocks
+``h <- h + conv(gelu(conv(gelu(rmsnorm(h)))))`` and the output divided by sqrt(n_layers + 1).
+Pairing forms:
 
 * ``pfaffian``: F[p, q] = sum_a Y_pa Y_qa Wt[2 s_p + s_q, r(p, q), a] (+ F0[m_p, m_q]) as in
   ``code/backflow/pf_kernel.py:5`` and ``code/backflow/models.py:178-211``;
@@ -42,6 +46,7 @@ class Workload:
     batch: int
     base_f0: bool = False  # config 0: additive antisymmetric base orbital matrix F0
     seed: int = 1234
+    network: str = "conv"  # "conv" (above) or "resconv" (the released states' residual CNN)
     t: float = 1.0
     notes: str = ""
     extra: dict = field(default_factory=dict)
@@ -64,6 +69,8 @@ class Workload:
 
     def conv_shapes(self):
         C = self.channels
+        if self.network == "resconv":  # two convs per residual block, embedding by lookup
+            return [(C, C)] * (2 * self.n_layers)
         return [(C, 2)] + [(C, C)] * self.n_layers
 
     def cnn_flops(self) -> float:
@@ -96,14 +103,18 @@ WORKLOADS = {
                       notes="16x16 BCS backflow with base pair function, 8192 configurations"),
     "pf32": Workload("pf32", "pfaffian", 32, 32, 448, 448, -0.2, 8.0, 32, 6, 1024,
                      notes="32x32 Pfaffian backflow, 1024 configurations"),
+    # the released tpm02_L16_pfaffian state (code/states.py:114-115): 16 residual blocks of 64
+    # channels, d = 32, U = 8, t' = -0.2, 112 + 112 electrons
+    "tpm02_L16": Workload("tpm02_L16", "pfaffian", 16, 16, 112, 112, -0.2, 8.0, 64, 16, 8192,
+                          network="resconv", notes="released 16x16 Pfaffian backflow, t'=-0.2"),
 }
 
 
 def small_workload(kind="pfaffian", LX=4, LY=4, n_up=2, n_dn=2, tprime=-0.2, channels=4,
-                   n_layers=2, base_f0=False, seed=7, U=8.0) -> Workload:
+                   n_layers=2, base_f0=False, seed=7, U=8.0, network="conv") -> Workload:
     """A lattice small enough for exact enumeration (tests)."""
     return Workload(f"small_{kind}_{LX}x{LY}", kind, LX, LY, n_up, n_dn, tprime, U, channels,
-                    n_layers, 16, base_f0=base_f0, seed=seed)
+                    n_layers, 16, base_f0=base_f0, seed=seed, network=network)
 
 
 @dataclass
@@ -116,11 +127,15 @@ class Params:
     theta: float
     F0: np.ndarray | None = None  # pfaffian base (2N, 2N), antisymmetric
     phi0: np.ndarray | None = None  # bcs base pair function (N,)
+    emb: np.ndarray | None = None  # resconv: (C, 4) vectors of the occupations (empty, up, down, double)
 
 
 def make_params(w: Workload, seed: int | None = None) -> Params:
     rng = np.random.Generator(np.random.PCG64(w.seed if seed is None else seed))
     conv_w, conv_b = [], []
+    emb = None
+    if w.network == "resconv":
+        emb = (rng.standard_normal((w.channels, 4)) / 2.0).astype(np.float32)
     for co, ci in w.conv_shapes():
         conv_w.append((rng.standard_normal((co, ci, 3, 3)) / np.sqrt(9 * ci)).astype(np.float32))
         conv_b.append((0.1 * rng.standard_normal(co)).astype(np.float32))
@@ -135,7 +150,7 @@ def make_params(w: Workload, seed: int | None = None) -> Params:
     else:
         W = (rng.standard_normal((N, d)) / np.sqrt(d)).astype(np.float32)
         phi0 = rng.standard_normal(N).astype(np.float32)
-    return Params(conv_w, conv_b, W, 0.55, F0, phi0)
+    return Params(conv_w, conv_b, W, 0.55, F0, phi0, emb)
 
 
 def make_configurations(w: Workload, batch: int | None = None, seed: int | None = None):

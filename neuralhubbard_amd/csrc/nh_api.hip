@@ -63,8 +63,15 @@ int check_model(const nh_model* m) {
   if (!m) return NH_ERR_ARG;
   if (m->kind != NH_KIND_PFAFFIAN && m->kind != NH_KIND_BCS) return NH_ERR_ARG;
   if (m->LX < 3 || m->LY < 3 || m->LX * m->LY > 1024) return NH_ERR_SHAPE;
-  if (m->channels != 8 && m->channels != 16 && m->channels != 32) return NH_ERR_SHAPE;
   if (m->n_layers < 0 || m->n_up < 0 || m->n_dn < 0) return NH_ERR_ARG;
+  if (m->network == NH_NET_RESCONV) {
+    if (!resconv_supported(m)) return NH_ERR_SHAPE;
+    if (m->precision != NH_PREC_FP32) return NH_ERR_UNSUPPORTED;
+  } else if (m->network != NH_NET_CONV) {
+    return NH_ERR_ARG;
+  } else if (m->channels != 8 && m->channels != 16 && m->channels != 32) {
+    return NH_ERR_SHAPE;
+  }
   const int N = m->LX * m->LY;
   if (m->n_up > N || m->n_dn > N) return NH_ERR_SHAPE;
   if (!m->conv_w || !m->conv_b || !m->wt) return NH_ERR_ARG;

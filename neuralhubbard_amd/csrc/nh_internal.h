@@ -17,6 +17,10 @@ int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t
 int launch_cnn_mfma(hipStream_t st, const nh_model* m, bool bf16, const int8_t* s, const int32_t* item_b,
                     const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
 bool cnn_mfma_supported(const nh_model* m);
+// the residual CNN of the released states (network == NH_NET_RESCONV)
+int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
+                   const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
+bool resconv_supported(const nh_model* m);
 
 // Pairing block + Pfaffian (or determinant) of images [img0, img_end), writing the signed
 // log of eta_g(s) psi(g.s) per image into img_sign / img_logabs (indexed by global image).

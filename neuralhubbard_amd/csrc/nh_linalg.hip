@@ -1083,6 +1083,8 @@ int launch_image_amplitude(hipStream_t st, const nh_model* m, const int8_t* s, c
     case 8: return launch_image_d<8>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     case 16:
       return launch_image_d<16>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
+    case 32:
+      return launch_image_d<32>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     default: return NH_ERR_SHAPE;
   }
 }

@@ -1,0 +1,222 @@
+// Residual CNN of the released backflow states on MFMA (gfx950), FP32 (v_mfma_f32_16x16x4_f32):
+// an embedding of the four site occupations (empty, up, down, double) into C = 64 channels,
+// n_blocks residual blocks h <- h + conv3x3(gelu(conv3x3(gelu(rmsnorm(h))))) with periodic 3x3
+// convolutions and biases, RMSNorm over all channels and sites of one image without affine
+// parameters, and the output h / sqrt(n_blocks + 1) split into y_up | y_dn of d = 32 features
+// (network table of the reference: data/supplement/tableS1_network/network_settings.json:21-56;
+// quantax ResConv as built at code/backflow/models.py:319-329; features at models.py:163-167).
+//
+// Lattices with LY == 16 sites per row and LX % 4 == 0, LX <= 16.  One block of 16 LX threads
+// evaluates one image; wave w owns lattice rows 4w .. 4w+3 for all 64 channels: the residual
+// stream h stays in registers in the MFMA result layout (lane l, register q -> site
+// y = 4 (l / 16) + q, channel 16 quarter + l % 16).  A lattice row of 16 sites is the M = 16
+// dimension of an MFMA tile, so the y-shift of a 3x3 tap is a DPP rotation inside rows of 16
+// lanes (periodic for free) and the x-shift reads the neighbouring lattice row.  Each conv
+// input sits in LDS as [site][64 + 4 pad], channel c at (c % 4) * 16 + c / 4, so a lane's 8
+// K-values of consecutive 16x16x4 steps are two contiguous float4 reads.  Weights are read in
+// their natural [tap][c_in][c_out] layout: each B fragment row is 16 consecutive floats.
+
+#include "nh_common.h"
+#include "nh_internal.h"
+
+namespace nh {
+
+namespace {
+
+typedef float rc_f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRcC = 64;              // channels
+constexpr int kRcLY = 16;             // sites per lattice row
+constexpr int kRcStride = kRcC + 4;   // LDS floats per site (padding spreads the banks)
+constexpr int kRcClasses = 4;         // empty, up, down, double
+constexpr float kRmsEps = 1e-5f;      // added to the mean square before the square root
+
+__device__ inline int rc_pos(int c) { return (c & 3) * 16 + (c >> 2); }
+
+// DPP rotation within rows of 16 lanes: row_ror:1 (0x121) gives lane y the value of lane y - 1,
+// row_ror:15 (0x12F) that of lane y + 1.
+template <int CTRL>
+__device__ inline float rc_rot(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ inline float rc_wave_sum(float x) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+// act[site][pos(c)] = gelu(scale * v) for the wave's rows x0 .. x0+3
+__device__ inline void rc_store(float* act, const rc_f32x4 (&v)[4][4], float scale, int x0, int lane) {
+  const int j = lane & 15, g16 = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int pos = rc_pos(16 * qq + j);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int site = (x0 + r) * kRcLY + 4 * g16 + q;
+        act[site * kRcStride + pos] = gelu_tanh(scale * v[r][qq][q]);
+      }
+    }
+}
+
+// acc = bias + periodic conv3x3 of act (weights [tap][c_in][c_out]) on rows x0 .. x0+3
+template <int DYI>
+__device__ inline void rc_taps(const float* act, const float* __restrict__ Wl, int LX, int x0, int lane, int kh,
+                               rc_f32x4 (&acc)[4][4]) {
+  const int j = lane & 15, g16 = lane >> 4;
+  float fs[6][8];
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    int xr = x0 - 1 + u;
+    xr += xr < 0 ? LX : 0;
+    xr -= xr >= LX ? LX : 0;
+    const float4* p = reinterpret_cast<const float4*>(act + (xr * kRcLY + j) * kRcStride + 16 * g16 + 8 * kh);
+    const float4 p0 = p[0], p1 = p[1];
+    const float v[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      if (DYI == 1) fs[u][kk] = v[kk];
+      else if (DYI == 0) fs[u][kk] = rc_rot<0x121>(v[kk]);  // dy = -1: lane y <- y - 1
+      else fs[u][kk] = rc_rot<0x12F>(v[kk]);                // dy = +1: lane y <- y + 1
+    }
+  }
+#pragma unroll
+  for (int dxi = 0; dxi < 3; ++dxi) {
+    const int tap = dxi * 3 + DYI;
+    // B fragment of K-step kk: lane l holds W[tap][c_in = 32 kh + 4 kk + l / 16][16 qq + l % 16]
+    const float* wt = Wl + ((size_t)tap * kRcC + 32 * kh + g16) * kRcC + j;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      float wv[8];
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) wv[kk] = wt[4 * kk * kRcC + 16 * qq];
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          acc[r][qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(fs[r + dxi][kk], wv[kk], acc[r][qq], 0, 0, 0);
+    }
+  }
+}
+
+__device__ inline void rc_conv(const float* act, const float* __restrict__ Wl, const float* __restrict__ bl, int LX,
+                               int x0, int lane, rc_f32x4 (&acc)[4][4]) {
+  const int j = lane & 15;
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+    const float bv = bl[16 * qq + j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][qq] = rc_f32x4{bv, bv, bv, bv};
+  }
+#pragma unroll 1
+  for (int kh = 0; kh < 2; ++kh) {
+    rc_taps<0>(act, Wl, LX, x0, lane, kh, acc);
+    rc_taps<1>(act, Wl, LX, x0, lane, kh, acc);
+    rc_taps<2>(act, Wl, LX, x0, lane, kh, acc);
+  }
+}
+
+// emb [4][64] (occupation class -> channels), Wc [2 n_blocks][9][64][64], Bc [2 n_blocks][64];
+// Y[(img - img0)][spin][site][32]
+__global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const float* __restrict__ emb,
+                                                 const float* __restrict__ Wc, const float* __restrict__ Bc,
+                                                 float out_scale, const int8_t* __restrict__ s,
+                                                 const int32_t* __restrict__ item_b,
+                                                 const int32_t* __restrict__ item_hop, int img0, int img_end,
+                                                 int ipi, float* __restrict__ Y) {
+  extern __shared__ float lds[];
+  const int N = LX * kRcLY;
+  float* act = lds;
+  float* part = lds + N * kRcStride;  // one partial sum per wave
+  const int img = img0 + blockIdx.x;
+  if (img >= img_end) return;
+  const int item = img / ipi;
+  const int grp = (ipi == kGroup) ? (img & 7) : 0;
+  const int32_t hop = item_hop ? item_hop[item] : kHopNone;
+  if (hop == kHopInvalid) return;
+  const int b = item_b ? item_b[item] : item;
+  const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave, nw = blockDim.x / kWave;
+  const int x0 = 4 * wave, j = lane & 15, g16 = lane >> 4;
+
+  // embedding of the image's occupations: class = n_up + 2 n_dn
+  rc_f32x4 h[4][4];
+  {
+    const int8_t* sb = s + (size_t)b * 2 * N;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int site = (x0 + r) * kRcLY + 4 * g16 + q;
+        const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY)) > 0;
+        const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
+        const float* e = emb + (up + 2 * dn) * kRcC + j;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) h[r][qq][q] = e[16 * qq];
+      }
+  }
+  const float inv_count = 1.0f / (float)(kRcC * N);
+  for (int blk = 0; blk < n_blocks; ++blk) {
+    float ss = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ss = fmaf(h[r][qq][q], h[r][qq][q], ss);
+    ss = rc_wave_sum(ss);
+    if (lane == 0) part[wave] = ss;
+    __syncthreads();  // partial sums visible; every wave is done reading the previous conv input
+    float tot = 0.f;
+    for (int w = 0; w < nw; ++w) tot += part[w];
+    const float rinv = 1.0f / sqrtf(fmaf(tot, inv_count, kRmsEps));
+    rc_store(act, h, rinv, x0, lane);
+    __syncthreads();
+    const float* W1 = Wc + (size_t)(2 * blk) * 9 * kRcC * kRcC;
+    rc_f32x4 acc[4][4];
+    rc_conv(act, W1, Bc + (2 * blk) * kRcC, LX, x0, lane, acc);
+    __syncthreads();  // every wave is done reading the first conv input
+    rc_store(act, acc, 1.0f, x0, lane);
+    __syncthreads();
+    rc_conv(act, W1 + 9 * kRcC * kRcC, Bc + (2 * blk + 1) * kRcC, LX, x0, lane, acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
+  }
+  // features: channel c = 32 spin + a
+  float* yo = Y + (size_t)(img - img0) * (2 * N * (kRcC / 2));
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int site = (x0 + r) * kRcLY + 4 * g16 + q;
+        yo[((size_t)(qq >> 1) * N + site) * (kRcC / 2) + 16 * (qq & 1) + j] = out_scale * h[r][qq][q];
+      }
+}
+
+}  // namespace
+
+bool resconv_supported(const nh_model* m) {
+  return m->channels == kRcC && m->LY == kRcLY && m->LX >= 4 && m->LX <= 16 && m->LX % 4 == 0 && m->n_layers >= 1;
+}
+
+int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
+                   const int32_t* item_hop, int img0, int img_end, int ipi, float* Y) {
+  if (!resconv_supported(m)) return NH_ERR_UNSUPPORTED;
+  const int nimg = img_end - img0;
+  if (nimg <= 0) return NH_OK;
+  const int N = m->LX * kRcLY;
+  const size_t lds = sizeof(float) * ((size_t)N * kRcStride + 16);
+  const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
+  hipLaunchKernelGGL(k_resconv, dim3(nimg), dim3(16 * m->LX), lds, st, m->LX, m->n_layers, m->conv_w,
+                     m->conv_w + kRcClasses * kRcC, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+  NH_LAUNCH_CHECK();
+  return NH_OK;
+}
+
+}  // namespace nh

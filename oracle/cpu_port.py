@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 class CpuModel(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int) for n in ("kind", "LX", "LY", "n_up", "n_dn", "C", "L", "project")] + [
+    _fields_ = [(n, ctypes.c_int) for n in ("kind", "LX", "LY", "n_up", "n_dn", "C", "L", "project", "net")] + [
         (n, ctypes.c_void_p) for n in ("w", "b", "wt", "base")]
 
 
@@ -56,7 +56,8 @@ class CpuPath:
         self._keep.append(None if base is None else np.ascontiguousarray(base, np.float32))
         ptrs = [a.ctypes.data if a is not None else 0 for a in self._keep]
         self.m = CpuModel(0 if workload.kind == "pfaffian" else 1, workload.LX, workload.LY, workload.n_up,
-                          workload.n_dn, workload.channels, workload.n_layers, 1 if project else 0, *ptrs)
+                          workload.n_dn, workload.channels, workload.n_layers, 1 if project else 0,
+                          1 if getattr(workload, "network", "conv") == "resconv" else 0, *ptrs)
 
     def psi(self, s):
         s = np.ascontiguousarray(s, np.int8).reshape(-1, 2 * self.w.N)

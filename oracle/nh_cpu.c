@@ -27,7 +27,8 @@
 typedef struct {
   int kind; /* 0 pfaffian, 1 bcs */
   int LX, LY, n_up, n_dn, C, L, project;
-  const float* w;    /* packed conv weights */
+  int net;           /* 0: conv stack; 1: residual CNN of the released states (L blocks) */
+  const float* w;    /* packed conv weights (residual CNN: embedding [4][C] first) */
   const float* b;    /* biases */
   const float* wt;   /* pf [4][N][d], bcs [N][d] */
   const float* base; /* pf F0 [2N][2N] or bcs phi0 [N] or NULL */
@@ -55,7 +56,7 @@ static REAL gelu(REAL x) {
 }
 
 typedef struct {
-  REAL *h, *g, *acc, *y, *F, *tau, *v;
+  REAL *h, *g, *u, *acc, *y, *F, *tau, *v;
   int* modes;
 } scratch;
 
@@ -64,6 +65,7 @@ static scratch alloc_scratch(const cpu_model* m) {
   scratch s;
   s.h = malloc(sizeof(REAL) * C * N);
   s.g = malloc(sizeof(REAL) * C * N);
+  s.u = malloc(sizeof(REAL) * C * N);
   s.acc = malloc(sizeof(REAL) * C);
   s.y = malloc(sizeof(REAL) * 2 * N * (C / 2));
   s.F = malloc(sizeof(REAL) * (size_t)n * n);
@@ -74,7 +76,7 @@ static scratch alloc_scratch(const cpu_model* m) {
 }
 
 static void free_scratch(scratch* s) {
-  free(s->h); free(s->g); free(s->acc); free(s->y); free(s->F); free(s->tau); free(s->v); free(s->modes);
+  free(s->h); free(s->g); free(s->u); free(s->acc); free(s->y); free(s->F); free(s->tau); free(s->v); free(s->modes);
 }
 
 /* out[co][site] (+)= b[co] + sum_tap sum_ci w[tap][ci][co] in[ci][nbr(site, tap)] */
@@ -98,9 +100,38 @@ static void conv(const cpu_model* m, const float* w, const float* b, int cin, co
   }
 }
 
+/* residual CNN: h = emb[class(site)], blocks h += conv(gelu(conv(gelu(rmsnorm(h))))),
+ * h / sqrt(L + 1) (network_settings.json:21-56 of the reference's data/supplement) */
+static void features_resconv(const cpu_model* m, const int8_t* x, scratch* s) {
+  const int N = m->LX * m->LY, C = m->C;
+  for (int site = 0; site < N; ++site) {
+    const int cls = (x[site] > 0) + 2 * (x[N + site] > 0);
+    for (int c = 0; c < C; ++c) s->h[c * N + site] = (REAL)m->w[cls * C + c];
+  }
+  const float* w = m->w + 4 * C;
+  for (int l = 0; l < m->L; ++l, w += 2 * 9 * C * C) {
+    double ss = 0.0;
+    for (int i = 0; i < C * N; ++i) ss += (double)s->h[i] * s->h[i];
+    const REAL rinv = (REAL)(1.0 / sqrt(ss / (C * N) + 1e-5));
+    for (int i = 0; i < C * N; ++i) s->g[i] = gelu(s->h[i] * rinv);
+    conv(m, w, m->b + 2 * l * C, C, s->g, s->u, s->acc, 0);
+    for (int i = 0; i < C * N; ++i) s->g[i] = gelu(s->u[i]);
+    conv(m, w + 9 * C * C, m->b + (2 * l + 1) * C, C, s->g, s->h, s->acc, 1);
+  }
+  const REAL sc = (REAL)(1.0 / sqrt((double)m->L + 1.0));
+  for (int i = 0; i < C * N; ++i) s->h[i] *= sc;
+}
+
 /* features of configuration x (+-1, 2N) into s->y [2N][d] */
 static void features(const cpu_model* m, const int8_t* x, scratch* s) {
   const int N = m->LX * m->LY, C = m->C, d = C / 2;
+  if (m->net == 1) {
+    features_resconv(m, x, s);
+    for (int spin = 0; spin < 2; ++spin)
+      for (int site = 0; site < N; ++site)
+        for (int a = 0; a < d; ++a) s->y[(spin * N + site) * d + a] = s->h[(spin * d + a) * N + site];
+    return;
+  }
   for (int i = 0; i < 2 * N; ++i) s->g[i] = (REAL)x[i];
   conv(m, m->w, m->b, 2, s->g, s->h, s->acc, 0);
   const float* w = m->w + 9 * 2 * C;

@@ -98,6 +98,52 @@ def features(s, conv_w, conv_b, LX, LY):
     return np.swapaxes(h.reshape(2, C // 2, LX * LY), 1, 2)
 
 
+RMS_EPS = 1e-5  # eqx.nn.RMSNorm default, added to the mean square
+
+
+def occupation_class(s, N):
+    """Site occupation 0 empty, 1 up, 2 down, 3 double (the embedding order of
+    ``data/supplement/tableS1_network/network_settings.json:50``); s (..., 2N)."""
+    s = np.asarray(s)
+    return (s[..., :N] > 0).astype(np.int64) + 2 * (s[..., N:] > 0).astype(np.int64)
+
+
+def rmsnorm(h):
+    """h / sqrt(mean(h^2) + eps) over all channels and sites of each image, no affine
+    parameters (``network_settings.json:53``); h (B, C, LX, LY)."""
+    return h / np.sqrt(np.mean(h * h, axis=(1, 2, 3), keepdims=True) + RMS_EPS)
+
+
+def conv3x3_batch(h, w, b):
+    """`conv3x3` over a batch: h (B, C, LX, LY)."""
+    w = np.asarray(w, np.float64)
+    out = np.zeros((h.shape[0], w.shape[0]) + h.shape[2:])
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            sh = np.roll(h, shift=(-dx, -dy), axis=(2, 3))
+            out += np.einsum("oc,bcxy->boxy", w[:, :, dx + 1, dy + 1], sh, optimize=True)
+    return out + np.asarray(b, np.float64)[None, :, None, None]
+
+
+def resconv_features(S, emb, conv_w, conv_b, LX, LY):
+    """(B, 2, N, d) features of the released states' residual CNN (quantax ResConv as built at
+    ``code/backflow/models.py:319-329``, architecture ``network_settings.json:21-56``): the
+    embedding vector of each site's occupation, blocks h + conv(gelu(conv(gelu(rmsnorm(h))))),
+    output / sqrt(blocks + 1), channels split y_up | y_dn (``models.py:163-167``).
+    S (B, 2N) or (2N,); emb (C, 4); conv_w / conv_b: the 2 * blocks convs."""
+    S = np.atleast_2d(np.asarray(S))
+    B, N = S.shape[0], LX * LY
+    emb = np.asarray(emb, np.float64).reshape(-1, 4)
+    C = emb.shape[0]
+    h = np.transpose(emb[:, occupation_class(S, N)], (1, 0, 2)).reshape(B, C, LX, LY)
+    nb = len(conv_w) // 2
+    for k in range(nb):
+        a = conv3x3_batch(gelu(rmsnorm(h)), conv_w[2 * k], conv_b[2 * k])
+        h = h + conv3x3_batch(gelu(a), conv_w[2 * k + 1], conv_b[2 * k + 1])
+    h = h / math.sqrt(nb + 1)
+    return np.swapaxes(h.reshape(B, 2, C // 2, N), 2, 3)
+
+
 def softplus(t):
     return math.log1p(math.exp(-abs(t))) + max(t, 0.0)
 
@@ -250,7 +296,16 @@ class Model:
 
     def features(self, s):
         w = self.w
+        if getattr(w, "network", "conv") == "resconv":
+            return resconv_features(s, self.p.emb, self.p.conv_w, self.p.conv_b, w.LX, w.LY)[0]
         return features(s, self.p.conv_w, self.p.conv_b, w.LX, w.LY)
+
+    def features_batch(self, S):
+        """(B, 2, N, d) features of a batch of configurations."""
+        w = self.w
+        if getattr(w, "network", "conv") == "resconv":
+            return resconv_features(S, self.p.emb, self.p.conv_w, self.p.conv_b, w.LX, w.LY)
+        return np.stack([self.features(s) for s in np.asarray(S)])
 
     def image_matrix(self, s, g):
         """Matrix whose Pfaffian (determinant for BCS, times the block sign) is

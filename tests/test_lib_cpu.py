@@ -10,7 +10,7 @@ import pytest
 
 from neuralhubbard_amd import _lib
 from neuralhubbard_amd import backflow as bf
-from neuralhubbard_amd.configs import WORKLOADS, make_params
+from neuralhubbard_amd.configs import WORKLOADS, make_params, small_workload
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -41,6 +41,32 @@ def test_workspace_queries_need_no_gpu():
     assert lib.nh_local_energy_workspace_size(ctypes.byref(m), -0.2, 4) > 0
     bad = _lib.NhModel(0, 2, 16, 112, 112, 32, 6, 0, 1, 0, 16, 16, 16, 0, 0)
     assert lib.nh_log_psi_workspace_size(ctypes.byref(bad), 8) == 0
+
+
+def test_residual_cnn_model_checks():
+    lib = _lib.load()
+    R = _lib.NH_NET_RESCONV
+    ok = _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 0, 1, R, 16, 16, 16, 0, 0)
+    assert lib.nh_log_psi_workspace_size(ctypes.byref(ok), 64) > 64 * 8 * 2 * 256 * 32 * 4
+    for bad in (_lib.NhModel(0, 16, 16, 112, 112, 32, 16, 0, 1, R, 16, 16, 16, 0, 0),   # C != 64
+                _lib.NhModel(0, 16, 24, 168, 168, 64, 16, 0, 1, R, 16, 16, 16, 0, 0),   # LY != 16
+                _lib.NhModel(0, 20, 16, 140, 140, 64, 16, 0, 1, R, 16, 16, 16, 0, 0),   # LX > 16
+                _lib.NhModel(0, 16, 16, 112, 112, 64, 0, 0, 1, R, 16, 16, 16, 0, 0),    # no blocks
+                _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 1, 1, R, 16, 16, 16, 16, 16),  # mixed
+                _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 0, 1, 7, 16, 16, 16, 0, 0)):   # network
+        assert lib.nh_log_psi_workspace_size(ctypes.byref(bad), 64) == 0
+
+
+def test_pack_resconv_layout():
+    wk = small_workload("pfaffian", LX=4, LY=16, n_up=12, n_dn=12, channels=64, n_layers=2, network="resconv")
+    p = make_params(wk)
+    w, b = bf.pack_conv(p)
+    C = 64
+    assert w.size == 4 * C + 4 * 9 * C * C and b.size == 4 * C
+    assert w[2 * C + 7] == p.emb[7, 2]  # [class][c]
+    # conv 3 (second conv of block 1), tap (dx, dy) = (0, +1), c_in 9, c_out 40
+    assert w[4 * C + 3 * 9 * C * C + ((1 * 3 + 2) * C + 9) * C + 40] == p.conv_w[3][40, 9, 1, 2]
+    assert b[3 * C + 40] == p.conv_b[3][40]
 
 
 def test_argument_errors_without_gpu():

@@ -1,0 +1,170 @@
+"""Released-state loaders and the residual CNN restatement on the CPU.
+
+* the fixture tests/golden/tpm02_L16_pfaffian.npz against the reference's own files when the
+  reference checkout is present (md5 of the weight file, code/states.py:114; every leaf; the
+  packed chains bit for bit, code/states.py:328-352), and the parameter count 1,208,320 of
+  code/states.py:114 without it;
+* the oracle's residual CNN: its batched convolution against the scipy-pinned `conv3x3`,
+  RMSNorm's closed form, translation equivariance of the whole network, the C restatement
+  (libnhcpu64) against the numpy one, and the fixture's features and amplitudes against a
+  recomputation (drift guard);
+* the fixture's FP64 E_loc of released chains against the paper's E/N: one chain's E_loc / N
+  has a spread of sqrt(0.0272 / 256) = 0.0103 around -0.74533 (code/states.py:114-115).
+"""
+
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from neuralhubbard_amd import released as R
+from neuralhubbard_amd.configs import Params, make_configurations, make_params, small_workload
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "tpm02_L16_pfaffian.npz")
+REF_WEIGHTS = "/root/reference/weights"
+NAME = "tpm02_L16_pfaffian"
+
+
+def fixture():
+    f = dict(np.load(FIXTURE, allow_pickle=False))
+    p = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, None, f["emb"])
+    return f, p
+
+
+def test_fixture_parameter_count_and_chains():
+    f, p = fixture()
+    assert R.n_params(p, pad=1791) == R.STATES[NAME].nparams == 1_208_320
+    S = R.unpack_spins(f["bits"], f["shape"])
+    N = R.STATES[NAME].N
+    assert S.shape == (2048, 2 * N) and S.dtype == np.int8
+    assert np.all((S[:, :N] > 0).sum(1) == 112) and np.all((S[:, N:] > 0).sum(1) == 112)
+    rec = R.pack_spins(S)
+    np.testing.assert_array_equal(rec["bits"], f["bits"])
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_WEIGHTS), reason="reference checkout not present")
+def test_fixture_matches_reference_files():
+    path = os.path.join(REF_WEIGHTS, f"{NAME}.eqx")
+    with open(path, "rb") as fh:
+        assert hashlib.md5(fh.read()).hexdigest() == R.STATES[NAME].md5
+    f, p = fixture()
+    wk, q = R.load_released(NAME, path)
+    np.testing.assert_array_equal(q.emb, p.emb)
+    np.testing.assert_array_equal(np.stack(q.conv_w), f["conv_w"])
+    np.testing.assert_array_equal(np.stack(q.conv_b), f["conv_b"])
+    np.testing.assert_array_equal(q.W, f["W"])
+    assert np.float32(q.theta) == f["theta"]
+    spins_path = os.path.join(REF_WEIGHTS, f"{NAME}_spins.npz")
+    S = R.load_spins(spins_path, R.STATES[NAME])
+    n = int(f["shape"][0])
+    np.testing.assert_array_equal(S[:n], R.unpack_spins(f["bits"], f["shape"]))
+    with np.load(spins_path, allow_pickle=False) as g:
+        np.testing.assert_array_equal(R.pack_spins(S)["bits"], g["bits"])
+
+
+def test_loader_rejects_malformed_leaves():
+    rng = np.random.default_rng(0)
+    C, N = 8, 16
+    leaves = [np.int64(C), rng.standard_normal((C, 4, 1, 1)).astype(np.float32)]
+    for _ in range(2):
+        leaves += [rng.standard_normal((C, C, 3, 3)).astype(np.float32), np.zeros((C, 1, 1), np.float32)]
+    leaves += [np.bool_(False), rng.standard_normal((3, N, C // 2)).astype(np.float32), np.float32(0.5),
+               np.zeros(7, np.float32)]
+    p = R.released_params(leaves, 4, 4)
+    assert len(p.conv_w) == 2 and p.emb.shape == (C, 4) and p.theta == 0.5
+    bad = list(leaves)
+    bad[-1] = np.ones(7, np.float32)
+    with pytest.raises(ValueError):
+        R.released_params(bad, 4, 4)
+    with pytest.raises(ValueError):
+        R.released_params(leaves[:3] + leaves[4:], 4, 4)  # a conv without its bias
+
+
+def test_conv_batch_and_rmsnorm_closed_forms():
+    rng = np.random.default_rng(1)
+    h = rng.standard_normal((3, 5, 4, 6))
+    w = rng.standard_normal((7, 5, 3, 3))
+    b = rng.standard_normal(7)
+    out = O.conv3x3_batch(h, w, b)
+    for i in range(3):
+        np.testing.assert_allclose(out[i], O.conv3x3(h[i], w, b), rtol=1e-12, atol=1e-12)
+    r = O.rmsnorm(h)
+    ms = np.mean(h**2, axis=(1, 2, 3))
+    np.testing.assert_allclose(np.mean(r**2, axis=(1, 2, 3)), ms / (ms + O.RMS_EPS), rtol=1e-12)
+    np.testing.assert_allclose(O.rmsnorm(3.0 * h) * np.sqrt(1 + O.RMS_EPS / (9 * ms))[:, None, None, None],
+                               r * np.sqrt(1 + O.RMS_EPS / ms)[:, None, None, None], rtol=1e-10)
+
+
+def test_resconv_is_translation_equivariant():
+    wk = small_workload("pfaffian", LX=4, LY=6, n_up=5, n_dn=4, channels=8, n_layers=2, network="resconv")
+    p = make_params(wk)
+    s = make_configurations(wk, 1)[0]
+    y = O.resconv_features(s, p.emb, p.conv_w, p.conv_b, wk.LX, wk.LY)[0]
+    N = wk.N
+    for dx, dy in ((1, 0), (0, 1), (3, 5)):
+        def shift(v):
+            return np.roll(v.reshape(-1, wk.LX, wk.LY), (dx, dy), axis=(1, 2)).reshape(v.shape)
+        ss = np.concatenate([shift(s[:N]), shift(s[N:])])
+        ys = O.resconv_features(ss, p.emb, p.conv_w, p.conv_b, wk.LX, wk.LY)[0]
+        for spin in range(2):
+            np.testing.assert_allclose(ys[spin], np.roll(y[spin].reshape(wk.LX, wk.LY, -1), (dx, dy), axis=(0, 1))
+                                       .reshape(N, -1), rtol=1e-12, atol=1e-12)
+
+
+def test_resconv_embedding_order():
+    """Site classes 0..3 are (empty, up, down, double): with zero convolutions the network
+    returns emb[:, class] / sqrt(blocks + 1) at each site."""
+    wk = small_workload("pfaffian", LX=2, LY=3, n_up=2, n_dn=2, channels=4, n_layers=1, network="resconv")
+    p = make_params(wk)
+    zero = [np.zeros_like(w) for w in p.conv_w]
+    zb = [np.zeros_like(b) for b in p.conv_b]
+    s = -np.ones(12, np.int8)
+    s[[0, 3]] = 1       # up at sites 0 and 3
+    s[6 + 2] = 1        # down at site 2
+    s[6 + 3] = 1        # down at site 3 (double)
+    y = O.resconv_features(s, p.emb, zero, zb, 2, 3)[0]  # (2, N, 2)
+    h = np.concatenate([y[0], y[1]], axis=1) * np.sqrt(2)  # (N, C)
+    np.testing.assert_allclose(h[0], p.emb[:, 1], rtol=1e-6)
+    np.testing.assert_allclose(h[1], p.emb[:, 0], rtol=1e-6)
+    np.testing.assert_allclose(h[2], p.emb[:, 2], rtol=1e-6)
+    np.testing.assert_allclose(h[3], p.emb[:, 3], rtol=1e-6)
+
+
+def test_c_restatement_matches_numpy_resconv():
+    from oracle.cpu_port import CpuPath
+
+    wk = small_workload("pfaffian", LX=4, LY=4, n_up=3, n_dn=3, channels=8, n_layers=2, network="resconv")
+    p = make_params(wk)
+    S = make_configurations(wk, 3)
+    ref = O.Model(wk, p)
+    rs, rl = ref.psi_batch(S)
+    cs, cl = CpuPath(wk, p, bits=64).psi(S)
+    np.testing.assert_array_equal(cs, rs)
+    # the C restatement takes the Wt table rounded to float32, the numpy one keeps it in float64
+    np.testing.assert_allclose(cl, rl, rtol=0, atol=2e-6)
+    e_ref = np.array([O.eloc(ref, s, wk.U, 1.0, wk.tprime) for s in S])
+    e_c = CpuPath(wk, p, bits=64).local_energy(S, rs, rl)
+    np.testing.assert_allclose(e_c, e_ref, rtol=1e-5, atol=1e-5)
+
+
+def test_fixture_oracle_values_are_current():
+    from oracle.cpu_port import CpuPath
+
+    f, p = fixture()
+    info = R.STATES[NAME]
+    wk = info.workload()
+    S = R.unpack_spins(f["bits"], f["shape"])
+    y = O.resconv_features(S[:1], p.emb, p.conv_w, p.conv_b, wk.LX, wk.LY)
+    np.testing.assert_allclose(y, f["features"][:1], rtol=0, atol=1e-5 * np.abs(y).max())
+    sg, la = CpuPath(wk, p, bits=64).psi(S[:2])
+    np.testing.assert_array_equal(sg, f["psi_sign"][:2])
+    np.testing.assert_allclose(la, f["psi_logabs"][:2], rtol=0, atol=1e-9)
+    spread = np.sqrt(info.var_per_site / info.N)
+    e = f["eloc"] / info.N
+    assert np.all(np.abs(e - info.E_per_site) < 5 * spread)
+    # the check discriminates a plausible convention mistake: up and down embeddings exchanged
+    e_swap = f["eloc_swapped_embedding"] / info.N
+    assert np.any(np.abs(e_swap - info.E_per_site) > 5 * spread)
