@@ -52,8 +52,10 @@ enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1 };
  *   n_layers blocks h <- h + conv3x3(gelu_tanh(conv3x3(gelu_tanh(rmsnorm(h))))) with
  *   rmsnorm(h) = h / sqrt(mean over all channels and sites of h^2 + 1e-5), then
  *   h / sqrt(n_layers + 1).  conv_w = emb [4][C] followed by the 2 n_layers convs,
- *   conv_b [2 n_layers][C].  C == 64, LY == 16, LX in {4, 8, 12, 16}, NH_PREC_FP32 (FP32 MFMA);
- *   conv_w_lp and conv_w_mfma are unused.
+ *   conv_b [2 n_layers][C].  C == 64, LY == 16, LX in {4, 8, 12, 16}.  NH_PREC_FP32 runs FP32
+ *   MFMA; NH_PREC_MIXED runs split-BF16 MFMA (FP32 accumulation) on conv_w_lp, the BF16
+ *   A-fragments of the convs (layout in neuralhubbard_amd/mixed.py, pack_resconv_bf16).
+ *   conv_w_mfma is unused.
  * pfaffian: wt = [4][N][d] spin-block table [uu, ud, du, dd] with the distance envelope
  *           (code/backflow/pf_kernel.py:107-124); base = F0 [2N][2N] antisymmetric or NULL.
  * bcs:      wt = [N][d] (code/backflow/bcs_kernel.py:63-70); base = phi0 [N] or NULL,

@@ -193,8 +193,6 @@ class _Backflow:
         self.conv_w_lp = None
         self.conv_w_mfma = None
         self.network = _lib.NH_NET_RESCONV if workload.network == "resconv" else _lib.NH_NET_CONV
-        if precision == "mixed" and self.network == _lib.NH_NET_RESCONV:
-            raise _lib.NhError("the residual CNN runs in FP32 only")
         if self.network == _lib.NH_NET_CONV and workload.LY == 16 and workload.channels == 32:
             from .mixed import pack_mfma_f32
 

@@ -66,7 +66,7 @@ int check_model(const nh_model* m) {
   if (m->n_layers < 0 || m->n_up < 0 || m->n_dn < 0) return NH_ERR_ARG;
   if (m->network == NH_NET_RESCONV) {
     if (!resconv_supported(m)) return NH_ERR_SHAPE;
-    if (m->precision != NH_PREC_FP32) return NH_ERR_UNSUPPORTED;
+    if (m->precision == NH_PREC_MIXED && !m->conv_w_lp) return NH_ERR_UNSUPPORTED;
   } else if (m->network != NH_NET_CONV) {
     return NH_ERR_ARG;
   } else if (m->channels != 8 && m->channels != 16 && m->channels != 32) {
@@ -78,7 +78,8 @@ int check_model(const nh_model* m) {
   if (m->kind == NH_KIND_PFAFFIAN && ((m->n_up + m->n_dn) % 2 || m->n_up + m->n_dn > 1024)) return NH_ERR_SHAPE;
   if (m->kind == NH_KIND_BCS && (m->n_up != m->n_dn || m->n_up > 192)) return NH_ERR_SHAPE;
   if (m->precision != NH_PREC_FP32 && m->precision != NH_PREC_MIXED) return NH_ERR_ARG;
-  if (m->precision == NH_PREC_MIXED && (!m->conv_w_lp || !cnn_mfma_supported(m))) return NH_ERR_UNSUPPORTED;
+  if (m->network == NH_NET_CONV && m->precision == NH_PREC_MIXED && (!m->conv_w_lp || !cnn_mfma_supported(m)))
+    return NH_ERR_UNSUPPORTED;
   return NH_OK;
 }
 

@@ -199,6 +199,196 @@ __global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const flo
       }
 }
 
+// ---------------------------------------------------------------- split-BF16 variant
+//
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation; every operand is carried as hi + lo BF16
+// parts and the product accumulates lo*hi + hi*lo + hi*hi, which keeps the features within
+// ~1e-5 of FP32.  The operand roles are swapped against the FP32 kernel: A = weights (M = 16
+// output channels of a quarter), B = activations (N = 16 sites of a lattice row), so the
+// result (lane l, register e -> channel 16 quarter + 4 (l / 16) + e, site y = l % 16) gives each
+// lane four consecutive channels of one site, stored to LDS as one 8-byte hi and one 8-byte lo
+// write.  Activations: two planes (hi, lo) of [site][64 + 8 pad] BF16; a lane's B fragment is
+// the 8 channels 32 kh + 8 (l / 16) .. + 7 of its site, one 16-byte read per plane.  Weights
+// arrive in fragment order (neuralhubbard_amd/mixed.py, pack_resconv_bf16): for conv, tap,
+// K-half kh, quarter qq and lane l, 8 BF16 = W[tap][32 kh + 8 (l / 16) + e][16 qq + l % 16];
+// all hi fragments, then all lo fragments.
+
+typedef short rb_bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kRbStride = kRcC + 8;  // BF16 per site per plane (144 bytes: conflict-free reads)
+
+__device__ inline uint32_t rb_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// x * sigmoid(2 u) = 0.5 x (1 + tanh u), u = sqrt(2/pi) (x + 0.044715 x^3), with the hardware
+// exp2 and reciprocal
+__device__ inline float rb_gelu(float x) {
+  const float u2 = -2.0f * 0.7978845608028654f * 1.4426950408889634f * fmaf(0.044715f * x, x * x, x);
+  return x * __builtin_amdgcn_rcpf(1.0f + exp2f(u2));
+}
+
+template <int CTRL>
+__device__ inline int rb_rot(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false); }
+
+// act planes = gelu(scale * v), BF16 hi and lo, for the wave's rows x0 .. x0+3
+__device__ inline void rb_store(uint16_t* act, const rc_f32x4 (&v)[4][4], float scale, int x0, int lane, int N) {
+  const int j = lane & 15, g16 = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      uint32_t hi[4], lo[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float g = rb_gelu(scale * v[r][qq][e]);
+        hi[e] = rb_bf16(g);
+        lo[e] = rb_bf16(g - __uint_as_float(hi[e] << 16));
+      }
+      uint16_t* dst = act + ((x0 + r) * kRcLY + j) * kRbStride + 16 * qq + 4 * g16;
+      *reinterpret_cast<uint2*>(dst) = make_uint2(hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16));
+      *reinterpret_cast<uint2*>(dst + N * kRbStride) = make_uint2(lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16));
+    }
+}
+
+template <int DYI>
+__device__ inline void rb_taps(const uint16_t* act, const rb_bf16x8* __restrict__ Wh, const rb_bf16x8* __restrict__ Wl,
+                               int LX, int N, int x0, int lane, int kh, rc_f32x4 (&acc)[4][4]) {
+  const int j = lane & 15, g16 = lane >> 4;
+  rb_bf16x8 bh[6], bl[6];
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    int xr = x0 - 1 + u;
+    xr += xr < 0 ? LX : 0;
+    xr -= xr >= LX ? LX : 0;
+    const uint16_t* src = act + (xr * kRcLY + j) * kRbStride + 32 * kh + 8 * g16;
+    int4 vh = *reinterpret_cast<const int4*>(src);
+    int4 vl = *reinterpret_cast<const int4*>(src + N * kRbStride);
+    if (DYI != 1) {
+      constexpr int CTRL = DYI == 0 ? 0x121 : 0x12F;  // dy = -1: lane y <- y - 1; dy = +1: y <- y + 1
+      vh.x = rb_rot<CTRL>(vh.x); vh.y = rb_rot<CTRL>(vh.y); vh.z = rb_rot<CTRL>(vh.z); vh.w = rb_rot<CTRL>(vh.w);
+      vl.x = rb_rot<CTRL>(vl.x); vl.y = rb_rot<CTRL>(vl.y); vl.z = rb_rot<CTRL>(vl.z); vl.w = rb_rot<CTRL>(vl.w);
+    }
+    bh[u] = __builtin_bit_cast(rb_bf16x8, vh);
+    bl[u] = __builtin_bit_cast(rb_bf16x8, vl);
+  }
+#pragma unroll
+  for (int dxi = 0; dxi < 3; ++dxi) {
+    const int tap = dxi * 3 + DYI;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int f = ((tap * 2 + kh) * 4 + qq) * kWave + lane;
+      const rb_bf16x8 wh = Wh[f], wl = Wl[f];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        acc[r][qq] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, bh[r + dxi], acc[r][qq], 0, 0, 0);
+        acc[r][qq] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bl[r + dxi], acc[r][qq], 0, 0, 0);
+        acc[r][qq] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bh[r + dxi], acc[r][qq], 0, 0, 0);
+      }
+    }
+  }
+}
+
+__device__ inline void rb_conv(const uint16_t* act, const rb_bf16x8* __restrict__ Wh, const rb_bf16x8* __restrict__ Wl,
+                               const float* __restrict__ bl, int LX, int N, int x0, int lane, rc_f32x4 (&acc)[4][4]) {
+  const int g16 = lane >> 4;
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+    const float4 b4 = *reinterpret_cast<const float4*>(bl + 16 * qq + 4 * g16);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][qq] = rc_f32x4{b4.x, b4.y, b4.z, b4.w};
+  }
+#pragma unroll 1
+  for (int kh = 0; kh < 2; ++kh) {
+    rb_taps<0>(act, Wh, Wl, LX, N, x0, lane, kh, acc);
+    rb_taps<1>(act, Wh, Wl, LX, N, x0, lane, kh, acc);
+    rb_taps<2>(act, Wh, Wl, LX, N, x0, lane, kh, acc);
+  }
+}
+
+// emb [4][64], Wlp = split-BF16 fragments of the 2 n_blocks convs (hi then lo), Bc [2 n_blocks][64]
+__global__ void __launch_bounds__(256) k_resconv_bf16(int LX, int n_blocks, const float* __restrict__ emb,
+                                                      const rb_bf16x8* __restrict__ Wlp, const float* __restrict__ Bc,
+                                                      float out_scale, const int8_t* __restrict__ s,
+                                                      const int32_t* __restrict__ item_b,
+                                                      const int32_t* __restrict__ item_hop, int img0, int img_end,
+                                                      int ipi, float* __restrict__ Y) {
+  extern __shared__ float lds[];
+  const int N = LX * kRcLY;
+  uint16_t* act = reinterpret_cast<uint16_t*>(lds);  // hi plane [N][72], lo plane [N][72]
+  float* part = lds + N * kRbStride;                 // 2 N * 72 BF16 = N * 72 floats
+  const int img = img0 + blockIdx.x;
+  if (img >= img_end) return;
+  const int item = img / ipi;
+  const int grp = (ipi == kGroup) ? (img & 7) : 0;
+  const int32_t hop = item_hop ? item_hop[item] : kHopNone;
+  if (hop == kHopInvalid) return;
+  const int b = item_b ? item_b[item] : item;
+  const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave, nw = blockDim.x / kWave;
+  const int x0 = 4 * wave, j = lane & 15, g16 = lane >> 4;
+  constexpr int kFrag = 9 * 2 * 4 * kWave;  // fragments of one conv
+  const rb_bf16x8* Wlo = Wlp + (size_t)2 * n_blocks * kFrag;
+
+  rc_f32x4 h[4][4];
+  {
+    const int8_t* sb = s + (size_t)b * 2 * N;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int site = (x0 + r) * kRcLY + j;
+      const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY)) > 0;
+      const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
+      const float* e = emb + (up + 2 * dn) * kRcC + 4 * g16;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const float4 v = *reinterpret_cast<const float4*>(e + 16 * qq);
+        h[r][qq] = rc_f32x4{v.x, v.y, v.z, v.w};
+      }
+    }
+  }
+  const float inv_count = 1.0f / (float)(kRcC * N);
+  for (int blk = 0; blk < n_blocks; ++blk) {
+    float ss = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ss = fmaf(h[r][qq][e], h[r][qq][e], ss);
+    ss = rc_wave_sum(ss);
+    if (lane == 0) part[wave] = ss;
+    __syncthreads();  // partial sums visible; every wave is done reading the previous conv input
+    float tot = 0.f;
+    for (int w = 0; w < nw; ++w) tot += part[w];
+    const float rinv = 1.0f / sqrtf(fmaf(tot, inv_count, kRmsEps));
+    rb_store(act, h, rinv, x0, lane, N);
+    __syncthreads();
+    rc_f32x4 acc[4][4];
+    const size_t c1 = (size_t)(2 * blk) * kFrag;
+    rb_conv(act, Wlp + c1, Wlo + c1, Bc + (2 * blk) * kRcC, LX, N, x0, lane, acc);
+    __syncthreads();  // every wave is done reading the first conv input
+    rb_store(act, acc, 1.0f, x0, lane, N);
+    __syncthreads();
+    rb_conv(act, Wlp + c1 + kFrag, Wlo + c1 + kFrag, Bc + (2 * blk + 1) * kRcC, LX, N, x0, lane, acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
+  }
+  float* yo = Y + (size_t)(img - img0) * (2 * N * (kRcC / 2));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int site = (x0 + r) * kRcLY + j;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const rc_f32x4 v = out_scale * h[r][qq];
+      *reinterpret_cast<float4*>(yo + ((size_t)(qq >> 1) * N + site) * (kRcC / 2) + 16 * (qq & 1) + 4 * g16) =
+          make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+}
+
 }  // namespace
 
 bool resconv_supported(const nh_model* m) {
@@ -213,6 +403,15 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
   const int N = m->LX * kRcLY;
   const size_t lds = sizeof(float) * ((size_t)N * kRcStride + 16);
   const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
+  if (m->precision == NH_PREC_MIXED) {
+    if (!m->conv_w_lp) return NH_ERR_UNSUPPORTED;
+    const size_t lds_bf16 = sizeof(float) * ((size_t)N * kRbStride + 16);
+    hipLaunchKernelGGL(k_resconv_bf16, dim3(nimg), dim3(16 * m->LX), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
+                       reinterpret_cast<const rb_bf16x8*>(m->conv_w_lp), m->conv_b, scale, s, item_b, item_hop, img0,
+                       img_end, ipi, Y);
+    NH_LAUNCH_CHECK();
+    return NH_OK;
+  }
   hipLaunchKernelGGL(k_resconv, dim3(nimg), dim3(16 * m->LX), lds, st, m->LX, m->n_layers, m->conv_w,
                      m->conv_w + kRcClasses * kRcC, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
   NH_LAUNCH_CHECK();

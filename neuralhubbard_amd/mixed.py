@@ -1,4 +1,5 @@
-"""Fragment-ordered conv weights for the MFMA feature network (LY == 16, C == 32).
+"""Fragment-ordered conv weights for the MFMA feature networks (LY == 16; C == 32 for the conv
+stack, C == 64 for the residual CNN, `pack_resconv_bf16`).
 
 FP32 (v_mfma_f32_16x16x4_f32): for layer l, tap, output half hf, K-step kk and lane:
     W[tap][ci = 4 kk + lane // 16][co = 16 hf + lane % 16]
@@ -62,7 +63,29 @@ def pack_bf16(params) -> np.ndarray:
     return np.concatenate([hi, lo])
 
 
+def pack_resconv_bf16(params) -> np.ndarray:
+    """Split-BF16 A fragments of the residual CNN's convolutions (C = 64), weights as the A
+    operand of v_mfma_f32_16x16x32_bf16: for conv, tap, K-half kh, quarter qq, lane, j < 8:
+        W[tap][ci = 32 kh + 8 (lane // 16) + j][co = 16 qq + lane % 16]
+    hi parts of every conv, then lo parts."""
+    W = np.stack([np.asarray(w, np.float32) for w in params.conv_w])  # (n, co, ci, 3, 3)
+    n, Cc = W.shape[0], W.shape[1]
+    T = np.transpose(W, (0, 3, 4, 2, 1)).reshape(n, 9, Cc, Cc)  # [conv][tap][ci][co]
+    kh = np.arange(2)[:, None, None, None]
+    qq = np.arange(4)[None, :, None, None]
+    lane = np.arange(64)[None, None, :, None]
+    j = np.arange(8)[None, None, None, :]
+    ci = 32 * kh + 8 * (lane // 16) + j + 0 * qq  # (2, 4, 64, 8)
+    co = 16 * qq + lane % 16 + 0 * kh + 0 * j
+    frag = T[:, :, ci, co].reshape(-1)  # (n, 9, 2, 4, 64, 8)
+    hi = bf16_bits(frag)
+    lo = bf16_bits(frag - (hi.astype(np.uint32) << 16).view(np.float32))
+    return np.concatenate([hi, lo])
+
+
 def pack_lowp(model):
     import torch
 
+    if model.workload.network == "resconv":
+        return torch.as_tensor(pack_resconv_bf16(model.params).view(np.int16), device="cuda")
     return torch.as_tensor(pack_bf16(model.params).view(np.int16), device="cuda")

@@ -52,9 +52,11 @@ def test_residual_cnn_model_checks():
                 _lib.NhModel(0, 16, 24, 168, 168, 64, 16, 0, 1, R, 16, 16, 16, 0, 0),   # LY != 16
                 _lib.NhModel(0, 20, 16, 140, 140, 64, 16, 0, 1, R, 16, 16, 16, 0, 0),   # LX > 16
                 _lib.NhModel(0, 16, 16, 112, 112, 64, 0, 0, 1, R, 16, 16, 16, 0, 0),    # no blocks
-                _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 1, 1, R, 16, 16, 16, 16, 16),  # mixed
+                _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 1, 1, R, 16, 16, 16, 0, 0),    # mixed, no BF16 copy
                 _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 0, 1, 7, 16, 16, 16, 0, 0)):   # network
         assert lib.nh_log_psi_workspace_size(ctypes.byref(bad), 64) == 0
+    mixed = _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 1, 1, R, 16, 16, 16, 0, 16)
+    assert lib.nh_log_psi_workspace_size(ctypes.byref(mixed), 64) > 0
 
 
 def test_pack_resconv_layout():
@@ -67,6 +69,24 @@ def test_pack_resconv_layout():
     # conv 3 (second conv of block 1), tap (dx, dy) = (0, +1), c_in 9, c_out 40
     assert w[4 * C + 3 * 9 * C * C + ((1 * 3 + 2) * C + 9) * C + 40] == p.conv_w[3][40, 9, 1, 2]
     assert b[3 * C + 40] == p.conv_b[3][40]
+
+
+def test_pack_resconv_bf16_layout():
+    from neuralhubbard_amd.mixed import pack_resconv_bf16
+
+    wk = small_workload("pfaffian", LX=4, LY=16, n_up=12, n_dn=12, channels=64, n_layers=1, network="resconv")
+    p = make_params(wk)
+    packed = pack_resconv_bf16(p)
+    nf = 2 * 9 * 2 * 4 * 64 * 8  # two convs
+    assert packed.size == 2 * nf
+    hi = (packed[:nf].astype(np.uint32) << 16).view(np.float32).reshape(2, 9, 2, 4, 64, 8)
+    lo = (packed[nf:].astype(np.uint32) << 16).view(np.float32).reshape(2, 9, 2, 4, 64, 8)
+    # conv 1, tap (dx, dy) = (+1, -1), kh 1, quarter 2, lane 37, j 5 -> ci = 32 + 16 + 5, co = 32 + 5
+    w = p.conv_w[1][37, 53, 2, 0]
+    assert abs(hi[1, 6, 1, 2, 37, 5] + lo[1, 6, 1, 2, 37, 5] - w) <= 2.0**-16 * abs(w)
+    full = np.stack([np.transpose(c, (2, 3, 1, 0)).reshape(9, 64, 64) for c in p.conv_w])
+    rec = hi + lo
+    assert np.all(np.abs(rec[:, :, 0, 0, 0, 0] - full[:, :, 0, 0]) <= 2.0**-16 * np.abs(full[:, :, 0, 0]))
 
 
 def test_argument_errors_without_gpu():
