@@ -1,10 +1,12 @@
 """Accepted-sample throughput of the 16x16 Pfaffian backflow path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--sweep P]
-                    [--form both|plain|mixed-da] [--no-cpu]
+                    [--form both|plain|mixed-da] [--no-cpu] [--workload pf16|tpm02_L16]
 
 Workload: the 16x16 Pfaffian backflow (t' = -0.2, U = 8, 112 + 112 electrons, 224 x 224
-Pfaffians, 6-layer 32-channel conv, 8-image projection) with seeded random-init weights.  C
+Pfaffians, 6-layer 32-channel conv, 8-image projection) with seeded random-init weights
+(``--workload tpm02_L16``: the released tpm02_L16_pfaffian state instead, 16-block 64-channel
+residual CNN, chains started from its released chains, no CPU leg).  C
 Metropolis chains live in HBM.  One step = one sweep of P exchange proposals per chain, then
 log psi_P and E_loc (every connected configuration evaluated in full) of every chain's state:
 C measured samples per step.  Two forms run from the same thermalised chains:
@@ -61,6 +63,12 @@ def parse():
     p.add_argument("--chunk", type=int, default=16, help="E_loc samples per chunk")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 checks")
     p.add_argument("--cpu-samples", type=int, default=12)
+    p.add_argument("--workload", default="pf16", choices=["pf16", "tpm02_L16"],
+                   help="pf16: the synthetic 6-layer 32-channel workload (default); tpm02_L16: the "
+                        "released tpm02_L16_pfaffian state (16-block 64-channel residual CNN) from --weights, "
+                        "chains started from its released chains")
+    p.add_argument("--weights", default=os.path.join(ROOT, "tests", "golden", "tpm02_L16_pfaffian.npz"),
+                   help="parameters and chains of the released state (written by tests/golden/make_released.py)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                    help="only copy BYTES between two device buffers (reference dispatch for HBM counters)")
     return p.parse_args()
@@ -146,9 +154,26 @@ def main():
     from neuralhubbard_amd.sampler import MetropolisSampler, Surrogate
 
     lib = _lib.load()
-    wk = WORKLOADS["pf16"]
-    params = make_params(wk)
     C = args.chains
+    if args.workload == "tpm02_L16":
+        from neuralhubbard_amd.configs import Params
+        from neuralhubbard_amd.released import STATES, unpack_spins
+
+        wk = STATES["tpm02_L16_pfaffian"].workload()
+        with np.load(args.weights, allow_pickle=False) as f:
+            params = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, None, f["emb"])
+            released = unpack_spins(f["bits"], f["shape"])
+        start0 = np.resize(released, (C, 2 * wk.N))
+        data = "released tpm02_L16_pfaffian parameters; chains started from its released equilibrated chains"
+        net = "16-block 64-channel residual CNN (released tpm02_L16_pfaffian)"
+    else:
+        wk = WORKLOADS["pf16"]
+        params = make_params(wk)
+        start0 = make_configurations(wk, C)
+        data = ("synthetic: seeded random configurations thermalised by the path's own Metropolis chains, "
+                "random-init weights of the workload's architecture")
+        net = "6-layer 32-channel conv"
+    resconv = wk.network == "resconv"
     forms = ["plain", "mixed-da"] if args.form == "both" else [args.form]
     states = {"plain": Variational(build_backflow(wk, params, "fp32")),
               "mixed-da": Variational(build_backflow(wk, params, "mixed"))}
@@ -157,7 +182,7 @@ def main():
     # thermalise with exact plain Metropolis on the mixed-precision amplitude and record the
     # (configuration, exact log|psi|) pairs the surrogate learns from
     t0 = time.perf_counter()
-    rec = MetropolisSampler(states["mixed-da"], make_configurations(wk, C), args.record_gap, seed=1,
+    rec = MetropolisSampler(states["mixed-da"], start0, args.record_gap, seed=1,
                             thermal_steps=args.thermal)
     X, Y = [], []
     for _ in range(args.records):
@@ -233,8 +258,9 @@ def main():
     launches, ms = r["kinds"][dom]
     achieved = flops[dom] / (ms * 1e-3)
     arith = "bf16" if (head == "mixed-da" and dom == "features") else "fp32"
-    traffic = read_traffic(images / max(1, launches)) if dom == "pairing+pfaffian" else None
-    roof = {"bound": "mfma", "kernel": "k_image_pf" if dom == "pairing+pfaffian" else "k_cnn_mfma",
+    traffic = read_traffic(images / max(1, launches)) if (dom == "pairing+pfaffian" and not resconv) else None
+    cnn_kernel = ("k_resconv_bf16" if arith == "bf16" else "k_resconv") if resconv else "k_cnn_mfma"
+    roof = {"bound": "mfma", "kernel": "k_image_pf" if dom == "pairing+pfaffian" else cnn_kernel,
             "achieved": achieved / 1e12, "peak": PEAK[arith] / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK[arith], "traffic": traffic, "launches": launches,
             "avg_launch_ms": ms / max(1, launches), "flops_per_launch": flops[dom] / max(1, launches),
@@ -254,10 +280,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16+fp32" if head == "mixed-da" else "fp32",
-        "data": "synthetic: seeded random configurations thermalised by the path's own Metropolis chains, "
-                "random-init weights of the workload's architecture",
+        "data": data,
         "config": {"workload": f"16x16 Pfaffian backflow, t'=-0.2, U=8, 112+112 electrons (224x224 Pfaffians), "
-                               f"6-layer 32-channel conv, 8-image projection; {C} chains, {args.sweep} exchange "
+                               f"{net}, 8-image projection; {C} chains, {args.sweep} exchange "
                                f"proposals per chain per step, then log psi_P + full E_loc of every chain",
                    "form": head, "chains": C, "sweep": args.sweep,
                    "connected_per_sample": r["n_conn"] / (args.steps * C), "parallelism": "single GPU"},
@@ -299,7 +324,10 @@ def main():
         sur_info["rms_log_ratio_err"] = float(np.sqrt(np.mean((d_sur[ok] - d_exact[ok]) ** 2)))
         sur_info["rms_exact_log_ratio"] = float(np.sqrt(np.mean(d_exact[ok] ** 2)))
         out["surrogate"] = sur_info
-    if not args.no_cpu:
+    if resconv and not args.no_cpu:
+        log("CPU leg skipped for the released state: one sample's E_loc takes minutes on the host; "
+            "its FP64 parity is checked by tests/test_gpu_released.py")
+    elif not args.no_cpu:
         n = max(1, min(args.cpu_samples, C))
         x = results[head]["sampler"].s[:n]
         gpu_psi, gpu_eloc = {}, {}

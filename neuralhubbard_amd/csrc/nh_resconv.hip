@@ -208,14 +208,23 @@ __global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const flo
 // result (lane l, register e -> channel 16 quarter + 4 (l / 16) + e, site y = l % 16) gives each
 // lane four consecutive channels of one site, stored to LDS as one 8-byte hi and one 8-byte lo
 // write.  Activations: two planes (hi, lo) of [site][64 + 8 pad] BF16; a lane's B fragment is
-// the 8 channels 32 kh + 8 (l / 16) .. + 7 of its site, one 16-byte read per plane.  Weights
-// arrive in fragment order (neuralhubbard_amd/mixed.py, pack_resconv_bf16): for conv, tap,
-// K-half kh, quarter qq and lane l, 8 BF16 = W[tap][32 kh + 8 (l / 16) + e][16 qq + l % 16];
-// all hi fragments, then all lo fragments.
+// the 8 channels 32 kh + 8 (l / 16) .. + 7 of its site, one 16-byte read per plane.
+//
+// Weights stream through LDS in slabs: a slab is one (conv, K-half kh, tap row dyi), i.e. the
+// three taps dxi = 0..2 for all four quarters, hi then lo (24 KB).  The workgroup loads slab
+// k + 1 from global memory into registers while its waves run the MFMAs of slab k from LDS,
+// then writes it to the other LDS buffer; one barrier per slab.  Every weight byte is read from
+// global memory once per image (not once per wave) and its latency hides behind 144 MFMAs per
+// wave.  Host packing (neuralhubbard_amd/mixed.py, pack_resconv_bf16): slab-major
+// [conv][kh][dyi][plane][dxi][qq][lane][8 BF16] with W[tap = 3 dxi + dyi][32 kh + 8 (l / 16) + e]
+// [16 qq + l % 16].
 
 typedef short rb_bf16x8 __attribute__((ext_vector_type(8)));
+typedef int rb_i32x4 __attribute__((ext_vector_type(4)));  // one 16-byte fragment
 
-constexpr int kRbStride = kRcC + 8;  // BF16 per site per plane (144 bytes: conflict-free reads)
+constexpr int kRbStride = kRcC + 8;            // BF16 per site per plane (144 bytes: conflict-free reads)
+constexpr int kSlabFrag = 3 * 4 * kWave;       // 16-byte fragments per plane of one slab
+constexpr int kSlabsPerConv = 6;               // 2 K-halves x 3 tap rows
 
 __device__ inline uint32_t rb_bf16(float x) {
   uint32_t u = __float_as_uint(x);
@@ -253,9 +262,17 @@ __device__ inline void rb_store(uint16_t* act, const rc_f32x4 (&v)[4][4], float 
     }
 }
 
+// fragments per thread when the workgroup copies one slab (hi + lo planes)
+template <int NT>
+constexpr int rb_per_thread() {
+  static_assert(2 * kSlabFrag % NT == 0, "threads must divide the slab");
+  return 2 * kSlabFrag / NT;
+}
+
+// MFMAs of one slab (tap row DYI, K-half kh) on rows x0 .. x0+3; wb = the slab in LDS
 template <int DYI>
-__device__ inline void rb_taps(const uint16_t* act, const rb_bf16x8* __restrict__ Wh, const rb_bf16x8* __restrict__ Wl,
-                               int LX, int N, int x0, int lane, int kh, rc_f32x4 (&acc)[4][4]) {
+__device__ __forceinline__ void rb_taps(const uint16_t* act, const rb_i32x4* wb, int LX, int N, int x0, int lane, int kh,
+                               rc_f32x4 (&acc)[4][4]) {
   const int j = lane & 15, g16 = lane >> 4;
   rb_bf16x8 bh[6], bl[6];
 #pragma unroll
@@ -276,11 +293,11 @@ __device__ inline void rb_taps(const uint16_t* act, const rb_bf16x8* __restrict_
   }
 #pragma unroll
   for (int dxi = 0; dxi < 3; ++dxi) {
-    const int tap = dxi * 3 + DYI;
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) {
-      const int f = ((tap * 2 + kh) * 4 + qq) * kWave + lane;
-      const rb_bf16x8 wh = Wh[f], wl = Wl[f];
+      const int f = (dxi * 4 + qq) * kWave + lane;
+      const rb_bf16x8 wh = __builtin_bit_cast(rb_bf16x8, wb[f]);
+      const rb_bf16x8 wl = __builtin_bit_cast(rb_bf16x8, wb[kSlabFrag + f]);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         acc[r][qq] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, bh[r + dxi], acc[r][qq], 0, 0, 0);
@@ -291,9 +308,35 @@ __device__ inline void rb_taps(const uint16_t* act, const rb_bf16x8* __restrict_
   }
 }
 
-__device__ inline void rb_conv(const uint16_t* act, const rb_bf16x8* __restrict__ Wh, const rb_bf16x8* __restrict__ Wl,
-                               const float* __restrict__ bl, int LX, int N, int x0, int lane, rc_f32x4 (&acc)[4][4]) {
-  const int g16 = lane >> 4;
+// One slab step: start loading slab k + 1 into registers, run slab k from LDS buffer k & 1,
+// park slab k + 1 in the other buffer, barrier.  The barrier also ends every wave's reads of
+// the conv input.
+template <int NT, int DYI>
+__device__ __forceinline__ void rb_step(const uint16_t* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
+                                        int n_slabs, int LX, int N, int x0, int t, int kh, rc_f32x4 (&acc)[4][4]) {
+  constexpr int kPer = rb_per_thread<NT>();
+  const bool more = k + 1 < n_slabs;
+  rb_i32x4 v[kPer];
+  if (more) {
+    const rb_i32x4* src = Wsl + (size_t)(k + 1) * 2 * kSlabFrag;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) v[i] = src[t + i * NT];
+  }
+  rb_taps<DYI>(act, wbuf + (k & 1) * 2 * kSlabFrag, LX, N, x0, t & (kWave - 1), kh, acc);
+  if (more) {
+    rb_i32x4* dst = wbuf + ((k + 1) & 1) * 2 * kSlabFrag;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) dst[t + i * NT] = v[i];
+  }
+  __syncthreads();
+  ++k;
+}
+
+template <int NT>
+__device__ __forceinline__ void rb_conv(const uint16_t* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
+                                        int n_slabs, const float* __restrict__ bl, int LX, int N, int x0, int t,
+                                        rc_f32x4 (&acc)[4][4]) {
+  const int g16 = (t & (kWave - 1)) >> 4;
 #pragma unroll
   for (int qq = 0; qq < 4; ++qq) {
     const float4 b4 = *reinterpret_cast<const float4*>(bl + 16 * qq + 4 * g16);
@@ -302,23 +345,29 @@ __device__ inline void rb_conv(const uint16_t* act, const rb_bf16x8* __restrict_
   }
 #pragma unroll 1
   for (int kh = 0; kh < 2; ++kh) {
-    rb_taps<0>(act, Wh, Wl, LX, N, x0, lane, kh, acc);
-    rb_taps<1>(act, Wh, Wl, LX, N, x0, lane, kh, acc);
-    rb_taps<2>(act, Wh, Wl, LX, N, x0, lane, kh, acc);
+    rb_step<NT, 0>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, t, kh, acc);
+    rb_step<NT, 1>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, t, kh, acc);
+    rb_step<NT, 2>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, t, kh, acc);
   }
 }
 
-// emb [4][64], Wlp = split-BF16 fragments of the 2 n_blocks convs (hi then lo), Bc [2 n_blocks][64]
-__global__ void __launch_bounds__(256) k_resconv_bf16(int LX, int n_blocks, const float* __restrict__ emb,
-                                                      const rb_bf16x8* __restrict__ Wlp, const float* __restrict__ Bc,
-                                                      float out_scale, const int8_t* __restrict__ s,
-                                                      const int32_t* __restrict__ item_b,
-                                                      const int32_t* __restrict__ item_hop, int img0, int img_end,
-                                                      int ipi, float* __restrict__ Y) {
+__host__ __device__ constexpr size_t rb_lds_floats(int N) {
+  return (size_t)N * kRbStride + 16 * kSlabFrag + 16;  // act planes, two slabs (2 x 2 planes of rb_i32x4), partials
+}
+
+// emb [4][64], Wsl = split-BF16 slabs of the 2 n_blocks convs, Bc [2 n_blocks][64]
+template <int NT>
+__global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const float* __restrict__ emb,
+                                                     const rb_i32x4* __restrict__ Wsl, const float* __restrict__ Bc,
+                                                     float out_scale, const int8_t* __restrict__ s,
+                                                     const int32_t* __restrict__ item_b,
+                                                     const int32_t* __restrict__ item_hop, int img0, int img_end,
+                                                     int ipi, float* __restrict__ Y) {
   extern __shared__ float lds[];
   const int N = LX * kRcLY;
-  uint16_t* act = reinterpret_cast<uint16_t*>(lds);  // hi plane [N][72], lo plane [N][72]
-  float* part = lds + N * kRbStride;                 // 2 N * 72 BF16 = N * 72 floats
+  uint16_t* act = reinterpret_cast<uint16_t*>(lds);                    // hi plane [N][72], lo plane [N][72]
+  rb_i32x4* wbuf = reinterpret_cast<rb_i32x4*>(lds + (size_t)N * kRbStride);  // 2 slabs
+  float* part = lds + (size_t)N * kRbStride + 16 * kSlabFrag;
   const int img = img0 + blockIdx.x;
   if (img >= img_end) return;
   const int item = img / ipi;
@@ -326,10 +375,15 @@ __global__ void __launch_bounds__(256) k_resconv_bf16(int LX, int n_blocks, cons
   const int32_t hop = item_hop ? item_hop[item] : kHopNone;
   if (hop == kHopInvalid) return;
   const int b = item_b ? item_b[item] : item;
-  const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave, nw = blockDim.x / kWave;
+  const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+  constexpr int nw = NT / kWave;
   const int x0 = 4 * wave, j = lane & 15, g16 = lane >> 4;
-  constexpr int kFrag = 9 * 2 * 4 * kWave;  // fragments of one conv
-  const rb_bf16x8* Wlo = Wlp + (size_t)2 * n_blocks * kFrag;
+  const int n_slabs = 2 * n_blocks * kSlabsPerConv;
+
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < rb_per_thread<NT>(); ++i) wbuf[t + i * NT] = Wsl[t + i * NT];  // slab 0, visible after
+                                                                                     // the first barrier
 
   rc_f32x4 h[4][4];
   {
@@ -358,19 +412,18 @@ __global__ void __launch_bounds__(256) k_resconv_bf16(int LX, int n_blocks, cons
         for (int e = 0; e < 4; ++e) ss = fmaf(h[r][qq][e], h[r][qq][e], ss);
     ss = rc_wave_sum(ss);
     if (lane == 0) part[wave] = ss;
-    __syncthreads();  // partial sums visible; every wave is done reading the previous conv input
+    __syncthreads();  // partial sums (and, first time, slab 0) visible
     float tot = 0.f;
+#pragma unroll
     for (int w = 0; w < nw; ++w) tot += part[w];
     const float rinv = 1.0f / sqrtf(fmaf(tot, inv_count, kRmsEps));
     rb_store(act, h, rinv, x0, lane, N);
     __syncthreads();
     rc_f32x4 acc[4][4];
-    const size_t c1 = (size_t)(2 * blk) * kFrag;
-    rb_conv(act, Wlp + c1, Wlo + c1, Bc + (2 * blk) * kRcC, LX, N, x0, lane, acc);
-    __syncthreads();  // every wave is done reading the first conv input
-    rb_store(act, acc, 1.0f, x0, lane, N);
+    rb_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk) * kRcC, LX, N, x0, t, acc);
+    rb_store(act, acc, 1.0f, x0, lane, N);  // the last slab barrier ended every read of act
     __syncthreads();
-    rb_conv(act, Wlp + c1 + kFrag, Wlo + c1 + kFrag, Bc + (2 * blk + 1) * kRcC, LX, N, x0, lane, acc);
+    rb_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk + 1) * kRcC, LX, N, x0, t, acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -405,10 +458,25 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
   const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
   if (m->precision == NH_PREC_MIXED) {
     if (!m->conv_w_lp) return NH_ERR_UNSUPPORTED;
-    const size_t lds_bf16 = sizeof(float) * ((size_t)N * kRbStride + 16);
-    hipLaunchKernelGGL(k_resconv_bf16, dim3(nimg), dim3(16 * m->LX), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                       reinterpret_cast<const rb_bf16x8*>(m->conv_w_lp), m->conv_b, scale, s, item_b, item_hop, img0,
-                       img_end, ipi, Y);
+    const size_t lds_bf16 = sizeof(float) * rb_lds_floats(N);
+    const rb_i32x4* wsl = reinterpret_cast<const rb_i32x4*>(m->conv_w_lp);
+    switch (m->LX) {
+      case 4:
+        hipLaunchKernelGGL(k_resconv_bf16<64>, dim3(nimg), dim3(64), lds_bf16, st, m->LX, m->n_layers, m->conv_w, wsl,
+                           m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+        break;
+      case 8:
+        hipLaunchKernelGGL(k_resconv_bf16<128>, dim3(nimg), dim3(128), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
+                           wsl, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+        break;
+      case 12:
+        hipLaunchKernelGGL(k_resconv_bf16<192>, dim3(nimg), dim3(192), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
+                           wsl, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+        break;
+      default:
+        hipLaunchKernelGGL(k_resconv_bf16<256>, dim3(nimg), dim3(256), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
+                           wsl, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+    }
     NH_LAUNCH_CHECK();
     return NH_OK;
   }

@@ -64,23 +64,28 @@ def pack_bf16(params) -> np.ndarray:
 
 
 def pack_resconv_bf16(params) -> np.ndarray:
-    """Split-BF16 A fragments of the residual CNN's convolutions (C = 64), weights as the A
-    operand of v_mfma_f32_16x16x32_bf16: for conv, tap, K-half kh, quarter qq, lane, j < 8:
-        W[tap][ci = 32 kh + 8 (lane // 16) + j][co = 16 qq + lane % 16]
-    hi parts of every conv, then lo parts."""
+    """Split-BF16 weights of the residual CNN's convolutions (C = 64) as A fragments of
+    v_mfma_f32_16x16x32_bf16, in slabs: for conv, K-half kh, tap row dyi, plane (hi, lo),
+    tap column dxi, quarter qq, lane, j < 8:
+        W[tap = 3 dxi + dyi][ci = 32 kh + 8 (lane // 16) + j][co = 16 qq + lane % 16]
+    (plane lo = W - hi rounded).  One slab (kh, dyi) is 2 x 3 x 4 x 64 fragments of 16 bytes."""
     W = np.stack([np.asarray(w, np.float32) for w in params.conv_w])  # (n, co, ci, 3, 3)
     n, Cc = W.shape[0], W.shape[1]
     T = np.transpose(W, (0, 3, 4, 2, 1)).reshape(n, 9, Cc, Cc)  # [conv][tap][ci][co]
-    kh = np.arange(2)[:, None, None, None]
-    qq = np.arange(4)[None, :, None, None]
-    lane = np.arange(64)[None, None, :, None]
-    j = np.arange(8)[None, None, None, :]
-    ci = 32 * kh + 8 * (lane // 16) + j + 0 * qq  # (2, 4, 64, 8)
-    co = 16 * qq + lane % 16 + 0 * kh + 0 * j
-    frag = T[:, :, ci, co].reshape(-1)  # (n, 9, 2, 4, 64, 8)
-    hi = bf16_bits(frag)
-    lo = bf16_bits(frag - (hi.astype(np.uint32) << 16).view(np.float32))
-    return np.concatenate([hi, lo])
+    kh = np.arange(2).reshape(2, 1, 1, 1, 1, 1)
+    dyi = np.arange(3).reshape(1, 3, 1, 1, 1, 1)
+    dxi = np.arange(3).reshape(1, 1, 3, 1, 1, 1)
+    qq = np.arange(4).reshape(1, 1, 1, 4, 1, 1)
+    lane = np.arange(64).reshape(1, 1, 1, 1, 64, 1)
+    j = np.arange(8).reshape(1, 1, 1, 1, 1, 8)
+    shape = (2, 3, 3, 4, 64, 8)
+    tap = np.broadcast_to(3 * dxi + dyi, shape)
+    ci = np.broadcast_to(32 * kh + 8 * (lane // 16) + j, shape)
+    co = np.broadcast_to(16 * qq + lane % 16, shape)
+    frag = T[:, tap, ci, co]  # (n, kh, dyi, dxi, qq, lane, j)
+    hi = bf16_bits(frag.reshape(-1)).reshape(frag.shape)
+    lo = bf16_bits((frag.reshape(-1) - (hi.reshape(-1).astype(np.uint32) << 16).view(np.float32))).reshape(frag.shape)
+    return np.stack([hi, lo], axis=3).reshape(-1)  # (n, kh, dyi, plane, dxi, qq, lane, j)
 
 
 def pack_lowp(model):

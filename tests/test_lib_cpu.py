@@ -77,16 +77,19 @@ def test_pack_resconv_bf16_layout():
     wk = small_workload("pfaffian", LX=4, LY=16, n_up=12, n_dn=12, channels=64, n_layers=1, network="resconv")
     p = make_params(wk)
     packed = pack_resconv_bf16(p)
-    nf = 2 * 9 * 2 * 4 * 64 * 8  # two convs
-    assert packed.size == 2 * nf
-    hi = (packed[:nf].astype(np.uint32) << 16).view(np.float32).reshape(2, 9, 2, 4, 64, 8)
-    lo = (packed[nf:].astype(np.uint32) << 16).view(np.float32).reshape(2, 9, 2, 4, 64, 8)
-    # conv 1, tap (dx, dy) = (+1, -1), kh 1, quarter 2, lane 37, j 5 -> ci = 32 + 16 + 5, co = 32 + 5
+    assert packed.size == 2 * 6 * 2 * 3 * 4 * 64 * 8  # two convs of six slabs, hi and lo planes
+    f = (packed.astype(np.uint32) << 16).view(np.float32).reshape(2, 2, 3, 2, 3, 4, 64, 8)
+    rec = f[:, :, :, 0] + f[:, :, :, 1]  # hi + lo
+    # conv 1, kh 1, tap row dyi 0 (dy = -1), tap column dxi 2 (dx = +1), quarter 2, lane 37, j 5:
+    # ci = 32 + 8 * 2 + 5 = 53, co = 32 + 5 = 37
     w = p.conv_w[1][37, 53, 2, 0]
-    assert abs(hi[1, 6, 1, 2, 37, 5] + lo[1, 6, 1, 2, 37, 5] - w) <= 2.0**-16 * abs(w)
-    full = np.stack([np.transpose(c, (2, 3, 1, 0)).reshape(9, 64, 64) for c in p.conv_w])
-    rec = hi + lo
-    assert np.all(np.abs(rec[:, :, 0, 0, 0, 0] - full[:, :, 0, 0]) <= 2.0**-16 * np.abs(full[:, :, 0, 0]))
+    assert abs(rec[1, 1, 0, 2, 2, 37, 5] - w) <= 2.0**-16 * abs(w)
+    full = np.stack([np.transpose(c, (2, 3, 1, 0)).reshape(9, 64, 64) for c in p.conv_w])  # [conv][tap][ci][co]
+    lane = np.arange(64)
+    for kh, dyi, dxi, qq in ((0, 1, 1, 0), (1, 2, 0, 3)):
+        want = full[:, 3 * dxi + dyi][:, 32 * kh + 8 * (lane // 16)[:, None] + np.arange(8)[None, :],
+                                      (16 * qq + lane % 16)[:, None]]
+        assert np.all(np.abs(rec[:, kh, dyi, dxi, qq] - want) <= 2.0**-16 * np.abs(want))
 
 
 def test_argument_errors_without_gpu():
