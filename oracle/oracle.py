@@ -26,6 +26,13 @@ Pinning status per function (see the tests named in brackets):
                             strings over the full Hilbert space       [test_oracle_eloc]
   philox4x32                published known-answer vectors           [test_oracle_philox]
   jackknife / energy        closed forms for i.i.d. blocks           [test_oracle_jackknife]
+                            the reference's published Table S4 values
+                            from its own energy series (17 states)   [test_energy_series]
+  resconv_features          translation equivariance, embedding order
+  (+ rmsnorm, conv batch)   with zero convs, RMSNorm closed form, the
+                            scipy-pinned conv3x3; the composite by the
+                            released state's E/N against the paper
+                            (a swapped embedding fails it)           [test_released, test_gpu_released]
   sur_forward / sur_loss    the same network in torch conv2d (circular
                             padding), translation invariance, f = N b3
                             for zero weights                         [test_oracle_surrogate_forward]
