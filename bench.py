@@ -220,19 +220,21 @@ def main():
             e = H.Oloc(st, smp)
             return smp, e
 
-        for _ in range(args.warmup):
+        for i in range(args.warmup):
             step()
+            log(f"{f}: warmup step {i + 1}/{args.warmup} done")
         sm.reset_stats()
         torch.cuda.synchronize()
         lib.nh_profile_reset()
         lib.nh_profile_enable(1)
         e_mean, e_sqr, spins = [], [], []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for i in range(args.steps):
             smp, e = step()
             e_mean.append(e.mean())
             e_sqr.append((e * e).mean())
             spins.append(smp.spins)
+            log(f"{f}: step {i + 1}/{args.steps} issued")
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         lib.nh_profile_enable(0)

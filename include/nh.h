@@ -100,6 +100,25 @@ int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A, float* sig
  * A [batch][n][n], n <= 192.  LU with partial pivoting, FP32, logs summed in FP64. */
 int nh_slogdet(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs);
 
+/* ---- f2: reverse mode of the occupied pairing block
+ *      (replaces the backward of pf_kernel.pairing_occ's custom_vjp, code/backflow/pf_kernel.py:354-365,
+ *      kernels _dy_kernel :193-213 and _dw_kernel :239-290)
+ * For the cotangent G [batch][n][n] of F = nh_pairing_occ(Y, idx, Wt), per sample:
+ *   dY[b,p,a]      = sum_q (G[p,q] - G[q,p]) Wt[2 s_p + s_q][r(p,q)][a] Y[b,q,a]
+ *   dWt[b][k][r][a] = sum over (p, q) with 2 s_p + s_q = k, r(p,q) = r of G[p,q] Y[p,a] Y[q,a].
+ * dY [batch][n][d], dWt [batch][4][N][d] (device; either may be NULL to skip it).  Wt must have
+ * the antisymmetry of build_wtilde (Wt[2 s_q + s_p][-r] = -Wt[2 s_p + s_q][r]).  d divides 256
+ * and d <= 64, N <= 1024, batch <= 65535; a mode outside [0, 2N) gives NaN rows of dY. */
+int nh_pairing_occ_vjp(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch, int32_t n,
+                       const float* Y, const int32_t* idx, const float* Wt, const float* G, float* dY,
+                       float* dWt);
+
+/* ---- f2: Pfaffian VJP (the reverse mode of fermix.slogpf, code/backflow/models.py:210)
+ * G[b] = cot[b] * d log|Pf F[b]| / dF[b] = cot[b] * (1/2) F[b]^{-T}, every entry of F taken as
+ * independent; cot NULL means 1.  F, G [batch][n][n] (device, distinct buffers), n <= 1024.
+ * Gauss-Jordan with partial pivoting in FP32 on G itself; a singular F gives NaN. */
+int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G);
+
 /* ---- a4: backflow features  (replaces PfaffianBackflow.features, code/backflow/models.py:163-167)
  * s int8 [batch][2N] -> y float [batch][2][N][d]  (no projection; the identity image). */
 int nh_features(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* y);

@@ -61,6 +61,8 @@ SIGNATURES = {
     "nh_pairing_ud": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "nh_slogpf": (_I, [_P, _I, _I, _P, _P, _P]),
     "nh_slogdet": (_I, [_P, _I, _I, _P, _P, _P]),
+    "nh_pairing_occ_vjp": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "nh_slogpf_vjp": (_I, [_P, _I, _I, _P, _P, _P]),
     "nh_features": (_I, [_P, _MP, _I, _P, _P]),
     "nh_log_psi_workspace_size": (_S, [_MP, _I]),
     "nh_log_psi": (_I, [_P, _MP, _I, _P, _P, _P, _P, _S]),

@@ -10,6 +10,8 @@ reference                                                  here (all device work
 ``pf_kernel.pairing_occ`` (``pf_kernel.py:335-351``)        `pairing_occ` -> ``nh_pairing_occ``
 ``bcs_kernel.pairing_ud`` (``bcs_kernel.py:257-271``)       `pairing_ud` -> ``nh_pairing_ud``
 ``fermix.slogpf`` / ``slogdet`` (``models.py:210, 297``)    `slogpf` / `slogdet` -> ``nh_slogpf``...
+``pairing_occ`` backward (``pf_kernel.py:354-365``)          `pairing_occ_vjp` -> ``nh_pairing_occ_vjp``
+``fermix.slogpf`` reverse mode                              `slogpf_vjp` -> ``nh_slogpf_vjp``
 ``PfaffianBackflow`` / ``BCSBackflow`` (``models.py``)      `PfaffianBackflow` / `BCSBackflow`
 ``quantax.state.Variational`` (``states.py:318-320``)       `Variational` -> ``nh_log_psi``
 ``quantax.operator.Hubbard.Oloc`` (``quick_start.py:33``)   `Hubbard.Oloc` -> ``nh_local_energy``
@@ -166,6 +168,38 @@ def slogdet(F):
     _lib.check(_lib.load().nh_slogdet(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs)),
                "nh_slogdet")
     return sign, logabs
+
+
+def pairing_occ_vjp(Y, idx, Wt, G, LX: int, LY: int):
+    """(dY (B, n, d), dWt (B, 4, N, d)): the VJP of `pairing_occ` for the cotangent G (B, n, n),
+    per sample (the backward of ``pairing_occ``'s custom_vjp, ``pf_kernel.py:354-365``)."""
+    torch = _torch()
+    _lib.require_gpu()
+    Y = Y.contiguous().float()
+    idx = idx.contiguous().to(torch.int32)
+    Wt = Wt.contiguous().float()
+    G = G.contiguous().float()
+    B, n, d = Y.shape
+    N = LX * LY
+    dY = torch.empty((B, n, d), dtype=torch.float32, device=Y.device)
+    dWt = torch.empty((B, 4, N, d), dtype=torch.float32, device=Y.device)
+    _lib.check(_lib.load().nh_pairing_occ_vjp(_lib.stream_handle(), LX, LY, d, B, n, _lib.ptr(Y), _lib.ptr(idx),
+                                              _lib.ptr(Wt), _lib.ptr(G), _lib.ptr(dY), _lib.ptr(dWt)),
+               "nh_pairing_occ_vjp")
+    return dY, dWt
+
+
+def slogpf_vjp(F, cot=None):
+    """cot * d log|Pf F| / dF = cot * F^{-T} / 2 for antisymmetric (B, n, n) float32."""
+    torch = _torch()
+    _lib.require_gpu()
+    F = F.contiguous().float()
+    B, n, _ = F.shape
+    G = torch.empty_like(F)
+    c = None if cot is None else torch.as_tensor(cot, dtype=torch.float32, device=F.device).contiguous()
+    _lib.check(_lib.load().nh_slogpf_vjp(_lib.stream_handle(), B, n, _lib.ptr(F), 0 if c is None else _lib.ptr(c),
+                                         _lib.ptr(G)), "nh_slogpf_vjp")
+    return G
 
 
 # ------------------------------------------------------------------------ models

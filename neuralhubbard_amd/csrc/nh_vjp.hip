@@ -1,0 +1,244 @@
+// Reverse-mode pieces of the amplitude (gfx950): the VJP of the occupied pairing block and the
+// Pfaffian VJP.
+//
+// nh_pairing_occ_vjp replaces the backward of the reference's pairing_occ custom_vjp
+// (code/backflow/pf_kernel.py:354-365): for the cotangent G of F = pairing_occ(Y, idx, Wt),
+//   dY[p, a]      = sum_q (G[p, q] - G[q, p]) Wt[2 s_p + s_q][r(p, q)][a] Y[q, a]   (_dy_kernel)
+//   dWt[b][r][a]  = sum over (p, q) with 2 s_p + s_q = b, r(p, q) = r of G[p, q] Y[p, a] Y[q, a]
+//                                                                                 (_dw_kernel)
+// per sample (the reference runs the custom_vjp per configuration under vmap).  The dY form uses
+// the table identity Wt[2 s_q + s_p][-r] = -Wt[2 s_p + s_q][r] of build_wtilde.  dWt loops over the
+// occupied p and finds the partner q of (p, r, s_q) through the inverse mode map, so every
+// output element is one thread's plain sum (no atomics, fixed order).
+//
+// nh_slogpf_vjp is the Pfaffian VJP: d log|Pf F| / dF = (1/2) F^{-T} (d Pf = (1/2) Pf tr(F^{-1} dF)),
+// scaled by the cotangent of log|Pf|.  It inverts the transposed copy of F in place in the output
+// buffer by Gauss-Jordan elimination with partial pivoting, one 256-thread block per matrix:
+// per step a one-wave pivot search, the pivot row, the old row k and column k staged in LDS, then
+// every element written once by its owner thread; the row interchanges are undone at the end as
+// one column permutation.
+
+#include "nh_common.h"
+
+namespace nh {
+
+namespace {
+
+constexpr int kVjpThreads = 256;
+
+__global__ void __launch_bounds__(kVjpThreads) k_vjp_dy(int LX, int LY, int d, int n, const float* __restrict__ Y,
+                                                        const int32_t* __restrict__ idx, const float* __restrict__ Wt,
+                                                        const float* __restrict__ G, float* __restrict__ dY) {
+  extern __shared__ int vj_lds[];
+  int* coord = vj_lds;  // (spin << 30) | (x << 15) | y of each occupied mode
+  const int b = blockIdx.y;
+  const int N = LX * LY;
+  const int32_t* ib = idx + (size_t)b * n;
+  for (int q = threadIdx.x; q < n; q += kVjpThreads) {
+    const int m = ib[q];
+    const int sp = m >= N, site = m - (m >= N ? N : 0);
+    coord[q] = (m < 0 || m >= 2 * N) ? -1 : ((sp << 30) | ((site / LY) << 15) | (site % LY));
+  }
+  __syncthreads();
+  const int rows = kVjpThreads / d;
+  const int pl = threadIdx.x / d, a = threadIdx.x - pl * d;
+  const int p = blockIdx.x * rows + pl;
+  if (p >= n) return;
+  const float* Gb = G + (size_t)b * n * n;
+  const float* Yb = Y + (size_t)b * n * d;
+  const int cp = coord[p];
+  if (cp < 0) {  // a mode outside [0, 2N)
+    dY[((size_t)b * n + p) * d + a] = NAN;
+    return;
+  }
+  const int sp = cp >> 30, xp = (cp >> 15) & 0x7fff, yp = cp & 0x7fff;
+  float acc = 0.f;
+  for (int q = 0; q < n; ++q) {
+    const int cq = coord[q];
+    if (cq < 0) continue;
+    const int sq = cq >> 30;
+    int dx = xp - ((cq >> 15) & 0x7fff), dy = yp - (cq & 0x7fff);
+    dx += dx < 0 ? LX : 0;
+    dy += dy < 0 ? LY : 0;
+    const float g = Gb[(size_t)p * n + q] - Gb[(size_t)q * n + p];
+    acc = fmaf(g * Wt[((size_t)(2 * sp + sq) * N + dx * LY + dy) * d + a], Yb[(size_t)q * d + a], acc);
+  }
+  dY[((size_t)b * n + p) * d + a] = acc;
+}
+
+__global__ void __launch_bounds__(kVjpThreads) k_vjp_dw(int LX, int LY, int d, int n, const float* __restrict__ Y,
+                                                        const int32_t* __restrict__ idx, const float* __restrict__ G,
+                                                        float* __restrict__ dWt) {
+  extern __shared__ int vj_lds[];
+  const int N = LX * LY;
+  int* inv = vj_lds;         // [2N]: occupied position of each mode, -1 if empty
+  int* coord = inv + 2 * N;  // [n]: (spin << 30) | site
+  const int b = blockIdx.y;
+  const int32_t* ib = idx + (size_t)b * n;
+  for (int m = threadIdx.x; m < 2 * N; m += kVjpThreads) inv[m] = -1;
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += kVjpThreads) {
+    const int m = ib[q];
+    const bool ok = m >= 0 && m < 2 * N;
+    if (ok) inv[m] = q;
+    coord[q] = ok ? (((m >= N) << 30) | (m - (m >= N ? N : 0))) : -1;
+  }
+  __syncthreads();
+  const int rs = kVjpThreads / d;
+  const int rl = threadIdx.x / d, a = threadIdx.x - rl * d;
+  const int r = blockIdx.x * rs + rl;
+  if (r >= N) return;
+  const int dxr = r / LY, dyr = r - dxr * LY;
+  const float* Gb = G + (size_t)b * n * n;
+  const float* Yb = Y + (size_t)b * n * d;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int p = 0; p < n; ++p) {
+    const int cp = coord[p];
+    if (cp < 0) continue;
+    const int sp = cp >> 30, site = cp & 0x3fffffff;
+    int xq = site / LY - dxr, yq = site % LY - dyr;
+    xq += xq < 0 ? LX : 0;
+    yq += yq < 0 ? LY : 0;
+    const int sq_site = xq * LY + yq;  // site_p - r
+    const float ypa = Yb[(size_t)p * d + a];
+#pragma unroll
+    for (int sq = 0; sq < 2; ++sq) {
+      const int q = inv[sq * N + sq_site];
+      if (q >= 0) acc[2 * sp + sq] = fmaf(Gb[(size_t)p * n + q] * ypa, Yb[(size_t)q * d + a], acc[2 * sp + sq]);
+    }
+  }
+  float* out = dWt + (size_t)b * 4 * N * d;
+#pragma unroll
+  for (int blk = 0; blk < 4; ++blk) out[((size_t)blk * N + r) * d + a] = acc[blk];
+}
+
+// In-place inverse of the transposed copy of F (so X = F^{-T}), scaled by 0.5 * cot.
+__global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __restrict__ F,
+                                                        const float* __restrict__ cot, float* __restrict__ G) {
+  extern __shared__ float vf_lds[];
+  float* rowk = vf_lds;             // [n] pivot row before scaling
+  float* oldk = rowk + n;           // [n] row k before the interchange
+  float* colk = oldk + n;           // [n] column k before the interchange
+  int* piv = reinterpret_cast<int*>(colk + n);  // [n]
+  int* perm = piv + n;                           // [n]
+  int* flag = perm + n;                          // [1]
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+  float* A = G + (size_t)b * n * n;
+  const float* Fb = F + (size_t)b * n * n;
+  for (size_t e = t; e < (size_t)n * n; e += kVjpThreads) {
+    const int i = (int)(e / n), j = (int)(e - (size_t)i * n);
+    A[e] = Fb[(size_t)j * n + i];
+  }
+  if (t == 0) flag[0] = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (wave == 0) {
+      float best = -1.f;
+      int bi = k;
+      for (int i = k + lane; i < n; i += kWave) {
+        const float v = fabsf(A[(size_t)i * n + k]);
+        if (v > best) { best = v; bi = i; }
+      }
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o, kWave);
+        const int oi = __shfl_xor(bi, o, kWave);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (lane == 0) {
+        piv[k] = bi;
+        if (!(best > 0.f)) flag[0] = 1;  // singular (or NaN)
+      }
+    }
+    __syncthreads();
+    if (flag[0]) break;
+    const int p = piv[k];
+    for (int j = t; j < n; j += kVjpThreads) {
+      rowk[j] = A[(size_t)p * n + j];
+      oldk[j] = A[(size_t)k * n + j];
+      colk[j] = A[(size_t)j * n + k];
+    }
+    __syncthreads();
+    const float dinv = 1.0f / rowk[k];
+    for (size_t e = t; e < (size_t)n * n; e += kVjpThreads) {
+      const int i = (int)(e / n), j = (int)(e - (size_t)i * n);
+      const float rk = (j == k) ? 1.0f : rowk[j];
+      float v;
+      if (i == k) {
+        v = rk * dinv;
+      } else {
+        const float base = (j == k) ? 0.0f : (i == p ? oldk[j] : A[e]);
+        const float f = colk[i == p ? k : i];  // column k entry of row i after the interchange
+        v = base - f * rk * dinv;
+      }
+      A[e] = v;
+    }
+    __syncthreads();
+  }
+  if (flag[0]) {
+    for (size_t e = t; e < (size_t)n * n; e += kVjpThreads) A[e] = NAN;
+    return;
+  }
+  // undo the row interchanges as one column permutation, then scale
+  if (t == 0) {
+    for (int j = 0; j < n; ++j) perm[j] = j;
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = piv[k];
+      if (p != k) {
+        const int tmp = perm[k];
+        perm[k] = perm[p];
+        perm[p] = tmp;
+      }
+    }
+  }
+  const float scale = 0.5f * (cot ? cot[b] : 1.0f);
+  for (int i = 0; i < n; ++i) {
+    __syncthreads();  // perm ready; the previous row's reads of rowk are done
+    for (int j = t; j < n; j += kVjpThreads) rowk[j] = A[(size_t)i * n + j];
+    __syncthreads();
+    for (int j = t; j < n; j += kVjpThreads) A[(size_t)i * n + j] = scale * rowk[perm[j]];
+  }
+}
+
+}  // namespace
+
+}  // namespace nh
+
+using namespace nh;
+
+extern "C" int nh_pairing_occ_vjp(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch, int32_t n,
+                                  const float* Y, const int32_t* idx, const float* Wt, const float* G, float* dY,
+                                  float* dWt) {
+  if (batch < 0 || n < 0 || LX < 1 || LY < 1) return NH_ERR_ARG;
+  if (batch == 0 || n == 0) return NH_OK;
+  if (!Y || !idx || !Wt || !G || (!dY && !dWt)) return NH_ERR_ARG;
+  const int N = LX * LY;
+  if (N > 1024 || n > 2 * N || d < 1 || d > 64 || (kVjpThreads % d) != 0 || LX >= (1 << 15) || LY >= (1 << 15))
+    return NH_ERR_SHAPE;
+  if (batch > 65535) return NH_ERR_SHAPE;
+  hipStream_t st = (hipStream_t)stream;
+  if (dY) {
+    const int rows = kVjpThreads / d;
+    hipLaunchKernelGGL(k_vjp_dy, dim3((n + rows - 1) / rows, batch), dim3(kVjpThreads), sizeof(int) * n, st, LX, LY,
+                       d, n, Y, idx, Wt, G, dY);
+    NH_LAUNCH_CHECK();
+  }
+  if (dWt) {
+    const int rs = kVjpThreads / d;
+    hipLaunchKernelGGL(k_vjp_dw, dim3((N + rs - 1) / rs, batch), dim3(kVjpThreads), sizeof(int) * (2 * N + n), st,
+                       LX, LY, d, n, Y, idx, G, dWt);
+    NH_LAUNCH_CHECK();
+  }
+  return NH_OK;
+}
+
+extern "C" int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G) {
+  if (batch < 0 || n < 0) return NH_ERR_ARG;
+  if (batch == 0 || n == 0) return NH_OK;
+  if (!F || !G || F == G) return NH_ERR_ARG;
+  if (n > 1024) return NH_ERR_SHAPE;
+  const size_t lds = sizeof(float) * 3 * (size_t)n + sizeof(int) * (2 * (size_t)n + 1);
+  hipLaunchKernelGGL(k_pf_vjp, dim3(batch), dim3(kVjpThreads), lds, (hipStream_t)stream, n, F, cot, G);
+  NH_LAUNCH_CHECK();
+  return NH_OK;
+}

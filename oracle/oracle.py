@@ -17,6 +17,8 @@ Pinning status per function (see the tests named in brackets):
   build_wtilde / wt_ud      antisymmetry identity + normalisation    [test_oracle_wtilde]
   conv / features           scipy.ndimage.correlate(mode="wrap")     [test_oracle_conv]
   pairing_occ               full 2N x 2N einsum path, sliced         [test_oracle_pairing]
+  pairing_occ_vjp /         finite differences of <G, F(Y, Wt)> and of
+  slogpf_vjp                log|Pf| along random skew directions      [test_oracle_vjp_*]
   slogpf                    Pf^2 = det, exact rational expansion,
                             Pf(B A B^T) = det(B) Pf(A), block form    [test_oracle_pfaffian]
   slogdet                   numpy (LAPACK) slogdet                   [test_oracle_det]
@@ -193,6 +195,34 @@ def pairing_occ(Y, modes, Wt, LX, LY):
     # (Y_p * Y_q) first, then the table: commutative, so F is exactly antisymmetric
     # (the order of ``code/backflow/pf_kernel.py:164-165``)
     return ((Yo[:, None, :] * Yo[None, :, :]) * Wt[blk, r]).sum(-1)
+
+
+def pairing_occ_vjp(Y, modes, Wt, G, LX, LY):
+    """Cotangents (dY (n, d), dWt (4, N, d)) of F = `pairing_occ` for the cotangent G (n, n),
+    Y (n, d) the features of the listed modes (the reference's kernel-level argument), every
+    entry of F (diagonal included) a function of Y and Wt: the VJP that
+    ``code/backflow/pf_kernel.py:354-365`` computes with ``_dy_kernel`` and ``_dw_kernel``.
+    dY[p] = sum_q G[p, q] Wt[b(p, q), r(p, q)] Y[q] + G[q, p] Wt[b(q, p), r(q, p)] Y[q];
+    dWt[b, r] = sum over pairs (p, q) with b(p, q) = b, r(p, q) = r of G[p, q] Y[p] Y[q]."""
+    disp, _, _ = lattice_tables(LX, LY)
+    N = LX * LY
+    m = np.asarray(modes)
+    sp, site = np.divmod(m, N)
+    r = disp[site[:, None], site[None, :]]
+    blk = 2 * sp[:, None] + sp[None, :]
+    Yo = np.asarray(Y, np.float64)
+    G = np.asarray(G, np.float64)
+    W = np.asarray(Wt, np.float64)[blk, r]  # (n, n, d): Wt of the pair (p, q)
+    dY = np.einsum("pq,pqa,qa->pa", G, W, Yo) + np.einsum("qp,qpa,qa->pa", G, W, Yo)
+    dWt = np.zeros((4, N, Yo.shape[1]))
+    np.add.at(dWt, (blk.ravel(), r.ravel()), (G[:, :, None] * Yo[:, None, :] * Yo[None, :, :]).reshape(-1, Yo.shape[1]))
+    return dY, dWt
+
+
+def slogpf_vjp(A):
+    """d log|Pf A| / dA with every entry independent: (1/2) A^{-T} (from d Pf = (1/2) Pf
+    tr(A^{-1} dA)), the Pfaffian VJP behind ``fermix.slogpf`` at ``code/backflow/models.py:210``."""
+    return 0.5 * np.linalg.inv(np.asarray(A, np.float64)).T
 
 
 def pairing_full(y, W, theta, LX, LY):

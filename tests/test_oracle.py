@@ -401,3 +401,56 @@ def test_product_energy_statistics_match_oracle():
     assert abs(r["var_per_site"] - vv) < 1e-9 and abs(r["var_err"] - ve) < 1e-9
     few = M.energy(Em[:7], Eq[:7], 256)
     assert np.isnan(few["E_err"])
+
+
+# ------------------------------------------------------------------------ VJPs (f2)
+
+
+def _valid_wt(LX, LY, d, seed):
+    rng = np.random.default_rng(seed)
+    return O.build_wtilde(rng.standard_normal((3, LX * LY, d)), 0.3, LX, LY)
+
+
+def test_oracle_vjp_pairing_finite_differences():
+    """<G, F(Y + h dY, Wt + h dW)> - <G, F(Y - h dY, ...)> over 2h against <dY, VJP_Y> + <dW, VJP_W>,
+    with random directions, for both the table argument and the features."""
+    rng = np.random.default_rng(3)
+    LX, LY, d = 4, 3, 4
+    N = LX * LY
+    Wt = _valid_wt(LX, LY, d, 4)
+    modes = np.sort(rng.choice(2 * N, 10, replace=False))
+    Y = rng.standard_normal((10, d))
+    G = rng.standard_normal((10, 10))
+    dY, dWt = O.pairing_occ_vjp(Y, modes, Wt, G, LX, LY)
+    for _ in range(3):
+        vY = rng.standard_normal(Y.shape)
+        vW = rng.standard_normal(Wt.shape)
+        h = 1e-6
+
+        def f(t):
+            Yfull = np.zeros((2 * N, d))  # pairing_occ takes features by mode
+            Yfull[modes] = Y + t * vY
+            return np.sum(G * O.pairing_occ(Yfull, modes, Wt + t * vW, LX, LY))
+
+        fd = (f(h) - f(-h)) / (2 * h)
+        assert fd == pytest.approx(np.sum(dY * vY) + np.sum(dWt * vW), rel=1e-7, abs=1e-9)
+    # the reference's dY form G - G^T holds for tables with the antisymmetry identity
+    disp, _, _ = O.lattice_tables(LX, LY)
+    sp, site = np.divmod(modes, N)
+    Wpq = Wt[2 * sp[:, None] + sp[None, :], disp[site[:, None], site[None, :]]]
+    np.testing.assert_allclose(np.einsum("pq,pqa,qa->pa", G - G.T, Wpq, Y), dY, rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_vjp_pfaffian_finite_differences():
+    rng = np.random.default_rng(5)
+    n = 12
+    A = rng.standard_normal((n, n))
+    A = A - A.T
+    G = O.slogpf_vjp(A)
+    np.testing.assert_allclose(G, -G.T, atol=1e-12)
+    for _ in range(3):
+        V = rng.standard_normal((n, n))
+        V = V - V.T
+        h = 1e-6
+        fd = (O.slogpf(A + h * V)[1] - O.slogpf(A - h * V)[1]) / (2 * h)
+        assert fd == pytest.approx(np.sum(G * V), rel=1e-6, abs=1e-8)
