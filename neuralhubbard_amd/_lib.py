@@ -16,7 +16,7 @@ NH_OK, NH_ERR_ARG, NH_ERR_SHAPE, NH_ERR_WORKSPACE, NH_ERR_HIP, NH_ERR_UNSUPPORTE
 _NAMES = {1: "NH_ERR_ARG", 2: "NH_ERR_SHAPE", 3: "NH_ERR_WORKSPACE", 4: "NH_ERR_HIP", 5: "NH_ERR_UNSUPPORTED"}
 NH_KIND_PFAFFIAN, NH_KIND_BCS = 0, 1
 NH_PREC_FP32, NH_PREC_MIXED = 0, 1
-NH_NET_CONV, NH_NET_RESCONV = 0, 1
+NH_NET_CONV, NH_NET_RESCONV, NH_NET_RESCONV_HEAD = 0, 1, 2
 
 
 class NhError(RuntimeError):

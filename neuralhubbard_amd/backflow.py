@@ -94,11 +94,14 @@ def build_wt_ud(W, theta, LX, LY) -> np.ndarray:
 
 def pack_conv(params: Params):
     """Conv weights as [conv][tap = (dx+1)*3 + (dy+1)][c_in][c_out] and biases [conv][C]; a
-    residual CNN's occupation embedding goes first as [class][C]."""
+    residual CNN's occupation embedding goes first as [class][C], a linear head last as
+    [k][c][d]."""
     w = [np.ascontiguousarray(np.transpose(np.asarray(cw, np.float32), (2, 3, 1, 0))).reshape(-1)
          for cw in params.conv_w]
     if params.emb is not None:
         w.insert(0, np.ascontiguousarray(np.asarray(params.emb, np.float32).T).reshape(-1))
+    if params.head is not None:  # (2, d, C) -> [k][c][d]
+        w.append(np.ascontiguousarray(np.transpose(np.asarray(params.head, np.float32), (0, 2, 1))).reshape(-1))
     b = [np.asarray(cb, np.float32).reshape(-1) for cb in params.conv_b]
     return np.concatenate(w), np.concatenate(b)
 
@@ -226,7 +229,10 @@ class _Backflow:
             self.base = None if params.phi0 is None else _dev(params.phi0, torch.float32)
         self.conv_w_lp = None
         self.conv_w_mfma = None
-        self.network = _lib.NH_NET_RESCONV if workload.network == "resconv" else _lib.NH_NET_CONV
+        if workload.network == "resconv":
+            self.network = _lib.NH_NET_RESCONV_HEAD if params.head is not None else _lib.NH_NET_RESCONV
+        else:
+            self.network = _lib.NH_NET_CONV
         if self.network == _lib.NH_NET_CONV and workload.LY == 16 and workload.channels == 32:
             from .mixed import pack_mfma_f32
 

@@ -47,6 +47,7 @@ class Workload:
     base_f0: bool = False  # config 0: additive antisymmetric base orbital matrix F0
     seed: int = 1234
     network: str = "conv"  # "conv" (above) or "resconv" (the released states' residual CNN)
+    features: int = 0  # d of a linear head on the network output (BCS states); 0: d = channels / 2
     t: float = 1.0
     notes: str = ""
     extra: dict = field(default_factory=dict)
@@ -61,7 +62,7 @@ class Workload:
 
     @property
     def d(self) -> int:
-        return self.channels // 2
+        return self.features or self.channels // 2
 
     @property
     def n_neighbor(self) -> int:
@@ -75,7 +76,8 @@ class Workload:
 
     def cnn_flops(self) -> float:
         """2 * 9 * N * sum_l C_in C_out, the convention of the reference's cost table."""
-        return 2.0 * 9 * self.N * sum(co * ci for co, ci in self.conv_shapes())
+        head = 2.0 * self.N * self.channels * 2 * self.features  # linear head, BCS states
+        return 2.0 * 9 * self.N * sum(co * ci for co, ci in self.conv_shapes()) + head
 
     def pair_flops(self) -> float:
         n = self.n_up + self.n_dn
@@ -111,10 +113,10 @@ WORKLOADS = {
 
 
 def small_workload(kind="pfaffian", LX=4, LY=4, n_up=2, n_dn=2, tprime=-0.2, channels=4,
-                   n_layers=2, base_f0=False, seed=7, U=8.0, network="conv") -> Workload:
+                   n_layers=2, base_f0=False, seed=7, U=8.0, network="conv", features=0) -> Workload:
     """A lattice small enough for exact enumeration (tests)."""
     return Workload(f"small_{kind}_{LX}x{LY}", kind, LX, LY, n_up, n_dn, tprime, U, channels,
-                    n_layers, 16, base_f0=base_f0, seed=seed, network=network)
+                    n_layers, 16, base_f0=base_f0, seed=seed, network=network, features=features)
 
 
 @dataclass
@@ -128,6 +130,7 @@ class Params:
     F0: np.ndarray | None = None  # pfaffian base (2N, 2N), antisymmetric
     phi0: np.ndarray | None = None  # bcs base pair function (N,)
     emb: np.ndarray | None = None  # resconv: (C, 4) vectors of the occupations (empty, up, down, double)
+    head: np.ndarray | None = None  # resconv with features > 0: (2, d, C) linear head y_k = head[k] x
 
 
 def make_params(w: Workload, seed: int | None = None) -> Params:
@@ -150,7 +153,10 @@ def make_params(w: Workload, seed: int | None = None) -> Params:
     else:
         W = (rng.standard_normal((N, d)) / np.sqrt(d)).astype(np.float32)
         phi0 = rng.standard_normal(N).astype(np.float32)
-    return Params(conv_w, conv_b, W, 0.55, F0, phi0, emb)
+    head = None
+    if w.network == "resconv" and w.features:
+        head = (rng.standard_normal((2, w.features, w.channels)) / np.sqrt(w.channels)).astype(np.float32)
+    return Params(conv_w, conv_b, W, 0.55, F0, phi0, emb, head)
 
 
 def make_configurations(w: Workload, batch: int | None = None, seed: int | None = None):

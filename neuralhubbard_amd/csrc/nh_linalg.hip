@@ -1171,6 +1171,72 @@ __global__ void k_pairing_occ(int LX, int LY, int d, int n, const float* __restr
   Fb[e] = (p > q) ? acc : -acc;
 }
 
+// Tiled form for d in {4, 8, 16, 32, 64}: a block owns kPairRows rows p of one sample with all of
+// the sample's occupied features (padded rows) and site coordinates in LDS; its threads sweep q
+// so that every F row is written with consecutive addresses, and the Wt row of a pair comes from
+// L2 as float4s.  Same arithmetic order as k_pairing_occ (exactly antisymmetric output).
+constexpr int kPairRows = 16;
+
+template <int D>
+__global__ void __launch_bounds__(256) k_pairing_occ_t(int LX, int LY, int n, const float* __restrict__ Y,
+                                                       const int32_t* __restrict__ idx,
+                                                       const float* __restrict__ Wt, float* __restrict__ F) {
+  extern __shared__ float pt_lds[];
+  constexpr int YS = D + 4;
+  float* ys = pt_lds;                                   // [n][YS]
+  int* coord = reinterpret_cast<int*>(ys + n * YS);     // [n]: spin << 30 | x << 15 | y, -1 if invalid
+  const int N = LX * LY;
+  const size_t b = blockIdx.y;
+  const int t = threadIdx.x;
+  const float4* Y4 = reinterpret_cast<const float4*>(Y + b * n * D);
+  for (int e = t; e < n * (D / 4); e += 256) {
+    const int p = e / (D / 4), a4 = e - p * (D / 4);
+    *reinterpret_cast<float4*>(ys + p * YS + 4 * a4) = Y4[e];
+  }
+  const int32_t* ib = idx + b * n;
+  for (int q = t; q < n; q += 256) {
+    const int m = ib[q];
+    const int sp = m >= N, site = m - sp * N;
+    coord[q] = (m < 0 || m >= 2 * N) ? -1 : ((sp << 30) | ((site / LY) << 15) | (site % LY));
+  }
+  __syncthreads();
+  float* Fb = F + b * (size_t)n * n;
+  const int p0 = blockIdx.x * kPairRows;
+  for (int pr = 0; pr < kPairRows; ++pr) {
+    const int p = p0 + pr;
+    if (p >= n) break;
+    for (int q = t; q < n; q += 256) {
+      float val = 0.f;
+      if (p != q) {
+        const int hi = p > q ? p : q, lo = p > q ? q : p;
+        const int ch = coord[hi], cl = coord[lo];
+        if (ch < 0 || cl < 0) {
+          val = NAN;
+        } else {
+          int dx = ((ch >> 15) & 0x7fff) - ((cl >> 15) & 0x7fff), dy = (ch & 0x7fff) - (cl & 0x7fff);
+          dx += dx < 0 ? LX : 0;
+          dy += dy < 0 ? LY : 0;
+          const float4* w4 = reinterpret_cast<const float4*>(Wt + ((size_t)(2 * (ch >> 30) + (cl >> 30)) * N +
+                                                                   dx * LY + dy) * D);
+          const float4* yh = reinterpret_cast<const float4*>(ys + hi * YS);
+          const float4* yl = reinterpret_cast<const float4*>(ys + lo * YS);
+          float acc = 0.f;
+#pragma unroll
+          for (int a4 = 0; a4 < D / 4; ++a4) {
+            const float4 w = w4[a4], u = yh[a4], v = yl[a4];
+            acc = fmaf(u.x * v.x, w.x, acc);
+            acc = fmaf(u.y * v.y, w.y, acc);
+            acc = fmaf(u.z * v.z, w.z, acc);
+            acc = fmaf(u.w * v.w, w.w, acc);
+          }
+          val = p > q ? acc : -acc;
+        }
+      }
+      Fb[(size_t)p * n + q] = val;
+    }
+  }
+}
+
 __global__ void k_pairing_ud(int LX, int LY, int d, int n_up, int n_dn, const float* __restrict__ Yu,
                              const float* __restrict__ Yd, const int32_t* __restrict__ iu,
                              const int32_t* __restrict__ idn, const float* __restrict__ Wt,
@@ -1236,8 +1302,23 @@ extern "C" int nh_pairing_occ(void* stream, int32_t LX, int32_t LY, int32_t d, i
   if (!Y || !idx || !Wt || !F) return NH_ERR_ARG;
   if (batch > 65535) return NH_ERR_SHAPE;
   const int threads = 256;
-  dim3 grid((n * n + threads - 1) / threads, batch);
-  hipLaunchKernelGGL(k_pairing_occ, grid, dim3(threads), 0, (hipStream_t)stream, LX, LY, d, n, Y, idx, Wt, F);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = sizeof(float) * ((size_t)n * (d + 4) + n);
+  const bool tiled = (d == 4 || d == 8 || d == 16 || d == 32 || d == 64) && lds <= kMaxLds && LX < (1 << 15) &&
+                     LY < (1 << 15);
+  if (tiled) {
+    dim3 grid((n + kPairRows - 1) / kPairRows, batch);
+    switch (d) {
+      case 4: hipLaunchKernelGGL(k_pairing_occ_t<4>, grid, dim3(threads), lds, st, LX, LY, n, Y, idx, Wt, F); break;
+      case 8: hipLaunchKernelGGL(k_pairing_occ_t<8>, grid, dim3(threads), lds, st, LX, LY, n, Y, idx, Wt, F); break;
+      case 16: hipLaunchKernelGGL(k_pairing_occ_t<16>, grid, dim3(threads), lds, st, LX, LY, n, Y, idx, Wt, F); break;
+      case 32: hipLaunchKernelGGL(k_pairing_occ_t<32>, grid, dim3(threads), lds, st, LX, LY, n, Y, idx, Wt, F); break;
+      default: hipLaunchKernelGGL(k_pairing_occ_t<64>, grid, dim3(threads), lds, st, LX, LY, n, Y, idx, Wt, F);
+    }
+  } else {
+    dim3 grid((n * n + threads - 1) / threads, batch);
+    hipLaunchKernelGGL(k_pairing_occ, grid, dim3(threads), 0, st, LX, LY, d, n, Y, idx, Wt, F);
+  }
   NH_LAUNCH_CHECK();
   return NH_OK;
 }

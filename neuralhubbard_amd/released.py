@@ -7,12 +7,14 @@ Reference interfaces mirrored here (ChenAo-Phys/NeuralHubbard):
   them with ``np.load(allow_pickle=False)``, so nothing in the file is executed.
 * ``weights/<name>_spins.npz`` (``code/states.py:328-352``): ``bits = np.packbits(spins > 0,
   axis=-1)`` and ``shape = (nchains, 2N)``; `unpack_spins` / `pack_spins` / `load_spins`.
-* the state table of ``code/states.py:99-128`` for the Pfaffian backflow states this library
+* the state table of ``code/states.py:99-128`` for the 16x16 backflow states this library
   evaluates (`STATES`), with the projected E/N and sigma^2/N of the paper.
 
 `released_params` maps the leaves of a ``PfaffianBackflow`` (``code/backflow/models.py:121-151``:
 ``net`` = quantax ResConv with 16 blocks of 64 channels and a final bias, ``W`` (3, N, 32),
-``theta``, ``pad``) onto this library's `Params` of a ``network="resconv"`` workload.
+``theta``, ``pad``) or a ``BCSBackflow`` (``models.py:214-249``: the ResConv trunk without the
+final bias, ``head`` (2, 64, 64), ``W`` (N, 64), ``theta``, ``delta_d``, ``pad``) onto this
+library's `Params` of a ``network="resconv"`` workload.
 """
 
 from __future__ import annotations
@@ -27,9 +29,10 @@ from .configs import Params, Workload
 
 @dataclass(frozen=True)
 class StateInfo:
-    """A released Pfaffian backflow state (``code/states.py:39-92``, rows at ``:105-119``)."""
+    """A released backflow state (``code/states.py:39-92``, rows at ``:105-119``)."""
 
     name: str
+    ansatz: str  # "pfaffian" or "bcs"
     L: int
     tprime: float
     md5: str
@@ -49,16 +52,20 @@ class StateInfo:
         return 7 * self.N // 16
 
     def workload(self, batch: int = 8192) -> Workload:
-        return Workload(self.name, "pfaffian", self.L, self.L, self.n_up, self.n_up, self.tprime, self.U,
-                        64, 16, batch, network="resconv", notes=f"released state {self.name}")
+        # BCS: d = 64 features per spin from a linear head on the 64 channels (models.py:246-270)
+        return Workload(self.name, self.ansatz, self.L, self.L, self.n_up, self.n_up, self.tprime, self.U,
+                        64, 16, batch, network="resconv", features=64 if self.ansatz == "bcs" else 0,
+                        notes=f"released state {self.name}")
 
 
 STATES = {
     s.name: s
     for s in (
-        StateInfo("tp0_L16_pfaffian", 16, 0.0, "17b770fe57367b09b27ad37538842ba0", 1_208_320,
+        StateInfo("tp0_L16_bcs", "bcs", 16, 0.0, "cdfdb8e6065b904232bf10dec6ca2227", 1_208_320,
+                  -0.75878, 2e-5, 0.0255, 0.0020),
+        StateInfo("tp0_L16_pfaffian", "pfaffian", 16, 0.0, "17b770fe57367b09b27ad37538842ba0", 1_208_320,
                   -0.75813, 1e-5, 0.0259, 0.0002),
-        StateInfo("tpm02_L16_pfaffian", 16, -0.2, "aef48e8b7c10cfa183f8c0e8dcf84d06", 1_208_320,
+        StateInfo("tpm02_L16_pfaffian", "pfaffian", 16, -0.2, "aef48e8b7c10cfa183f8c0e8dcf84d06", 1_208_320,
                   -0.74533, 1e-5, 0.0272, 0.0005),
     )
 }
@@ -74,10 +81,21 @@ def load_eqx_leaves(path: str) -> list:
     return leaves
 
 
+def dwave_table(LX: int, LY: int) -> np.ndarray:
+    """F^d as a function of the displacement r = dx * LY + dy: -1 on x bonds, +1 on y bonds
+    (minimum image), 0 elsewhere (``code/backflow/models.py:69-83``)."""
+    dx, dy = np.divmod(np.arange(LX * LY), LY)
+    mx, my = np.minimum(dx, LX - dx), np.minimum(dy, LY - dy)
+    return np.where((mx == 1) & (my == 0), -1.0, np.where((mx == 0) & (my == 1), 1.0, 0.0)).astype(np.float32)
+
+
 def released_params(leaves: list, LX: int, LY: int) -> Params:
-    """`Params` of a released Pfaffian backflow from its leaves: the (C, 4, 1, 1) occupation
-    embedding, then 2 x blocks convolutions (C, C, 3, 3) each followed by its (C, 1, 1) bias,
-    the integer and boolean fields of the network, W (3, N, d), theta, and the zero pad."""
+    """`Params` of a released backflow from its leaves: the (C, 4, 1, 1) occupation embedding, then
+    2 x blocks convolutions (C, C, 3, 3), each followed by its (C, 1, 1) bias (the BCS trunk's last
+    convolution has none: zero here), the integer and boolean fields of the network, and the
+    pairing leaves: Pfaffian W (3, N, d), theta, pad (``models.py:121-151``); BCS head (2, d, C),
+    W (N, d), theta, delta_d, pad (``models.py:214-249``), delta_d F^d going into the
+    displacement-indexed base table phi0."""
     arrays = [a for a in leaves if a.dtype.kind == "f"]
     emb = arrays[0]
     if emb.ndim != 4 or emb.shape[1:] != (4, 1, 1):
@@ -85,28 +103,52 @@ def released_params(leaves: list, LX: int, LY: int) -> Params:
     C = emb.shape[0]
     conv_w, conv_b = [], []
     i = 1
-    while i + 1 < len(arrays) and arrays[i].shape == (C, C, 3, 3):
-        if arrays[i + 1].shape != (C, 1, 1):
-            raise ValueError(f"convolution {len(conv_w)} has no (C, 1, 1) bias")
+    no_bias = []  # convolutions without a bias leaf
+    while i < len(arrays) and arrays[i].shape == (C, C, 3, 3):
         conv_w.append(np.ascontiguousarray(arrays[i], np.float32))
-        conv_b.append(np.ascontiguousarray(arrays[i + 1].reshape(C), np.float32))
-        i += 2
+        i += 1
+        if i < len(arrays) and arrays[i].shape == (C, 1, 1):
+            conv_b.append(np.ascontiguousarray(arrays[i].reshape(C), np.float32))
+            i += 1
+        else:
+            conv_b.append(np.zeros(C, np.float32))
+            no_bias.append(len(conv_w) - 1)
     if not conv_w or len(conv_w) % 2:
         raise ValueError(f"expected an even number of convolutions, got {len(conv_w)}")
-    W, theta, pad = arrays[i], arrays[i + 1], arrays[i + 2]
+    if no_bias and no_bias != [len(conv_w) - 1]:
+        raise ValueError(f"only the last convolution may lack a bias, not {no_bias}")
+    missing_bias = len(no_bias)
     N = LX * LY
-    if W.shape != (3, N, C // 2) or theta.shape != () or pad.ndim != 1 or i + 3 != len(arrays):
-        raise ValueError(f"unexpected tail of leaves: W {W.shape}, theta {theta.shape}, pad {pad.shape}")
+    tail = arrays[i:]
+    if len(tail) == 3:  # Pfaffian: W, theta, pad
+        W, theta, pad = tail
+        if missing_bias or W.shape != (3, N, C // 2) or theta.shape != () or pad.ndim != 1:
+            raise ValueError(f"unexpected Pfaffian tail: W {W.shape}, theta {theta.shape}, pad {pad.shape}")
+        head = phi0 = None
+    elif len(tail) == 5:  # BCS: head, W, theta, delta_d, pad
+        head, W, theta, delta_d, pad = tail
+        d = head.shape[1] if head.ndim == 3 else -1
+        if (head.shape != (2, d, C) or W.shape != (N, d) or theta.shape != () or delta_d.shape != ()
+                or pad.ndim != 1 or missing_bias != 1):
+            raise ValueError(f"unexpected BCS tail: head {head.shape}, W {W.shape}, pad {pad.shape}")
+        head = np.ascontiguousarray(head, np.float32)
+        phi0 = (np.float32(delta_d) * dwave_table(LX, LY)).astype(np.float32)
+    else:
+        raise ValueError(f"unexpected number of pairing leaves: {len(tail)}")
     if np.any(pad != 0):
         raise ValueError("the pad leaf is not zero")
-    return Params(conv_w, conv_b, np.ascontiguousarray(W, np.float32), float(theta), None, None,
-                  np.ascontiguousarray(emb.reshape(C, 4), np.float32))
+    return Params(conv_w, conv_b, np.ascontiguousarray(W, np.float32), float(theta), None, phi0,
+                  np.ascontiguousarray(emb.reshape(C, 4), np.float32), head)
 
 
 def n_params(params: Params, pad: int = 0) -> int:
-    """Float parameter count as quantax counts it (``code/states.py:321-324``)."""
+    """Float parameter count as quantax counts it (``code/states.py:321-324``): the BCS trunk's last
+    convolution has no bias, and its delta_d is one more scalar."""
     arrays = [params.emb, params.W, *params.conv_w, *params.conv_b]
-    return int(sum(np.asarray(a).size for a in arrays) + 1 + pad)
+    n = sum(np.asarray(a).size for a in arrays) + 1 + pad
+    if params.head is not None:
+        n += params.head.size - params.conv_b[-1].size + 1
+    return int(n)
 
 
 def load_released(name: str, path: str):

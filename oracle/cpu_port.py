@@ -57,7 +57,8 @@ class CpuPath:
         ptrs = [a.ctypes.data if a is not None else 0 for a in self._keep]
         self.m = CpuModel(0 if workload.kind == "pfaffian" else 1, workload.LX, workload.LY, workload.n_up,
                           workload.n_dn, workload.channels, workload.n_layers, 1 if project else 0,
-                          1 if getattr(workload, "network", "conv") == "resconv" else 0, *ptrs)
+                          (2 if params.head is not None else 1) if getattr(workload, "network", "conv") == "resconv"
+                          else 0, *ptrs)
 
     def psi(self, s):
         s = np.ascontiguousarray(s, np.int8).reshape(-1, 2 * self.w.N)

@@ -27,7 +27,8 @@
 typedef struct {
   int kind; /* 0 pfaffian, 1 bcs */
   int LX, LY, n_up, n_dn, C, L, project;
-  int net;           /* 0: conv stack; 1: residual CNN of the released states (L blocks) */
+  int net;           /* 0: conv stack; 1: residual CNN of the released states (L blocks); 2: the same
+                        with a linear head [2][C][d = C] after the embedding and convs (BCS states) */
   const float* w;    /* packed conv weights (residual CNN: embedding [4][C] first) */
   const float* b;    /* biases */
   const float* wt;   /* pf [4][N][d], bcs [N][d] */
@@ -60,6 +61,8 @@ typedef struct {
   int* modes;
 } scratch;
 
+static int feat_dim(const cpu_model* m) { return m->net == 2 ? m->C : m->C / 2; }
+
 static scratch alloc_scratch(const cpu_model* m) {
   int N = m->LX * m->LY, n = m->n_up + m->n_dn, C = m->C;
   scratch s;
@@ -67,7 +70,7 @@ static scratch alloc_scratch(const cpu_model* m) {
   s.g = malloc(sizeof(REAL) * C * N);
   s.u = malloc(sizeof(REAL) * C * N);
   s.acc = malloc(sizeof(REAL) * C);
-  s.y = malloc(sizeof(REAL) * 2 * N * (C / 2));
+  s.y = malloc(sizeof(REAL) * 2 * N * feat_dim(m));
   s.F = malloc(sizeof(REAL) * (size_t)n * n);
   s.tau = malloc(sizeof(REAL) * n);
   s.v = malloc(sizeof(REAL) * n);
@@ -124,7 +127,19 @@ static void features_resconv(const cpu_model* m, const int8_t* x, scratch* s) {
 
 /* features of configuration x (+-1, 2N) into s->y [2N][d] */
 static void features(const cpu_model* m, const int8_t* x, scratch* s) {
-  const int N = m->LX * m->LY, C = m->C, d = C / 2;
+  const int N = m->LX * m->LY, C = m->C, d = feat_dim(m);
+  if (m->net == 2) { /* y_k[site][dd] = sum_c head[k][c][dd] h[c][site] */
+    features_resconv(m, x, s);
+    const float* head = m->w + 4 * C + (size_t)2 * m->L * 9 * C * C;
+    for (int k = 0; k < 2; ++k)
+      for (int site = 0; site < N; ++site)
+        for (int dd = 0; dd < d; ++dd) {
+          REAL acc = 0;
+          for (int c = 0; c < C; ++c) acc += (REAL)head[((size_t)k * C + c) * d + dd] * s->h[c * N + site];
+          s->y[((size_t)k * N + site) * d + dd] = acc;
+        }
+    return;
+  }
   if (m->net == 1) {
     features_resconv(m, x, s);
     for (int spin = 0; spin < 2; ++spin)
@@ -213,7 +228,7 @@ static float parity(long long k) { return (k & 1) ? -1.f : 1.f; }
 /* eta_g(s) psi(g.s) of configuration x under group element g. */
 static void image_amp(const cpu_model* m, const int8_t* x, int g, scratch* s, int8_t* xg, float* sign,
                       double* logabs) {
-  const int LX = m->LX, LY = m->LY, N = LX * LY, d = m->C / 2;
+  const int LX = m->LX, LY = m->LY, N = LX * LY, d = feat_dim(m);
   const int ne = m->n_up + m->n_dn;
   for (int i = 0; i < 2 * N; ++i) xg[gmap(i, g, N, LX, LY)] = x[i];
   features(m, xg, s);

@@ -134,11 +134,12 @@ def conv3x3_batch(h, w, b):
     return out + np.asarray(b, np.float64)[None, :, None, None]
 
 
-def resconv_features(S, emb, conv_w, conv_b, LX, LY):
+def resconv_features(S, emb, conv_w, conv_b, LX, LY, head=None):
     """(B, 2, N, d) features of the released states' residual CNN (quantax ResConv as built at
     ``code/backflow/models.py:319-329``, architecture ``network_settings.json:21-56``): the
     embedding vector of each site's occupation, blocks h + conv(gelu(conv(gelu(rmsnorm(h))))),
-    output / sqrt(blocks + 1), channels split y_up | y_dn (``models.py:163-167``).
+    output / sqrt(blocks + 1), channels split y_up | y_dn (``models.py:163-167``), or with a
+    (2, d, C) `head` the BCS features y_k[n] = head[k] x[:, n] (``models.py:264-270``).
     S (B, 2N) or (2N,); emb (C, 4); conv_w / conv_b: the 2 * blocks convs."""
     S = np.atleast_2d(np.asarray(S))
     B, N = S.shape[0], LX * LY
@@ -150,6 +151,8 @@ def resconv_features(S, emb, conv_w, conv_b, LX, LY):
         a = conv3x3_batch(gelu(rmsnorm(h)), conv_w[2 * k], conv_b[2 * k])
         h = h + conv3x3_batch(gelu(a), conv_w[2 * k + 1], conv_b[2 * k + 1])
     h = h / math.sqrt(nb + 1)
+    if head is not None:  # BCS: y_k = head[k] x (code/backflow/models.py:264-270)
+        return np.einsum("kdc,bcn->bknd", np.asarray(head, np.float64), h.reshape(B, C, N))
     return np.swapaxes(h.reshape(B, 2, C // 2, N), 2, 3)
 
 
@@ -334,14 +337,14 @@ class Model:
     def features(self, s):
         w = self.w
         if getattr(w, "network", "conv") == "resconv":
-            return resconv_features(s, self.p.emb, self.p.conv_w, self.p.conv_b, w.LX, w.LY)[0]
+            return resconv_features(s, self.p.emb, self.p.conv_w, self.p.conv_b, w.LX, w.LY, self.p.head)[0]
         return features(s, self.p.conv_w, self.p.conv_b, w.LX, w.LY)
 
     def features_batch(self, S):
         """(B, 2, N, d) features of a batch of configurations."""
         w = self.w
         if getattr(w, "network", "conv") == "resconv":
-            return resconv_features(S, self.p.emb, self.p.conv_w, self.p.conv_b, w.LX, w.LY)
+            return resconv_features(S, self.p.emb, self.p.conv_w, self.p.conv_b, w.LX, w.LY, self.p.head)
         return np.stack([self.features(s) for s in np.asarray(S)])
 
     def image_matrix(self, s, g):

@@ -122,15 +122,17 @@ def test_slogdet_matches_oracle():
             assert abs(la[b].item() - l_ref) < 1e-4 * max(1, n / 28)
 
 
-@pytest.mark.parametrize("LX,LY,d", [(8, 8, 8), (16, 16, 16), (5, 7, 4)])
-def test_pairing_occ_matches_oracle(LX, LY, d):
+@pytest.mark.parametrize("LX,LY,d,n", [(8, 8, 8, 40), (16, 16, 16, 40), (5, 7, 4, 40), (16, 16, 32, 224),
+                                        (4, 4, 12, 20)])
+def test_pairing_occ_matches_oracle(LX, LY, d, n):
+    """Tiled kernel for d in {4, 8, 16, 32, 64}; d = 12 takes the one-thread-per-entry kernel."""
     from neuralhubbard_amd.backflow import build_wtilde, pairing_occ
 
     rng = np.random.default_rng(2)
     N = LX * LY
     W = (rng.standard_normal((3, N, d)) / np.sqrt(d)).astype(np.float32)
     Wt = build_wtilde(W, 0.55, LX, LY)
-    B, n = 3, min(2 * N, 40)
+    B = 3
     idx = np.stack([np.sort(rng.choice(2 * N, n, replace=False)) for _ in range(B)]).astype(np.int32)
     Y = rng.standard_normal((B, n, d)).astype(np.float32)
     F = pairing_occ(torch.tensor(Y, device="cuda"), torch.tensor(idx, device="cuda"),
