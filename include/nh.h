@@ -35,7 +35,7 @@ enum {
 
 enum { NH_KIND_PFAFFIAN = 0, NH_KIND_BCS = 1 };
 enum { NH_PREC_FP32 = 0, NH_PREC_MIXED = 1 };
-enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1 };
+enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1, NH_NET_RESCONV_HEAD = 2 };
 
 /* One backflow network and its pairing parameters (all device pointers).
  * Features y[spin][site][a] = h[spin * d + a][site], d = C / 2 (code/backflow/models.py:163-167).
@@ -56,6 +56,10 @@ enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1 };
  *   MFMA; NH_PREC_MIXED runs split-BF16 MFMA (FP32 accumulation) on conv_w_lp, the BF16
  *   A-fragments of the convs (layout in neuralhubbard_amd/mixed.py, pack_resconv_bf16).
  *   conv_w_mfma is unused.
+ * network NH_NET_RESCONV_HEAD: the same network followed by the BCS states' linear head
+ *   (code/backflow/models.py:261-270): y[k][site][a] = sum_c head[k][a][c] x[c][site], x the
+ *   scaled output, d = C features per spin; conv_w ends with the head as [k][c][a] (2 x C x C);
+ *   a missing bias (the trunk's last conv) is a zero row of conv_b.
  * pfaffian: wt = [4][N][d] spin-block table [uu, ud, du, dd] with the distance envelope
  *           (code/backflow/pf_kernel.py:107-124); base = F0 [2N][2N] antisymmetric or NULL.
  * bcs:      wt = [N][d] (code/backflow/bcs_kernel.py:63-70); base = phi0 [N] or NULL,
@@ -64,7 +68,7 @@ typedef struct nh_model {
   int32_t kind;       /* NH_KIND_* */
   int32_t LX, LY;     /* periodic lattice, both >= 3 */
   int32_t n_up, n_dn; /* electrons per spin */
-  int32_t channels;   /* C: 8, 16 or 32 (NH_NET_CONV), 64 (NH_NET_RESCONV) */
+  int32_t channels;   /* C: 8, 16 or 32 (NH_NET_CONV), 64 (NH_NET_RESCONV[_HEAD]) */
   int32_t n_layers;   /* residual layers after the embedding conv, or residual blocks */
   int32_t precision;  /* NH_PREC_FP32 or NH_PREC_MIXED */
   int32_t project;    /* 1: 8-element projection, 0: the bare network */

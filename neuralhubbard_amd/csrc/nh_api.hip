@@ -64,7 +64,7 @@ int check_model(const nh_model* m) {
   if (m->kind != NH_KIND_PFAFFIAN && m->kind != NH_KIND_BCS) return NH_ERR_ARG;
   if (m->LX < 3 || m->LY < 3 || m->LX * m->LY > 1024) return NH_ERR_SHAPE;
   if (m->n_layers < 0 || m->n_up < 0 || m->n_dn < 0) return NH_ERR_ARG;
-  if (m->network == NH_NET_RESCONV) {
+  if (m->network == NH_NET_RESCONV || m->network == NH_NET_RESCONV_HEAD) {
     if (!resconv_supported(m)) return NH_ERR_SHAPE;
     if (m->precision == NH_PREC_MIXED && !m->conv_w_lp) return NH_ERR_UNSUPPORTED;
   } else if (m->network != NH_NET_CONV) {
@@ -109,7 +109,7 @@ struct ItemBuffers {
 
 static ItemBuffers carve_items(Carver& c, const nh_model* m, int64_t n_items) {
   const int N = m->LX * m->LY;
-  const int D = m->channels / 2;
+  const int D = feature_dim(m);
   const int64_t images = n_items * ipi_of(m);
   const size_t tri = pf_global_tri_floats(m);
   ItemBuffers b;

@@ -143,7 +143,8 @@ static int launch_c(hipStream_t st, const nh_model* m, const int8_t* s, const in
 
 int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
                const int32_t* item_hop, int img0, int img_end, int ipi, float* Y) {
-  if (m->network == NH_NET_RESCONV) return launch_resconv(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);
+  if (m->network == NH_NET_RESCONV || m->network == NH_NET_RESCONV_HEAD)
+    return launch_resconv(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);
   if (m->precision == NH_PREC_MIXED)
     return launch_cnn_mfma(st, m, true, s, item_b, item_hop, img0, img_end, ipi, Y);
   if (m->conv_w_mfma && cnn_mfma_supported(m))

@@ -7,6 +7,9 @@ namespace nh {
 
 int check_model(const nh_model* m);
 
+// features per spin: C / 2 (the two halves of the channels), or C from the linear head
+inline int feature_dim(const nh_model* m) { return m->network == NH_NET_RESCONV_HEAD ? m->channels : m->channels / 2; }
+
 // slots (threads) of one feature-network block for an N-site lattice
 int cnn_slots(int N);
 

@@ -1078,13 +1078,15 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
 int launch_image_amplitude(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
                            const int32_t* item_hop, int img0, int img_end, int ipi, const float* Y,
                            float* img_sign, double* img_logabs, float* gtri) {
-  switch (m->channels / 2) {
+  switch (feature_dim(m)) {
     case 4: return launch_image_d<4>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     case 8: return launch_image_d<8>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     case 16:
       return launch_image_d<16>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     case 32:
       return launch_image_d<32>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
+    case 64:
+      return launch_image_d<64>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, img_sign, img_logabs, gtri);
     default: return NH_ERR_SHAPE;
   }
 }

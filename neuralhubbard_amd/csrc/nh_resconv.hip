@@ -46,6 +46,22 @@ __device__ inline float rc_wave_sum(float x) {
   return x;
 }
 
+// Linear head of the BCS states: y[k][site][dd] = sum_c head[k][c][dd] x[site][c], x the scaled
+// network output staged in LDS as [site][kRcStride] FP32 (natural channel order).
+__device__ inline void rc_head(const float* xs, const float* __restrict__ head, int N, float* __restrict__ yo) {
+  for (int o = threadIdx.x; o < 2 * N * kRcC; o += blockDim.x) {
+    const int dd = o & (kRcC - 1);
+    const int ks = o / kRcC;  // k * N + site
+    const int site = ks % N, k = ks / N;
+    const float* hk = head + (size_t)k * kRcC * kRcC + dd;
+    const float* x = xs + site * kRcStride;
+    float acc = 0.f;
+#pragma unroll 8
+    for (int c = 0; c < kRcC; ++c) acc = fmaf(hk[c * kRcC], x[c], acc);
+    yo[o] = acc;
+  }
+}
+
 // act[site][pos(c)] = gelu(scale * v) for the wave's rows x0 .. x0+3
 __device__ inline void rc_store(float* act, const rc_f32x4 (&v)[4][4], float scale, int x0, int lane) {
   const int j = lane & 15, g16 = lane >> 4;
@@ -123,7 +139,8 @@ __device__ inline void rc_conv(const float* act, const float* __restrict__ Wl, c
 // Y[(img - img0)][spin][site][32]
 __global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const float* __restrict__ emb,
                                                  const float* __restrict__ Wc, const float* __restrict__ Bc,
-                                                 float out_scale, const int8_t* __restrict__ s,
+                                                 const float* __restrict__ head, float out_scale,
+                                                 const int8_t* __restrict__ s,
                                                  const int32_t* __restrict__ item_b,
                                                  const int32_t* __restrict__ item_hop, int img0, int img_end,
                                                  int ipi, float* __restrict__ Y) {
@@ -185,6 +202,19 @@ __global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const flo
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
+  }
+  if (head) {  // BCS: d = 64 features per spin from the linear head
+    __syncthreads();  // every wave is done reading the last conv input
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          act[((x0 + r) * kRcLY + 4 * g16 + q) * kRcStride + 16 * qq + j] = out_scale * h[r][qq][q];
+    __syncthreads();
+    rc_head(act, head, N, Y + (size_t)(img - img0) * (2 * N * kRcC));
+    return;
   }
   // features: channel c = 32 spin + a
   float* yo = Y + (size_t)(img - img0) * (2 * N * (kRcC / 2));
@@ -359,7 +389,8 @@ __host__ __device__ constexpr size_t rb_lds_floats(int N) {
 template <int NT>
 __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const float* __restrict__ emb,
                                                      const rb_i32x4* __restrict__ Wsl, const float* __restrict__ Bc,
-                                                     float out_scale, const int8_t* __restrict__ s,
+                                                     const float* __restrict__ head, float out_scale,
+                                                     const int8_t* __restrict__ s,
                                                      const int32_t* __restrict__ item_b,
                                                      const int32_t* __restrict__ item_hop, int img0, int img_end,
                                                      int ipi, float* __restrict__ Y) {
@@ -429,6 +460,20 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
   }
+  if (head) {  // BCS: the last slab barrier ended every read of act; stage FP32 [site][68] there
+    float* xs = lds;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const rc_f32x4 v = out_scale * h[r][qq];
+        *reinterpret_cast<float4*>(xs + ((x0 + r) * kRcLY + j) * kRcStride + 16 * qq + 4 * g16) =
+            make_float4(v[0], v[1], v[2], v[3]);
+      }
+    __syncthreads();
+    rc_head(xs, head, N, Y + (size_t)(img - img0) * (2 * N * kRcC));
+    return;
+  }
   float* yo = Y + (size_t)(img - img0) * (2 * N * (kRcC / 2));
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -456,6 +501,9 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
   const int N = m->LX * kRcLY;
   const size_t lds = sizeof(float) * ((size_t)N * kRcStride + 16);
   const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
+  const float* head = m->network == NH_NET_RESCONV_HEAD
+                          ? m->conv_w + kRcClasses * kRcC + (size_t)2 * m->n_layers * 9 * kRcC * kRcC
+                          : nullptr;
   if (m->precision == NH_PREC_MIXED) {
     if (!m->conv_w_lp) return NH_ERR_UNSUPPORTED;
     const size_t lds_bf16 = sizeof(float) * rb_lds_floats(N);
@@ -463,25 +511,26 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
     switch (m->LX) {
       case 4:
         hipLaunchKernelGGL(k_resconv_bf16<64>, dim3(nimg), dim3(64), lds_bf16, st, m->LX, m->n_layers, m->conv_w, wsl,
-                           m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+                           m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
         break;
       case 8:
         hipLaunchKernelGGL(k_resconv_bf16<128>, dim3(nimg), dim3(128), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
         break;
       case 12:
         hipLaunchKernelGGL(k_resconv_bf16<192>, dim3(nimg), dim3(192), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
         break;
       default:
         hipLaunchKernelGGL(k_resconv_bf16<256>, dim3(nimg), dim3(256), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
     }
     NH_LAUNCH_CHECK();
     return NH_OK;
   }
   hipLaunchKernelGGL(k_resconv, dim3(nimg), dim3(16 * m->LX), lds, st, m->LX, m->n_layers, m->conv_w,
-                     m->conv_w + kRcClasses * kRcC, m->conv_b, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+                     m->conv_w + kRcClasses * kRcC, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi,
+                     Y);
   NH_LAUNCH_CHECK();
   return NH_OK;
 }

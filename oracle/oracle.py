@@ -33,8 +33,9 @@ Pinning status per function (see the tests named in brackets):
   resconv_features          translation equivariance, embedding order
   (+ rmsnorm, conv batch)   with zero convs, RMSNorm closed form, the
                             scipy-pinned conv3x3; the composite by the
-                            released state's E/N against the paper
-                            (a swapped embedding fails it)           [test_released, test_gpu_released]
+                            released states' E/N against the paper
+                            (a swapped embedding fails it); the BCS
+                            head by tp0_L16_bcs's E/N                [test_released, test_gpu_released]
   sur_forward / sur_loss    the same network in torch conv2d (circular
                             padding), translation invariance, f = N b3
                             for zero weights                         [test_oracle_surrogate_forward]

@@ -1,14 +1,14 @@
-"""Writes ``tpm02_L16_pfaffian.npz``: the released tpm02_L16_pfaffian state as a fixture (run in
-the build container, where the reference checkout is readable:
-``python tests/golden/make_released.py [reference_root]``).
+"""Writes ``<name>.npz`` for the released tpm02_L16_pfaffian and tp0_L16_bcs states as fixtures
+(run in the build container, where the reference checkout is readable:
+``python tests/golden/make_released.py [reference_root] [name ...]``).
 
-The fixture holds data only: the parameters read from ``weights/tpm02_L16_pfaffian.eqx`` (leaf
-stream read with ``np.load(allow_pickle=False)``), the first CHAINS released chains of
-``weights/tpm02_L16_pfaffian_spins.npz`` in their packed form, and the FP64 oracle's values on
-a few of them: features of image 0, projected and bare (sign, log|psi|), and E_loc
-(t = 1, t' = -0.2, U = 8; the C restatement in FP64, every connected configuration in full),
-also with the up and down embedding vectors exchanged.
-The paper's E/N and sigma^2/N of this state are in `neuralhubbard_amd.released.STATES`.
+A fixture holds data only: the parameters read from ``weights/<name>.eqx`` (leaf stream read
+with ``np.load(allow_pickle=False)``; for BCS also the linear head and delta_d F^d as the base
+table phi0), the first CHAINS released chains of ``weights/<name>_spins.npz`` in their packed
+form, and the FP64 oracle's values on a few of them: features of image 0, projected and bare
+(sign, log|psi|), and E_loc (t = 1, the state's t', U = 8; the C restatement in FP64, every
+connected configuration in full), for the Pfaffian state also with the up and down embedding
+vectors exchanged.  The paper's E/N and sigma^2/N are in `neuralhubbard_amd.released.STATES`.
 """
 
 import copy
@@ -27,14 +27,14 @@ from neuralhubbard_amd import released as R  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 from oracle.cpu_port import CpuPath  # noqa: E402
 
-NAME = "tpm02_L16_pfaffian"
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_bcs")
 CHAINS = 2048  # chains kept for the E/N check
 N_PSI = 8      # chains with oracle amplitudes
 N_FEAT = 2     # chains with oracle features
 N_ELOC = 2     # chains with oracle E_loc
 
 
-def main(ref_root="/root/reference"):
+def make(NAME, ref_root):
     wdir = os.path.join(ref_root, "weights")
     info = R.STATES[NAME]
     wk, p = R.load_released(NAME, os.path.join(wdir, f"{NAME}.eqx"))
@@ -48,23 +48,32 @@ def main(ref_root="/root/reference"):
     t0 = time.time()
     el = cp.local_energy(S[:N_ELOC], sg[:N_ELOC], la[:N_ELOC])
     print(f"oracle E_loc of {N_ELOC} chains: {el} ({time.time() - t0:.0f} s)", flush=True)
-    # the same with the up and down embedding vectors exchanged (a plausible convention
-    # mistake), to show that E/N against the paper discriminates it
-    q = copy.copy(p)
-    q.emb = np.ascontiguousarray(p.emb[:, [0, 2, 1, 3]])
-    cq = CpuPath(wk, q, bits=64, threads=os.cpu_count())
-    qs, ql = cq.psi(S[:N_ELOC])
-    el_swap = cq.local_energy(S[:N_ELOC], qs, ql)
-    print(f"... with up/down embeddings exchanged: {el_swap}", flush=True)
+    extra = {}
+    if info.ansatz == "pfaffian":
+        # the same with the up and down embedding vectors exchanged (a plausible convention
+        # mistake), to show that E/N against the paper discriminates it
+        q = copy.copy(p)
+        q.emb = np.ascontiguousarray(p.emb[:, [0, 2, 1, 3]])
+        cq = CpuPath(wk, q, bits=64, threads=os.cpu_count())
+        qs, ql = cq.psi(S[:N_ELOC])
+        extra["eloc_swapped_embedding"] = cq.local_energy(S[:N_ELOC], qs, ql)
+        print(f"... with up/down embeddings exchanged: {extra['eloc_swapped_embedding']}", flush=True)
+    else:
+        extra["head"], extra["phi0"] = p.head, p.phi0
     out = os.path.join(HERE, f"{NAME}.npz")
     np.savez_compressed(
         out,
         emb=p.emb, conv_w=np.stack(p.conv_w), conv_b=np.stack(p.conv_b), W=p.W, theta=np.float32(p.theta),
         **R.pack_spins(S),
         features=feats.astype(np.float32), psi_sign=sg, psi_logabs=la, bare_sign=bsg, bare_logabs=bla,
-        eloc=el, eloc_swapped_embedding=el_swap,
+        eloc=el, **extra,
     )
     print(out, os.path.getsize(out))
+
+
+def main(ref_root="/root/reference", *names):
+    for name in names or NAMES:
+        make(name, ref_root)
 
 
 if __name__ == "__main__":

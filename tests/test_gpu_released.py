@@ -2,11 +2,11 @@
 
 * synthetic residual CNNs on 4x16 and 8x16 lattices against the FP64 oracle: features,
   bare and projected log|psi| and E_loc;
-* the released tpm02_L16_pfaffian state (fixture tests/golden/tpm02_L16_pfaffian.npz, written by
-  make_released.py from the reference's weight and chain files): features, log|psi| and E_loc
-  against the FP64 oracle values stored in the fixture, and the energy per site over the
-  released chains against the paper's E/N = -0.74533(1), sigma^2/N = 0.0272(5)
-  (code/states.py:114-115).
+* the released tpm02_L16_pfaffian and tp0_L16_bcs states (fixtures tests/golden/<name>.npz,
+  written by make_released.py from the reference's weight and chain files): features, log|psi|
+  and E_loc against the FP64 oracle values stored in the fixtures, and the energy per site over
+  the released chains against the paper: E/N = -0.74533(1), sigma^2/N = 0.0272(5)
+  (code/states.py:114-115) and E/N = -0.75878(2), sigma^2/N = 0.0255(20) (:104-105).
 
 Both precisions of the network: FP32 MFMA and split-BF16 MFMA ("mixed").  Tolerances: features
 1e-4 (FP32) or 1e-3 (split BF16) relative to their largest magnitude, log|psi| as described in
@@ -26,7 +26,6 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIXTURE = os.path.join(HERE, "golden", "tpm02_L16_pfaffian.npz")
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -35,11 +34,12 @@ def _gpu():
         pytest.fail("GPU tests need a visible MI355X")
 
 
-def _small(LX, n_el, blocks, seed):
+def _small(LX, n_el, blocks, seed, kind="pfaffian"):
     from neuralhubbard_amd.configs import make_configurations, make_params, small_workload
 
-    wk = small_workload("pfaffian", LX=LX, LY=16, n_up=n_el, n_dn=n_el, tprime=-0.2, channels=64,
-                        n_layers=blocks, seed=seed, network="resconv")
+    wk = small_workload(kind, LX=LX, LY=16, n_up=n_el, n_dn=n_el, tprime=-0.2 if kind == "pfaffian" else 0.0,
+                        channels=64, n_layers=blocks, seed=seed, network="resconv",
+                        features=64 if kind == "bcs" else 0)
     return wk, make_params(wk), make_configurations(wk, 4)
 
 
@@ -62,8 +62,8 @@ def _sensitivity(wk, p, S, project, rel=1e-7, trials=3):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
-@pytest.mark.parametrize("LX,n_el,blocks", [(4, 12, 2), (8, 28, 3)])
-def test_small_resconv_matches_oracle(LX, n_el, blocks, prec):
+@pytest.mark.parametrize("kind,LX,n_el,blocks", [("pfaffian", 4, 12, 2), ("pfaffian", 8, 28, 3), ("bcs", 4, 12, 2)])
+def test_small_resconv_matches_oracle(kind, LX, n_el, blocks, prec):
     """Random weights give some configurations nearly singular pairing matrices (condition
     numbers ~1e4 at 8x16), whose log|Pf| moves by ~1e-4 under feature changes of one FP32 unit;
     the log|psi| tolerance is 1e-4 plus 30 times that measured sensitivity (at relative feature
@@ -71,8 +71,8 @@ def test_small_resconv_matches_oracle(LX, n_el, blocks, prec):
     sides and is held to 1e-4 per site on 4x16, 1e-3 on 8x16 (ten times that in split BF16)."""
     from neuralhubbard_amd.backflow import Hubbard, LogArray, Samples, Variational, build_backflow
 
-    wk, p, S = _small(LX, n_el, blocks, seed=11 + LX)
-    print(f"resconv {LX}x16 n_el={n_el} blocks={blocks} {prec}", flush=True)
+    wk, p, S = _small(LX, n_el, blocks, seed=11 + LX, kind=kind)
+    print(f"resconv {kind} {LX}x16 n_el={n_el} blocks={blocks} {prec}", flush=True)
     mixed = prec == "mixed"
     ref = O.Model(wk, p)
     dev = build_backflow(wk, p, prec)
@@ -97,21 +97,26 @@ def test_small_resconv_matches_oracle(LX, n_el, blocks, prec):
     np.testing.assert_allclose(el / wk.N, er / wk.N, rtol=0, atol=(1e-4 if LX == 4 else 1e-3) * (10 if mixed else 1))
 
 
-def _released():
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_bcs")
+
+
+def _released(name="tpm02_L16_pfaffian"):
     from neuralhubbard_amd.configs import Params
     from neuralhubbard_amd.released import STATES, unpack_spins
 
-    f = dict(np.load(FIXTURE, allow_pickle=False))
-    info = STATES["tpm02_L16_pfaffian"]
-    p = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, None, f["emb"])
+    f = dict(np.load(os.path.join(HERE, "golden", f"{name}.npz"), allow_pickle=False))
+    info = STATES[name]
+    p = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, f.get("phi0"), f["emb"],
+               f.get("head"))
     return info, info.workload(), p, unpack_spins(f["bits"], f["shape"]), f
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
-def test_released_state_matches_oracle(prec):
+@pytest.mark.parametrize("name", NAMES)
+def test_released_state_matches_oracle(name, prec):
     from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow
 
-    info, wk, p, S, f = _released()
+    info, wk, p, S, f = _released(name)
     dev = build_backflow(wk, p, prec)
     nf, npsi, nel = len(f["features"]), len(f["psi_sign"]), len(f["eloc"])
     print(f"released {info.name} {prec}: features of {nf}, psi of {npsi}, E_loc of {nel} chains", flush=True)
@@ -132,10 +137,11 @@ def test_released_state_matches_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
-def test_released_state_energy_per_site(prec):
+@pytest.mark.parametrize("name", NAMES)
+def test_released_state_energy_per_site(name, prec):
     from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow
 
-    info, wk, p, S, _ = _released()
+    info, wk, p, S, _ = _released(name)
     K = 512
     dev = build_backflow(wk, p, prec)
     state = Variational(dev, symm=True)

@@ -1,15 +1,16 @@
 """Released-state loaders and the residual CNN restatement on the CPU.
 
-* the fixture tests/golden/tpm02_L16_pfaffian.npz against the reference's own files when the
-  reference checkout is present (md5 of the weight file, code/states.py:114; every leaf; the
-  packed chains bit for bit, code/states.py:328-352), and the parameter count 1,208,320 of
-  code/states.py:114 without it;
+* the fixtures tests/golden/tpm02_L16_pfaffian.npz and tp0_L16_bcs.npz against the reference's
+  own files when the reference checkout is present (md5 of the weight file, code/states.py:104,
+  114; every leaf; the packed chains bit for bit, code/states.py:328-352), and the parameter
+  count 1,208,320 of code/states.py:104, 114 without it;
 * the oracle's residual CNN: its batched convolution against the scipy-pinned `conv3x3`,
   RMSNorm's closed form, translation equivariance of the whole network, the C restatement
   (libnhcpu64) against the numpy one, and the fixture's features and amplitudes against a
   recomputation (drift guard);
 * the fixture's FP64 E_loc of released chains against the paper's E/N: one chain's E_loc / N
-  has a spread of sqrt(0.0272 / 256) = 0.0103 around -0.74533 (code/states.py:114-115).
+  has a spread of sqrt(sigma^2/N / N) around it (0.0103 around -0.74533, code/states.py:114-115;
+  0.0100 around -0.75878, :104-105).
 """
 
 import hashlib
@@ -23,22 +24,27 @@ from neuralhubbard_amd.configs import Params, make_configurations, make_params, 
 from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIXTURE = os.path.join(HERE, "golden", "tpm02_L16_pfaffian.npz")
 REF_WEIGHTS = "/root/reference/weights"
 NAME = "tpm02_L16_pfaffian"
 
 
-def fixture():
-    f = dict(np.load(FIXTURE, allow_pickle=False))
-    p = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, None, f["emb"])
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_bcs")
+PAD = {"tpm02_L16_pfaffian": 1791, "tp0_L16_bcs": 1854}  # zero pads of the weight files
+
+
+def fixture(name=NAME):
+    f = dict(np.load(os.path.join(HERE, "golden", f"{name}.npz"), allow_pickle=False))
+    p = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, f.get("phi0"), f["emb"],
+               f.get("head"))
     return f, p
 
 
-def test_fixture_parameter_count_and_chains():
-    f, p = fixture()
-    assert R.n_params(p, pad=1791) == R.STATES[NAME].nparams == 1_208_320
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_parameter_count_and_chains(name):
+    f, p = fixture(name)
+    assert R.n_params(p, pad=PAD[name]) == R.STATES[name].nparams == 1_208_320
     S = R.unpack_spins(f["bits"], f["shape"])
-    N = R.STATES[NAME].N
+    N = R.STATES[name].N
     assert S.shape == (2048, 2 * N) and S.dtype == np.int8
     assert np.all((S[:, :N] > 0).sum(1) == 112) and np.all((S[:, N:] > 0).sum(1) == 112)
     rec = R.pack_spins(S)
@@ -46,19 +52,25 @@ def test_fixture_parameter_count_and_chains():
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_WEIGHTS), reason="reference checkout not present")
-def test_fixture_matches_reference_files():
-    path = os.path.join(REF_WEIGHTS, f"{NAME}.eqx")
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_matches_reference_files(name):
+    path = os.path.join(REF_WEIGHTS, f"{name}.eqx")
     with open(path, "rb") as fh:
-        assert hashlib.md5(fh.read()).hexdigest() == R.STATES[NAME].md5
-    f, p = fixture()
-    wk, q = R.load_released(NAME, path)
+        assert hashlib.md5(fh.read()).hexdigest() == R.STATES[name].md5
+    f, p = fixture(name)
+    wk, q = R.load_released(name, path)
+    leaves = R.load_eqx_leaves(path)
+    assert sum(a.size for a in leaves if a.dtype.kind == "f") == R.STATES[name].nparams
+    if q.head is not None:
+        np.testing.assert_array_equal(q.head, f["head"])
+        np.testing.assert_array_equal(q.phi0, f["phi0"])
     np.testing.assert_array_equal(q.emb, p.emb)
     np.testing.assert_array_equal(np.stack(q.conv_w), f["conv_w"])
     np.testing.assert_array_equal(np.stack(q.conv_b), f["conv_b"])
     np.testing.assert_array_equal(q.W, f["W"])
     assert np.float32(q.theta) == f["theta"]
-    spins_path = os.path.join(REF_WEIGHTS, f"{NAME}_spins.npz")
-    S = R.load_spins(spins_path, R.STATES[NAME])
+    spins_path = os.path.join(REF_WEIGHTS, f"{name}_spins.npz")
+    S = R.load_spins(spins_path, R.STATES[name])
     n = int(f["shape"][0])
     np.testing.assert_array_equal(S[:n], R.unpack_spins(f["bits"], f["shape"]))
     with np.load(spins_path, allow_pickle=False) as g:
@@ -133,10 +145,13 @@ def test_resconv_embedding_order():
     np.testing.assert_allclose(h[3], p.emb[:, 3], rtol=1e-6)
 
 
-def test_c_restatement_matches_numpy_resconv():
+@pytest.mark.parametrize("kind,features", [("pfaffian", 0), ("bcs", 8)])
+def test_c_restatement_matches_numpy_resconv(kind, features):
+    """bcs with features = channels: the released BCS form, a linear head on the network output."""
     from oracle.cpu_port import CpuPath
 
-    wk = small_workload("pfaffian", LX=4, LY=4, n_up=3, n_dn=3, channels=8, n_layers=2, network="resconv")
+    wk = small_workload(kind, LX=4, LY=4, n_up=3, n_dn=3, channels=8, n_layers=2, network="resconv",
+                        features=features)
     p = make_params(wk)
     S = make_configurations(wk, 3)
     ref = O.Model(wk, p)
@@ -150,14 +165,15 @@ def test_c_restatement_matches_numpy_resconv():
     np.testing.assert_allclose(e_c, e_ref, rtol=1e-5, atol=1e-5)
 
 
-def test_fixture_oracle_values_are_current():
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_oracle_values_are_current(name):
     from oracle.cpu_port import CpuPath
 
-    f, p = fixture()
-    info = R.STATES[NAME]
+    f, p = fixture(name)
+    info = R.STATES[name]
     wk = info.workload()
     S = R.unpack_spins(f["bits"], f["shape"])
-    y = O.resconv_features(S[:1], p.emb, p.conv_w, p.conv_b, wk.LX, wk.LY)
+    y = O.resconv_features(S[:1], p.emb, p.conv_w, p.conv_b, wk.LX, wk.LY, p.head)
     np.testing.assert_allclose(y, f["features"][:1], rtol=0, atol=1e-5 * np.abs(y).max())
     sg, la = CpuPath(wk, p, bits=64).psi(S[:2])
     np.testing.assert_array_equal(sg, f["psi_sign"][:2])
@@ -165,6 +181,7 @@ def test_fixture_oracle_values_are_current():
     spread = np.sqrt(info.var_per_site / info.N)
     e = f["eloc"] / info.N
     assert np.all(np.abs(e - info.E_per_site) < 5 * spread)
-    # the check discriminates a plausible convention mistake: up and down embeddings exchanged
-    e_swap = f["eloc_swapped_embedding"] / info.N
-    assert np.any(np.abs(e_swap - info.E_per_site) > 5 * spread)
+    if "eloc_swapped_embedding" in f:
+        # the check discriminates a plausible convention mistake: up and down embeddings exchanged
+        e_swap = f["eloc_swapped_embedding"] / info.N
+        assert np.any(np.abs(e_swap - info.E_per_site) > 5 * spread)
