@@ -233,7 +233,11 @@ class _Backflow:
             self.network = _lib.NH_NET_RESCONV_HEAD if params.head is not None else _lib.NH_NET_RESCONV
         else:
             self.network = _lib.NH_NET_CONV
-        if self.network == _lib.NH_NET_CONV and workload.LY == 16 and workload.channels == 32:
+        if self.network != _lib.NH_NET_CONV:  # residual CNN: FP32 weight slabs
+            from .mixed import pack_resconv_f32
+
+            self.conv_w_mfma = _dev(pack_resconv_f32(params), torch.float32)
+        elif workload.LY == 16 and workload.channels == 32:
             from .mixed import pack_mfma_f32
 
             self.conv_w_mfma = _dev(pack_mfma_f32(params), torch.float32)

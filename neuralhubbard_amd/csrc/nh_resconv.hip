@@ -487,6 +487,189 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
   }
 }
 
+// ---------------------------------------------------------------- FP32 with weight slabs
+//
+// The FP32 kernel with the weight stream of the split-BF16 one: slabs of (conv, kh, dyi) in
+// B-fragment order, [dxi][qq][lane][8 floats] with W[tap = 3 dxi + dyi][32 kh + 4 kk + l / 16]
+// [16 qq + l % 16] at element kk (neuralhubbard_amd/mixed.py, pack_resconv_f32; 24 KB, the same
+// size as a split-BF16 slab), double-buffered in LDS, the next slab in flight during the MFMAs.
+
+template <int DYI>
+__device__ __forceinline__ void rs_taps(const float* act, const float4* wb, int LX, int x0, int lane, int kh,
+                                        rc_f32x4 (&acc)[4][4]) {
+  const int j = lane & 15, g16 = lane >> 4;
+  float fs[6][8];
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    int xr = x0 - 1 + u;
+    xr += xr < 0 ? LX : 0;
+    xr -= xr >= LX ? LX : 0;
+    const float4* p = reinterpret_cast<const float4*>(act + (xr * kRcLY + j) * kRcStride + 16 * g16 + 8 * kh);
+    const float4 p0 = p[0], p1 = p[1];
+    const float v[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      if (DYI == 1) fs[u][kk] = v[kk];
+      else if (DYI == 0) fs[u][kk] = rc_rot<0x121>(v[kk]);
+      else fs[u][kk] = rc_rot<0x12F>(v[kk]);
+    }
+  }
+#pragma unroll
+  for (int dxi = 0; dxi < 3; ++dxi) {
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const float4* w = wb + ((dxi * 4 + qq) * kWave + lane) * 2;
+      const float4 w0 = w[0], w1 = w[1];
+      const float wv[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          acc[r][qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(fs[r + dxi][kk], wv[kk], acc[r][qq], 0, 0, 0);
+    }
+  }
+}
+
+template <int NT, int DYI>
+__device__ __forceinline__ void rs_step(const float* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
+                                        int n_slabs, int LX, int x0, int t, int kh, rc_f32x4 (&acc)[4][4]) {
+  constexpr int kPer = rb_per_thread<NT>();
+  const bool more = k + 1 < n_slabs;
+  rb_i32x4 v[kPer];
+  if (more) {
+    const rb_i32x4* src = Wsl + (size_t)(k + 1) * 2 * kSlabFrag;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) v[i] = src[t + i * NT];
+  }
+  rs_taps<DYI>(act, reinterpret_cast<const float4*>(wbuf + (k & 1) * 2 * kSlabFrag), LX, x0, t & (kWave - 1), kh,
+               acc);
+  if (more) {
+    rb_i32x4* dst = wbuf + ((k + 1) & 1) * 2 * kSlabFrag;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) dst[t + i * NT] = v[i];
+  }
+  __syncthreads();
+  ++k;
+}
+
+template <int NT>
+__device__ __forceinline__ void rs_conv(const float* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
+                                        int n_slabs, const float* __restrict__ bl, int LX, int x0, int t,
+                                        rc_f32x4 (&acc)[4][4]) {
+  const int j = t & 15;
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+    const float bv = bl[16 * qq + j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][qq] = rc_f32x4{bv, bv, bv, bv};
+  }
+#pragma unroll 1
+  for (int kh = 0; kh < 2; ++kh) {
+    rs_step<NT, 0>(act, wbuf, Wsl, k, n_slabs, LX, x0, t, kh, acc);
+    rs_step<NT, 1>(act, wbuf, Wsl, k, n_slabs, LX, x0, t, kh, acc);
+    rs_step<NT, 2>(act, wbuf, Wsl, k, n_slabs, LX, x0, t, kh, acc);
+  }
+}
+
+__host__ __device__ constexpr size_t rs_lds_floats(int N) {
+  return (size_t)N * kRcStride + 16 * kSlabFrag + 16;  // activations, two slabs, partials
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const float* __restrict__ emb,
+                                                   const rb_i32x4* __restrict__ Wsl, const float* __restrict__ Bc,
+                                                   const float* __restrict__ head, float out_scale,
+                                                   const int8_t* __restrict__ s,
+                                                   const int32_t* __restrict__ item_b,
+                                                   const int32_t* __restrict__ item_hop, int img0, int img_end,
+                                                   int ipi, float* __restrict__ Y) {
+  extern __shared__ float lds[];
+  const int N = LX * kRcLY;
+  float* act = lds;
+  rb_i32x4* wbuf = reinterpret_cast<rb_i32x4*>(lds + (size_t)N * kRcStride);
+  float* part = lds + (size_t)N * kRcStride + 16 * kSlabFrag;
+  const int img = img0 + blockIdx.x;
+  if (img >= img_end) return;
+  const int item = img / ipi;
+  const int grp = (ipi == kGroup) ? (img & 7) : 0;
+  const int32_t hop = item_hop ? item_hop[item] : kHopNone;
+  if (hop == kHopInvalid) return;
+  const int b = item_b ? item_b[item] : item;
+  const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+  constexpr int nw = NT / kWave;
+  const int x0 = 4 * wave, j = lane & 15, g16 = lane >> 4;
+  const int n_slabs = 2 * n_blocks * kSlabsPerConv;
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < rb_per_thread<NT>(); ++i) wbuf[t + i * NT] = Wsl[t + i * NT];  // slab 0
+
+  rc_f32x4 h[4][4];
+  {
+    const int8_t* sb = s + (size_t)b * 2 * N;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int site = (x0 + r) * kRcLY + 4 * g16 + q;
+        const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY)) > 0;
+        const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
+        const float* e = emb + (up + 2 * dn) * kRcC + j;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) h[r][qq][q] = e[16 * qq];
+      }
+  }
+  const float inv_count = 1.0f / (float)(kRcC * N);
+  for (int blk = 0; blk < n_blocks; ++blk) {
+    float ss = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ss = fmaf(h[r][qq][q], h[r][qq][q], ss);
+    ss = rc_wave_sum(ss);
+    if (lane == 0) part[wave] = ss;
+    __syncthreads();  // partial sums (and, first time, slab 0) visible
+    float tot = 0.f;
+#pragma unroll
+    for (int w = 0; w < nw; ++w) tot += part[w];
+    const float rinv = 1.0f / sqrtf(fmaf(tot, inv_count, kRmsEps));
+    rc_store(act, h, rinv, x0, lane);
+    __syncthreads();
+    rc_f32x4 acc[4][4];
+    rs_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk) * kRcC, LX, x0, t, acc);
+    rc_store(act, acc, 1.0f, x0, lane);  // the last slab barrier ended every read of act
+    __syncthreads();
+    rs_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk + 1) * kRcC, LX, x0, t, acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
+  }
+  if (head) {  // BCS: the last slab barrier ended every read of act
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          act[((x0 + r) * kRcLY + 4 * g16 + q) * kRcStride + 16 * qq + j] = out_scale * h[r][qq][q];
+    __syncthreads();
+    rc_head(act, head, N, Y + (size_t)(img - img0) * (2 * N * kRcC));
+    return;
+  }
+  float* yo = Y + (size_t)(img - img0) * (2 * N * (kRcC / 2));
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int site = (x0 + r) * kRcLY + 4 * g16 + q;
+        yo[((size_t)(qq >> 1) * N + site) * (kRcC / 2) + 16 * (qq & 1) + j] = out_scale * h[r][qq][q];
+      }
+}
+
 }  // namespace
 
 bool resconv_supported(const nh_model* m) {
@@ -523,6 +706,29 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
         break;
       default:
         hipLaunchKernelGGL(k_resconv_bf16<256>, dim3(nimg), dim3(256), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
+                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+    }
+    NH_LAUNCH_CHECK();
+    return NH_OK;
+  }
+  if (m->conv_w_mfma) {  // FP32 weight slabs
+    const size_t lds_fs = sizeof(float) * rs_lds_floats(N);
+    const rb_i32x4* wsl = reinterpret_cast<const rb_i32x4*>(m->conv_w_mfma);
+    switch (m->LX) {
+      case 4:
+        hipLaunchKernelGGL(k_resconv_fs<64>, dim3(nimg), dim3(64), lds_fs, st, m->LX, m->n_layers, m->conv_w, wsl,
+                           m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+        break;
+      case 8:
+        hipLaunchKernelGGL(k_resconv_fs<128>, dim3(nimg), dim3(128), lds_fs, st, m->LX, m->n_layers, m->conv_w,
+                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+        break;
+      case 12:
+        hipLaunchKernelGGL(k_resconv_fs<192>, dim3(nimg), dim3(192), lds_fs, st, m->LX, m->n_layers, m->conv_w,
+                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+        break;
+      default:
+        hipLaunchKernelGGL(k_resconv_fs<256>, dim3(nimg), dim3(256), lds_fs, st, m->LX, m->n_layers, m->conv_w,
                            wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
     }
     NH_LAUNCH_CHECK();

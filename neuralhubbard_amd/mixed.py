@@ -88,6 +88,27 @@ def pack_resconv_bf16(params) -> np.ndarray:
     return np.stack([hi, lo], axis=3).reshape(-1)  # (n, kh, dyi, plane, dxi, qq, lane, j)
 
 
+def pack_resconv_f32(params) -> np.ndarray:
+    """FP32 weights of the residual CNN's convolutions as B fragments of v_mfma_f32_16x16x4_f32, in
+    the same slabs as `pack_resconv_bf16`: for conv, K-half kh, tap row dyi, tap column dxi,
+    quarter qq, lane and K-step kk < 8:
+        W[tap = 3 dxi + dyi][ci = 32 kh + 4 kk + lane // 16][co = 16 qq + lane % 16]."""
+    W = np.stack([np.asarray(w, np.float32) for w in params.conv_w])  # (n, co, ci, 3, 3)
+    n, Cc = W.shape[0], W.shape[1]
+    T = np.transpose(W, (0, 3, 4, 2, 1)).reshape(n, 9, Cc, Cc)  # [conv][tap][ci][co]
+    kh = np.arange(2).reshape(2, 1, 1, 1, 1, 1)
+    dyi = np.arange(3).reshape(1, 3, 1, 1, 1, 1)
+    dxi = np.arange(3).reshape(1, 1, 3, 1, 1, 1)
+    qq = np.arange(4).reshape(1, 1, 1, 4, 1, 1)
+    lane = np.arange(64).reshape(1, 1, 1, 1, 64, 1)
+    kk = np.arange(8).reshape(1, 1, 1, 1, 1, 8)
+    shape = (2, 3, 3, 4, 64, 8)
+    tap = np.broadcast_to(3 * dxi + dyi, shape)
+    ci = np.broadcast_to(32 * kh + 4 * kk + lane // 16, shape)
+    co = np.broadcast_to(16 * qq + lane % 16, shape)
+    return np.ascontiguousarray(T[:, tap, ci, co]).reshape(-1)  # (n, kh, dyi, dxi, qq, lane, kk)
+
+
 def pack_lowp(model):
     import torch
 

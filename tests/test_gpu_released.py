@@ -8,7 +8,8 @@
   the released chains against the paper: E/N = -0.74533(1), sigma^2/N = 0.0272(5)
   (code/states.py:114-115) and E/N = -0.75878(2), sigma^2/N = 0.0255(20) (:104-105).
 
-Both precisions of the network: FP32 MFMA and split-BF16 MFMA ("mixed").  Tolerances: features
+Both precisions of the network: FP32 MFMA (with the LDS weight slabs, and reading the weights
+directly: "fp32-direct") and split-BF16 MFMA ("mixed").  Tolerances: features
 1e-4 (FP32) or 1e-3 (split BF16) relative to their largest magnitude, log|psi| as described in
 the small-lattice test and 1e-3 at 16x16 (the bar for the 16x16 Pfaffian), signs exact, E_loc
 1e-4 per site.  The E/N check allows four standard errors of the
@@ -61,7 +62,7 @@ def _sensitivity(wk, p, S, project, rel=1e-7, trials=3):
     return np.max([np.abs(_Perturbed(wk, p, rel, k).psi_batch(S, project)[1] - base) for k in range(trials)], 0)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("prec", ["fp32", "fp32-direct", "mixed"])
 @pytest.mark.parametrize("kind,LX,n_el,blocks", [("pfaffian", 4, 12, 2), ("pfaffian", 8, 28, 3), ("bcs", 4, 12, 2)])
 def test_small_resconv_matches_oracle(kind, LX, n_el, blocks, prec):
     """Random weights give some configurations nearly singular pairing matrices (condition
@@ -75,7 +76,9 @@ def test_small_resconv_matches_oracle(kind, LX, n_el, blocks, prec):
     print(f"resconv {kind} {LX}x16 n_el={n_el} blocks={blocks} {prec}", flush=True)
     mixed = prec == "mixed"
     ref = O.Model(wk, p)
-    dev = build_backflow(wk, p, prec)
+    dev = build_backflow(wk, p, "mixed" if mixed else "fp32")
+    if prec == "fp32-direct":  # the FP32 kernel reading conv_w without the weight slabs
+        dev.conv_w_mfma = None
     y = dev.features(torch.tensor(S, device="cuda")).cpu().numpy()
     yr = ref.features_batch(S)
     np.testing.assert_allclose(y, yr, rtol=0, atol=(1e-3 if mixed else 1e-4) * np.abs(yr).max())

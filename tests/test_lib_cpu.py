@@ -92,6 +92,23 @@ def test_pack_resconv_bf16_layout():
         assert np.all(np.abs(rec[:, kh, dyi, dxi, qq] - want) <= 2.0**-16 * np.abs(want))
 
 
+def test_pack_resconv_f32_layout():
+    from neuralhubbard_amd.mixed import pack_resconv_f32
+
+    wk = small_workload("pfaffian", LX=4, LY=16, n_up=12, n_dn=12, channels=64, n_layers=1, network="resconv")
+    p = make_params(wk)
+    f = pack_resconv_f32(p).reshape(2, 2, 3, 3, 4, 64, 8)
+    # conv 1, kh 1, dyi 2 (dy = +1), dxi 0 (dx = -1), quarter 3, lane 50, kk 6:
+    # ci = 32 + 24 + 50 // 16 = 59, co = 48 + 2 = 50
+    assert f[1, 1, 2, 0, 3, 50, 6] == p.conv_w[1][50, 59, 0, 2]
+    full = np.stack([np.transpose(c, (2, 3, 1, 0)).reshape(9, 64, 64) for c in p.conv_w])
+    lane = np.arange(64)
+    for kh, dyi, dxi, qq in ((0, 0, 0, 0), (1, 1, 2, 1)):
+        want = full[:, 3 * dxi + dyi][:, 32 * kh + 4 * np.arange(8)[None, :] + (lane // 16)[:, None],
+                                      (16 * qq + lane % 16)[:, None]]
+        np.testing.assert_array_equal(f[:, kh, dyi, dxi, qq], want)
+
+
 def test_argument_errors_without_gpu():
     lib = _lib.load()
     m = _lib.NhModel(0, 16, 16, 112, 112, 32, 6, 0, 1, 0, 0, 0, 0, 0, 0)  # null weights
