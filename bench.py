@@ -261,7 +261,7 @@ def main():
     achieved = flops[dom] / (ms * 1e-3)
     arith = "bf16" if (head == "mixed-da" and dom == "features") else "fp32"
     traffic = read_traffic(images / max(1, launches)) if (dom == "pairing+pfaffian" and not resconv) else None
-    cnn_kernel = ("k_resconv_bf16" if arith == "bf16" else "k_resconv") if resconv else "k_cnn_mfma"
+    cnn_kernel = ("k_resconv_bf16" if arith == "bf16" else "k_resconv_fs") if resconv else "k_cnn_mfma"
     roof = {"bound": "mfma", "kernel": "k_image_pf" if dom == "pairing+pfaffian" else cnn_kernel,
             "achieved": achieved / 1e12, "peak": PEAK[arith] / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK[arith], "traffic": traffic, "launches": launches,
