@@ -17,6 +17,12 @@ Pinning status per function (see the tests named in brackets):
   build_wtilde / wt_ud      antisymmetry identity + normalisation    [test_oracle_wtilde]
   conv / features           scipy.ndimage.correlate(mode="wrap")     [test_oracle_conv]
   pairing_occ               full 2N x 2N einsum path, sliced         [test_oracle_pairing]
+  apply_fermion_ops /       exact expectation over |psi|^2 of the local
+  transverse_local          values = <psi|O|psi> with O built from
+                            Jordan-Wigner matrices on the full Fock
+                            space; hermiticity <S+_a S-_j> =
+                            <S-_a S+_j>                              [test_oracle_transverse_*]
+  sz_autocorr               FFT form                                 [test_oracle_transverse_*]
   pairing_occ_vjp /         finite differences of <G, F(Y, Wt)> and of
   slogpf_vjp                log|Pf| along random skew directions      [test_oracle_vjp_*]
   slogpf                    Pf^2 = det, exact rational expansion,
@@ -469,6 +475,75 @@ def eloc(model, s, U, t, tprime, project=True, psi_s=None):
         sg, la = model.psi(sp, project)
         e += h * sg * psi_s[0] * math.exp(la - psi_s[1])
     return e
+
+
+# ------------------------------------------------------------------------ spin correlations
+
+
+def apply_fermion_ops(occ, ops):
+    """Apply c / c+ operators, listed left to right as written and acting right to left, to the
+    basis state c+_{m1} ... c+_{mk}|0> of the ascending modes `occ`: each operator contributes
+    (-1)^(occupied modes below its mode).  Returns (sign, occ') or None for a zero result."""
+    occ = sorted(occ)
+    sign = 1
+    for m, create in reversed(ops):
+        k = sum(1 for x in occ if x < m)
+        if create == (m in occ):
+            return None
+        sign *= (-1) ** k
+        if create:
+            occ.insert(k, m)
+        else:
+            occ.remove(m)
+    return sign, tuple(occ)
+
+
+def spin_flip(i, k, N):
+    """S+_i S-_k = c+_{i up} c_{i dn} c+_{k dn} c_{k up} as an operator list."""
+    return [(i, True), (N + i, False), (N + k, True), (k, False)]
+
+
+def transverse_local(s, a, LX, LY, psi_fn):
+    """(2, N) local values of S+_a S-_{a+r} (row 0) and S-_a S+_{a+r} (row 1): sum over s' of
+    <s|O|s'> psi(s') / psi(s), <s|O|s'> = <s'|O^dagger|s> from operator algebra, r = 0 the
+    diagonal n_up (1 - n_dn), n_dn (1 - n_up) of site a (``code/measure/transverse_spin.py``).
+    psi_fn(s) -> (sign, log|psi|)."""
+    N = LX * LY
+    s = np.asarray(s)
+    occ = tuple(int(m) for m in np.flatnonzero(s > 0))
+    ps = psi_fn(s)
+    ax, ay = divmod(a, LY)
+    T = np.zeros((2, N))
+    T[0, 0] = float(s[a] > 0 and s[N + a] < 0)
+    T[1, 0] = float(s[N + a] > 0 and s[a] < 0)
+    for j in range(N):
+        if j == a:
+            continue
+        r = ((j // LY - ax) % LX) * LY + (j % LY - ay) % LY
+        # O^dagger of S+_a S-_j is S+_j S-_a; that of S-_a S+_j is S+_a S-_j
+        for chan, ops in ((0, spin_flip(j, a, N)), (1, spin_flip(a, j, N))):
+            res = apply_fermion_ops(occ, ops)
+            if res is None:
+                continue
+            sign, occ2 = res
+            s2 = -np.ones(2 * N, np.int8)
+            s2[list(occ2)] = 1
+            sg, la = psi_fn(s2)
+            T[chan, r] += sign * sg * ps[0] * math.exp(la - ps[1])
+    return T
+
+
+def sz_autocorr(S, LX, LY):
+    """Batch mean of (1/N) sum_i Sz_i Sz_{i+r}, Sz = (n_up - n_dn) / 2, by direct sums
+    (``code/measure/observables.py:179-206``)."""
+    S = np.atleast_2d(np.asarray(S, np.float64))
+    N = LX * LY
+    sz = ((S[:, :N] - S[:, N:]) / 4).reshape(-1, LX, LY)
+    out = np.zeros((LX, LY))
+    for rx in range(LX):
+        for ry in range(LY):
+            out[rx, ry] = np.mean(np.sum(sz * np.roll(sz, (-rx, -ry), axis=(1, 2)), axis=(1, 2))) / N
+    return out.reshape(-1)
 
 
 # ------------------------------------------------------------------------ statistics
