@@ -8,8 +8,9 @@ Mirrors the spin channel of the reference's measurement (``code/measure/observab
 * ``T[0, r]`` = <S^+_a S^-_{a+r}>, ``T[1, r]`` = <S^-_a S^+_{a+r}> for a reference site a, from
   local values sum_{s'} <s|O|s'> psi(s') / psi(s) over the configurations s' that the terms connect
   (`transverse_local`; r = 0 holds n_up (1 - n_dn) and n_dn (1 - n_up) of site a);
-* ``C_s(r) = C_zz(r) + (T[0, r] + T[1, r]) / 2`` (`spin_correlation`), and the charge
-  correlation ``C_c(r)`` of the densities (`charge_correlation`).
+* ``C_s(r) = C_zz(r) + (T[0, r] + T[1, r]) / 2`` (`spin_correlation`), its structure factor and
+  the stripe order ``M^2_str`` (`structure_factor`, `m2_str`, ``code/measure/assemble.py:241-276``),
+  and the charge correlation ``C_c(r)`` of the densities (`charge_correlation`).
 
 A term S^+_a S^-_j connects s to s' with the up electron of a moved to j and the down electron of
 j moved to a (S^-_a S^+_j: the up electron of j to a, the down electron of a to j), the order in
@@ -149,3 +150,28 @@ def spin_correlation(state, spins, psi, LX: int, LY: int, refs=None):
     """(C_s (N,), C_zz (N,)) sample means (``code/measure/assemble.py:230-238``)."""
     cs = spin_correlation_samples(state, spins, psi, LX, LY, refs)
     return cs.mean(0), autocorr(sz_grid(spins, LX * LY), LX, LY)
+
+
+FAMILIES = {"pi8": (1, 8), "pi4": (1, 4)}  # stripe wavevector q / pi (``observables.paper_family``)
+
+
+def stripe_k(LX: int, LY: int, family: str):
+    """Integer momenta of k_alpha = (pi -+ q, pi), (pi, pi -+ q) (``observables.stripe_k``)."""
+    num, den = FAMILIES[family]
+    if (num * LX) % (2 * den) or (num * LY) % (2 * den):
+        raise ValueError(f"q = {num}/{den} pi is not on the momentum grid of {LX}x{LY}")
+    nx, ny = num * LX // (2 * den), num * LY // (2 * den)
+    hx, hy = LX // 2, LY // 2
+    return ((hx - nx, hy), (hx + nx, hy), (hx, hy - ny), (hx, hy + ny))
+
+
+def structure_factor(C, LX: int, LY: int):
+    """S(k) = sum_r e^{ik.r} C(r) on the momentum grid, (..., LX, LY) (``assemble.structure_factor``)."""
+    C = np.asarray(C, dtype=np.float64)
+    return (np.fft.ifft2(C.reshape(*C.shape[:-1], LX, LY), axes=(-2, -1)) * (LX * LY)).real
+
+
+def m2_str(C, LX: int, LY: int, family: str):
+    """M^2_str = (1/4N) sum_alpha S(k_alpha) (``assemble.family_mean`` of ``assemble.spin``)."""
+    Sk = structure_factor(C, LX, LY)
+    return np.mean([Sk[..., kx % LX, ky % LY] for kx, ky in stripe_k(LX, LY, family)], axis=0) / (LX * LY)

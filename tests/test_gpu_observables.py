@@ -6,7 +6,9 @@
 * the released tpm02_L16_pfaffian state: C_s(r) = C_zz(r) + (C_+-(r) + C_-+(r)) / 2 over its
   2048 released chains against the map the reference publishes
   (data/supplement/real_space_corr/figS17_real_space_corr_tpm02_16x16_pfaffian/maps.npz, 128 sweeps
-  x 8192 samples), four standard errors of our 2048-sample mean plus three of the published one.
+  x 8192 samples), and the stripe order M^2_str against Fig. 4 panel c
+  (data/fig4_order_params_tpm02/panel_c_orders.csv): four standard errors of our 2048-sample mean
+  plus three of the published one.
 """
 
 import os
@@ -50,13 +52,13 @@ def test_transverse_local_matches_oracle():
 
 def test_released_spin_correlation_matches_published_map():
     from neuralhubbard_amd.backflow import Variational, build_backflow
-    from neuralhubbard_amd.observables import spin_correlation_samples
+    from neuralhubbard_amd.observables import m2_str, spin_correlation_samples
     from tests.test_gpu_released import _released
 
     info, wk, p, S, _ = _released("tpm02_L16_pfaffian")
     L = wk.LX
     with np.load(os.path.join(GOLDEN, "real_space_corr_tpm02_L16.npz"), allow_pickle=False) as f:
-        Cs_pub, Cs_err = f["Cs"], f["Cs_err"]
+        Cs_pub, Cs_err, M2_pub, M2_err = f["Cs"], f["Cs_err"], float(f["M2_str"]), float(f["M2_str_err"])
     state = Variational(build_backflow(wk, p, "mixed"))
     s = torch.tensor(S, device="cuda")
     psi = state(s)
@@ -68,3 +70,8 @@ def test_released_spin_correlation_matches_published_map():
     for r in ((0, 0), (0, 1), (1, 0), (1, 1), (0, 2), (2, 0), (1, 2), (2, 1)):
         print(f"C_s{r}: {mean[r]:+.5f} +- {err[r]:.5f}, published {Cs_pub[r]:+.5f} +- {Cs_err[r]:.5f}", flush=True)
         assert abs(mean[r] - Cs_pub[r]) < 4 * err[r] + 3 * Cs_err[r]
+    # stripe order M^2_str = (1/4N) sum_alpha S(k_alpha), q = pi/4 (Fig. 4 panel c)
+    m2 = m2_str(per, L, L, "pi4")
+    m2_mean, m2_err = m2.mean(), m2.std(ddof=1) / np.sqrt(len(m2))
+    print(f"M2_str {m2_mean:.5f} +- {m2_err:.5f}, published {M2_pub:.6f} +- {M2_err:.6f}", flush=True)
+    assert abs(m2_mean - M2_pub) < 4 * m2_err + 3 * M2_err

@@ -46,3 +46,13 @@ def test_sz_autocorr_matches_direct_sums():
     S = np.where(rng.random((5, 2 * 24)) < 0.4, 1, -1).astype(np.int8)
     got = obs.autocorr(obs.sz_grid(torch.tensor(S), 24), 4, 6).numpy()
     np.testing.assert_allclose(got, O.sz_autocorr(S, 4, 6), atol=1e-12)
+
+
+def test_m2_str_of_published_map_matches_published_value():
+    """The structure factor and stripe momenta (assemble.py:241-276): M^2_str of the published
+    C_s map is the published M^2_str of Fig. 4 panel c (0.011789(9)); pi/8 stripes would give
+    another value."""
+    with np.load(os.path.join(GOLDEN, "real_space_corr_tpm02_L16.npz"), allow_pickle=False) as f:
+        Cs, M2 = f["Cs"].reshape(-1), float(f["M2_str"])
+    assert abs(obs.m2_str(Cs, 16, 16, "pi4") - M2) < 1e-6
+    assert abs(obs.m2_str(Cs, 16, 16, "pi8") - M2) > 1e-3
