@@ -23,6 +23,12 @@ Pinning status per function (see the tests named in brackets):
                             space; hermiticity <S+_a S-_j> =
                             <S-_a S+_j>                              [test_oracle_transverse_*]
   sz_autocorr               FFT form                                 [test_oracle_transverse_*]
+  dwave_local               exact |psi|^2 averages = <psi|A+A|psi>,
+                            <psi|c+c|psi> from Jordan-Wigner Fock
+                            matrices, every left bond, orientation and
+                            right term; C_p of the assembled means =
+                            <Delta+_0 Delta_r> built directly on the
+                            Fock space over all bonds                 [test_oracle_dwave_*]
   pairing_occ_vjp /         finite differences of <G, F(Y, Wt)> and of
   slogpf_vjp                log|Pf| along random skew directions      [test_oracle_vjp_*]
   slogpf                    Pf^2 = det, exact rational expansion,
@@ -531,6 +537,60 @@ def transverse_local(s, a, LX, LY, psi_fn):
             sg, la = psi_fn(s2)
             T[chan, r] += sign * sg * ps[0] * math.exp(la - ps[1])
     return T
+
+
+def _bond_ends(x, y, ax, LX, LY):
+    """Site indices (lower, upper) of the bond keyed (x, y, axis) (``observables.py:93-96``)."""
+    hi = ((x + 1) % LX, y) if ax == 0 else (x, (y + 1) % LY)
+    return x * LY + y, hi[0] * LY + hi[1]
+
+
+def dwave_local(s, LX, LY, psi_fn):
+    """Local values (F (4, 4N), G (3, N, 2)) of the d-wave terms at R = (0, 0)
+    (``code/measure/dwave_packed.py``): F[2 l + oL, 2 (2 site + axis) + oR] of
+    A+(bl, oL) A(br, oR), A+(bond, o) = c+_{a up} c+_{b dn} with (a, b) = (lower, upper) for o = 0
+    and swapped for o = 1, for the left bonds bl = (0, 0, 0), (0, 0, 1); G[li, j, s] of
+    c+_{i s} c_{j s} for the left sites i = (0, 0), (0, 1), (1, 0).  Each local value is
+    sum_{s'} <s'|O^dagger|s> psi(s') / psi(s); the diagonal terms are the occupations."""
+    N = LX * LY
+    s = np.asarray(s)
+    occ = tuple(int(m) for m in np.flatnonzero(s > 0))
+    ps = psi_fn(s)
+    F = np.zeros((4, 4 * N))
+    G = np.zeros((3, N, 2))
+
+    def amp_ratio(ops):
+        res = apply_fermion_ops(occ, ops)
+        if res is None:
+            return 0.0
+        sign, occ2 = res
+        s2 = -np.ones(2 * N, np.int8)
+        s2[list(occ2)] = 1
+        sg, la = psi_fn(s2)
+        return sign * sg * ps[0] * math.exp(la - ps[1])
+
+    for l, ax in enumerate((0, 1)):
+        lo, hi = _bond_ends(0, 0, ax, LX, LY)
+        for oL in (0, 1):
+            a, b = (lo, hi) if oL == 0 else (hi, lo)
+            for site in range(N):
+                for bax in (0, 1):
+                    clo, chi = _bond_ends(site // LY, site % LY, bax, LX, LY)
+                    for oR in (0, 1):
+                        c, d = (clo, chi) if oR == 0 else (chi, clo)
+                        t = 2 * (2 * site + bax) + oR
+                        if (c, d) == (a, b):
+                            F[2 * l + oL, t] = float(s[a] > 0 and s[N + b] > 0)
+                        else:  # O^dagger = c+_{c up} c+_{d dn} c_{b dn} c_{a up}
+                            F[2 * l + oL, t] += amp_ratio([(c, True), (N + d, True), (N + b, False), (a, False)])
+    for li, i in enumerate((0, 1, LY)):
+        for sp in (0, 1):
+            for j in range(N):
+                if j == i:
+                    G[li, j, sp] = float(s[sp * N + i] > 0)
+                else:  # O = c+_i c_j, O^dagger = c+_j c_i
+                    G[li, j, sp] = amp_ratio([(sp * N + j, True), (sp * N + i, False)])
+    return F, G
 
 
 def sz_autocorr(S, LX, LY):

@@ -8,7 +8,12 @@
   (data/supplement/real_space_corr/figS17_real_space_corr_tpm02_16x16_pfaffian/maps.npz, 128 sweeps
   x 8192 samples), and the stripe order M^2_str against Fig. 4 panel c
   (data/fig4_order_params_tpm02/panel_c_orders.csv): four standard errors of our 2048-sample mean
-  plus three of the published one.
+  plus three of the published one;
+* the d-wave terms F and G per sample from `observables.dwave_local` against the FP64 oracle
+  (pinned to Jordan-Wigner Fock matrices and the direct <Delta+_0 Delta_r> in tests/test_oracle.py),
+  and the pair correlation C_p(r) of both released 16x16 states over their 2048 released chains
+  against the published maps (figS17 for tpm02_L16_pfaffian, figS12 for tp0_L16_bcs) and
+  Delta^2_SC of Fig. 3 / Fig. 4 panel c.
 """
 
 import os
@@ -75,3 +80,68 @@ def test_released_spin_correlation_matches_published_map():
     m2_mean, m2_err = m2.mean(), m2.std(ddof=1) / np.sqrt(len(m2))
     print(f"M2_str {m2_mean:.5f} +- {m2_err:.5f}, published {M2_pub:.6f} +- {M2_err:.6f}", flush=True)
     assert abs(m2_mean - M2_pub) < 4 * m2_err + 3 * M2_err
+
+
+def test_dwave_local_matches_oracle():
+    from neuralhubbard_amd.backflow import Variational, build_backflow
+    from neuralhubbard_amd.configs import make_configurations, make_params, small_workload
+    from neuralhubbard_amd.observables import dwave_local
+
+    for kind in ("pfaffian", "bcs"):
+        wk = small_workload(kind=kind, LX=4, LY=4, n_up=5, n_dn=5, tprime=-0.2, channels=8, n_layers=2,
+                            base_f0=True, seed=3)
+        p = make_params(wk)
+        ref = O.Model(wk, p)
+        state = Variational(build_backflow(wk, p))
+        S = make_configurations(wk, 6)
+        s = torch.tensor(S, device="cuda")
+        psi = state(s)
+        print(f"d-wave terms, {kind} 4x4, {len(S)} samples", flush=True)
+        F, G = dwave_local(state, s, psi, 4, 4)
+        F, G = F.cpu().numpy(), G.cpu().numpy()
+        # every value is psi(s') / psi(s) times a sign: two FP32 log|psi| within the 1e-3 bound of
+        # FP64 leave the ratio within exp(2e-3) - 1
+        for b in range(len(S)):
+            f, g = O.dwave_local(S[b], 4, 4, ref.psi)
+            np.testing.assert_allclose(F[b], f, rtol=2e-3, atol=1e-6 * max(1.0, np.abs(f).max()))
+            np.testing.assert_allclose(G[b], g, rtol=2e-3, atol=1e-6 * max(1.0, np.abs(g).max()))
+            np.testing.assert_array_equal(np.sign(F[b][np.abs(f) > 1e-6]), np.sign(f[np.abs(f) > 1e-6]))
+            np.testing.assert_array_equal(np.sign(G[b][np.abs(g) > 1e-6]), np.sign(g[np.abs(g) > 1e-6]))
+
+
+@pytest.mark.parametrize("name,fixture", [("tpm02_L16_pfaffian", "real_space_corr_tpm02_L16.npz"),
+                                          ("tp0_L16_bcs", "real_space_corr_tp0_L16_bcs.npz")])
+def test_released_pair_correlation_matches_published_map(name, fixture):
+    """C_p(r) from 16 blocks of 128 released chains (leave-one-block-out errors) against the
+    published map at the shortest displacements and Delta^2_SC against the published value: four
+    of our standard errors plus three of the published ones."""
+    from neuralhubbard_amd.backflow import Variational, build_backflow
+    from neuralhubbard_amd.observables import dwave, dwave_measure
+    from tests.test_gpu_released import _released
+
+    info, wk, p, S, _ = _released(name)
+    L = wk.LX
+    with np.load(os.path.join(GOLDEN, fixture), allow_pickle=False) as f:
+        Cp_pub, Cp_err, d2_pub, d2_err = f["Cp"], f["Cp_err"], float(f["Delta_SC2"]), float(f["Delta_SC2_err"])
+    state = Variational(build_backflow(wk, p, "mixed"))
+    s = torch.tensor(S, device="cuda")
+    psi = state(s)
+    nblk = 16
+    blk = len(S) // nblk
+    Fs, Gs = [], []
+    for k in range(nblk):
+        sl = slice(k * blk, (k + 1) * blk)
+        F, G = dwave_measure(state, s[sl], type(psi)(psi.sign[sl], psi.logabs[sl]), L, L)
+        Fs.append(F.cpu().numpy())
+        Gs.append(G.cpu().numpy())
+        print(f"{name}: d-wave block {k + 1}/{nblk}", flush=True)
+    res = dwave(np.stack(Fs), np.stack(Gs), L, L)
+    Cp = res["C_p"].reshape(L, L)
+    Cp = 0.5 * (Cp + np.roll(Cp[::-1, ::-1], (1, 1), axis=(0, 1)))  # [C(r) + C(-r)] / 2
+    err = res["C_p_err"].reshape(L, L)
+    for r in ((0, 0), (0, 1), (1, 0), (1, 1), (0, 2), (2, 0), (2, 2), (0, 4)):
+        print(f"C_p{r}: {Cp[r]:+.6f} +- {err[r]:.6f}, published {Cp_pub[r]:+.6f} +- {Cp_err[r]:.6f}", flush=True)
+        assert abs(Cp[r] - Cp_pub[r]) < 4 * err[r] + 3 * Cp_err[r], r
+    d2, e2 = res["Delta2_SC"], res["Delta2_SC_err"]
+    print(f"Delta2_SC {d2:.6f} +- {e2:.6f}, published {d2_pub:.6f} +- {d2_err:.6f}", flush=True)
+    assert abs(d2 - d2_pub) < 4 * e2 + 3 * d2_err

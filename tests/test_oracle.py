@@ -465,12 +465,7 @@ def _fock_apply(v, m, create, nmodes):
     x = np.arange(v.size, dtype=np.int64)
     bit = np.int64(1) << (nmodes - 1 - m)
     above = (np.int64(1) << nmodes) - (bit << 1)  # the bits of the modes below m
-    y = x & above
-    par = np.zeros(v.size, np.int64)
-    while np.any(y):
-        par ^= y & 1
-        y >>= 1
-    sign = 1 - 2 * par
+    sign = 1 - 2 * (np.bitwise_count(x & above).astype(np.int64) & 1)
     has = (x & bit) != 0
     out = np.zeros_like(v)
     src = has if not create else ~has
@@ -528,3 +523,134 @@ def test_oracle_transverse_matches_jordan_wigner():
 
     t = torch.tensor(S)
     np.testing.assert_allclose(autocorr(sz_grid(t, N), LX, LY).numpy(), O.sz_autocorr(S, LX, LY), atol=1e-12)
+
+
+# ------------------------------------------------------------------------ d-wave pair terms (f4)
+
+
+def _dwave_exact_setup(kind="pfaffian"):
+    LX = LY = 3
+    N = 9
+    # no base orbital matrix F0: the state is then translation invariant (periodic convolutions,
+    # displacement-indexed pairing), which the translation fill of the assembly assumes
+    wk = small_workload(kind=kind, LX=LX, LY=LY, n_up=2, n_dn=2, tprime=-0.2, channels=4, n_layers=1,
+                        base_f0=False, seed=6)
+    m = O.Model(wk, make_params(wk))
+    basis = [u + tuple(N + d for d in dd) for u in itertools.combinations(range(N), 2)
+             for dd in itertools.combinations(range(N), 2)]
+    table = {occ: m.psi(config_of(occ, 2 * N)) for occ in basis}
+    nm = 2 * N
+    v = np.zeros(1 << nm)
+    for occ, (sg, la) in table.items():
+        v[sum(1 << (nm - 1 - k) for k in occ)] = sg * math.exp(la)
+    return LX, LY, N, basis, table, v
+
+
+def _fock_expect(v, ops, nm):
+    """<v|O|v> / <v|v> for O written left to right as (mode, create) operators."""
+    u = v
+    for mode, create in reversed(ops):
+        u = _fock_apply(u, mode, create, nm)
+    return float(v @ u) / float(v @ v)
+
+
+def _pair_ops(a, b, c, d, N):
+    """A+ A' = c+_{a up} c+_{b dn} c_{d dn} c_{c up} (A+ = c+_{a up} c+_{b dn}, A' its (c, d) pair)."""
+    return [(a, True), (N + b, True), (N + d, False), (c, False)]
+
+
+def test_oracle_dwave_terms_match_jordan_wigner():
+    """The exact |psi|^2 averages of the oracle's local F and G values equal <psi|A+(bl, oL)
+    A(br, oR)|psi> and <psi|c+_{i s} c_{j s}|psi> built on the dense Fock space by the independent
+    Jordan-Wigner routine, for both left bonds and orientations, every right bond and orientation
+    and every hopping term, on a 3x3 lattice with 2 + 2 electrons."""
+    LX, LY, N, basis, table, v = _dwave_exact_setup()
+    nm = 2 * N
+
+    def psi_fn(s):
+        return table[tuple(int(x) for x in np.flatnonzero(s > 0))]
+
+    top = max(t[1] for t in table.values())
+    w = {occ: math.exp(2 * (la - top)) for occ, (sg, la) in table.items()}
+    keep = [occ for occ in basis if table[occ][0] != 0]
+    wsum = sum(w[o] for o in basis)
+    F = np.zeros((4, 4 * N))
+    G = np.zeros((3, N, 2))
+    for occ in keep:
+        f, g = O.dwave_local(config_of(occ, nm), LX, LY, psi_fn)
+        F += w[occ] * f
+        G += w[occ] * g
+    F /= wsum
+    G /= wsum
+    for l, ax in enumerate((0, 1)):
+        lo, hi = (0, LY) if ax == 0 else (0, 1)
+        for oL in (0, 1):
+            a, b = (lo, hi) if oL == 0 else (hi, lo)
+            for site in range(N):
+                x, y = divmod(site, LY)
+                for bax in (0, 1):
+                    clo = site
+                    chi = ((x + 1) % LX) * LY + y if bax == 0 else x * LY + (y + 1) % LY
+                    for oR in (0, 1):
+                        c, d = (clo, chi) if oR == 0 else (chi, clo)
+                        exact = _fock_expect(v, _pair_ops(a, b, c, d, N), nm)
+                        got = F[2 * l + oL, 2 * (2 * site + bax) + oR]
+                        assert abs(got - exact) < 1e-10, (l, oL, site, bax, oR, got, exact)
+    for li, i in enumerate((0, 1, LY)):
+        for sp in (0, 1):
+            for j in range(N):
+                exact = _fock_expect(v, [(sp * N + i, True), (sp * N + j, False)], nm)
+                assert abs(G[li, j, sp] - exact) < 1e-10, (i, j, sp, G[li, j, sp], exact)
+    # keep the exact means for the assembly test
+    test_oracle_dwave_terms_match_jordan_wigner.means = (F, G)
+
+
+def test_oracle_dwave_pair_correlation_matches_direct_fock():
+    """C_p(r) assembled by `observables.DwaveIndex` from the measured terms at R = (0, 0) (the other
+    left bonds and sites by translation) equals (1/16) sum h h' <B+(bl) B(br)> built directly on
+    the Fock space for every pair of bonds, minus the Wick products of the exact <c+ c> of every
+    pair of sites (``code/measure/assemble.py:103-175``); without the subtraction it is
+    <Delta+_0 Delta_r> of main-text Eq. (2)."""
+    from neuralhubbard_amd.observables import DIRS, FORM_FACTOR, DwaveIndex, _bond_sites, _canon_bond
+
+    LX, LY, N, basis, table, v = _dwave_exact_setup()
+    nm = 2 * N
+    means = getattr(test_oracle_dwave_terms_match_jordan_wigner, "means", None)
+    if means is None:
+        test_oracle_dwave_terms_match_jordan_wigner()
+        means = test_oracle_dwave_terms_match_jordan_wigner.means
+    F, G = means
+    idx = DwaveIndex(LX, LY)
+    full = idx.pair_correlation(F, G)
+    bare = idx.pair_correlation(F, G, connected=False)
+
+    def ix(p):
+        return p[0] * LY + p[1]
+
+    cache = {}
+
+    def pair(a, b, c, d):
+        if (a, b, c, d) not in cache:
+            cache[a, b, c, d] = _fock_expect(v, _pair_ops(a, b, c, d, N), nm)
+        return cache[a, b, c, d]
+
+    def hop(i, j, sp):
+        if (i, j, sp) not in cache:
+            cache[i, j, sp] = _fock_expect(v, [(sp * N + i, True), (sp * N + j, False)], nm)
+        return cache[i, j, sp]
+
+    for n in range(N):
+        r2 = divmod(n, LY)
+        direct = wick = 0.0
+        for dl in DIRS:
+            a, b = map(ix, _bond_sites(_canon_bond((0, 0), dl, LX, LY), LX, LY))
+            for d2 in DIRS:
+                c, d = map(ix, _bond_sites(_canon_bond(r2, d2, LX, LY), LX, LY))
+                h = FORM_FACTOR[dl] * FORM_FACTOR[d2]
+                # B+ = (A+(o=0) + A+(o=1)) / sqrt 2: (a, b) and (b, a) on the left, (c, d), (d, c) right
+                direct += h * 0.5 * sum(pair(p, q, r, t) for p, q in ((a, b), (b, a)) for r, t in ((c, d), (d, c)))
+                wick += h * 0.5 * (hop(a, c, 0) * hop(b, d, 1) + hop(a, d, 0) * hop(b, c, 1)
+                                   + hop(b, c, 0) * hop(a, d, 1) + hop(b, d, 0) * hop(a, c, 1))
+        assert abs(bare[n] - direct / 16) < 1e-10, (n, bare[n], direct / 16)
+        assert abs(full[n] - (direct - wick) / 16) < 1e-10, (n, full[n], (direct - wick) / 16)
+    assert np.abs(full).max() > 1e-3  # the test is not trivially zero
