@@ -1,13 +1,14 @@
-"""Energy statistics of a run of sweeps (host side, float64), and the energy channel of the
-reference's measurement script with its file formats.
+"""Statistics of a run of sweeps (host side, float64) and the reference's measurement script with
+its file formats.
 
 Mirrors ``code/measure/assemble.py:48-97`` (``BLOCK_SIZES``, ``MIN_BLOCKS``, ``jackknife``,
 ``energy``): the per-sweep means of E_loc and E_loc^2 are grouped in blocks of 1, 2, 4 and 8
 consecutive sweeps (an incomplete last block is dropped); every block size that leaves at least
 8 blocks gives a leave-one-block-out jackknife error, and the quoted error is the largest.
-`run_energy` writes ``series.npz``, ``spins_last.npz`` and ``results.json`` as
-``code/measure_orderparams.py`` and ``code/measure/assemble.py`` do for the energy channel (the
-spin, charge and d-wave channels are not built).
+`run_measure` writes ``series.npz``, ``spins_last.npz``, ``results.json`` and ``results.npz`` as
+``code/measure_orderparams.py`` and ``code/measure/assemble.py`` do, for the energy, spin, charge
+and d-wave channels; `assemble` turns a series into E/N, M^2_str, N2(q_c), C_p(r) and
+Delta^2_SC.
 """
 
 from __future__ import annotations
@@ -51,8 +52,10 @@ def within_error_bars(a: dict, b: dict, k: float = 2.0) -> bool:
     return abs(a["E_per_site"] - b["E_per_site"]) <= k * float(np.hypot(a["E_err"], b["E_err"]))
 
 
-# ------------------------------------------------------------------------ energy channel of a run
+# ------------------------------------------------------------------------ a run of sweeps
 
+CHANNELS = ("energy", "spin", "charge", "dwave")
+NREF = 8  # reference sites of the transverse spin terms (code/measure_orderparams.py:27)
 SAVE_EVERY = 4  # sweeps between rewrites of series.npz (code/measure_orderparams.py:28)
 
 
@@ -71,86 +74,244 @@ def _json_value(v):
     return v
 
 
-def assemble_energy(series: dict) -> dict:
-    """The scalars and ``energy`` entry that ``assemble`` derives from a series
-    (``code/measure/assemble.py:301-311``)."""
+def _blocks(errs: dict) -> dict:
+    return {int(b): float(np.asarray(x)) for b, x in errs.items()}
+
+
+def spin(Czz, Tpm, Tmp, LX: int, LY: int, family: str) -> dict:
+    """C_s(r), S(k) and M^2_str (M2_zz: the zz part alone) from per-sweep Czz (S, N) and the
+    transverse terms Tpm, Tmp (S, N) or (S, NREF, N), reference sites averaged
+    (``assemble.spin_corr_series`` / ``assemble.spin``)."""
+    from .observables import stripe_k, structure_factor
+
+    Czz = np.asarray(Czz, np.float64)
+    Tpm, Tmp = np.asarray(Tpm, np.float64), np.asarray(Tmp, np.float64)
+    if Tpm.ndim == 3:
+        Tpm, Tmp = Tpm.mean(1), Tmp.mean(1)
+    Cs = Czz + 0.5 * (Tpm + Tmp)
+    N = LX * LY
+    ks = stripe_k(LX, LY, family)
+
+    def fam(Sk):
+        return np.mean([Sk[..., kx % LX, ky % LY] for kx, ky in ks], axis=0) / N
+
+    v, e, errs = jackknife(lambda x: x, fam(structure_factor(Cs, LX, LY)))
+    vz, ez, _ = jackknife(lambda x: x, fam(structure_factor(Czz, LX, LY)))
+    C, C_err, _ = jackknife(lambda x: x, Cs)
+    return {"C_s": C, "C_s_err": C_err, "S_k": structure_factor(C, LX, LY), "k_alpha": np.array(ks),
+            "family": family, "M2_str": float(v), "M2_str_err": float(e), "M2_str_err_blocks": _blocks(errs),
+            "M2_zz": float(vz), "M2_zz_err": float(ez), "nsweeps": int(len(Cs))}
+
+
+def charge(Cnn, LX: int, LY: int, family: str) -> dict:
+    """C_c(r), N(q) and N2 = (1/4N) sum_alpha N(q_alpha) at the charge wavevectors
+    (``assemble.charge``)."""
+    from .observables import charge_q, structure_factor
+
+    Cnn = np.asarray(Cnn, np.float64)
+    qs = charge_q(LX, LY, family)
+    C, C_err, _ = jackknife(lambda x: x, Cnn)
+    Nq = structure_factor(Cnn, LX, LY)
+    n2 = np.mean([Nq[..., qx % LX, qy % LY] for qx, qy in qs], axis=0) / (LX * LY)
+    v, e, _ = jackknife(lambda x: x, n2)
+    return {"C_c": C, "C_c_err": C_err, "N_q": structure_factor(C, LX, LY), "q_c": np.array(qs),
+            "N2": float(v), "N2_err": float(e), "nsweeps": int(len(Cnn))}
+
+
+def assemble(series: dict) -> dict:
+    """Every observable a series holds (``code/measure/assemble.py:301-327``)."""
+    from .observables import dwave
+
     LX, LY = int(series["LX"]), int(series["LY"])
-    res = {"LX": LX, "LY": LY, "family": str(series["family"])}
+    family = str(series["family"])
+    res = {"LX": LX, "LY": LY, "family": family}
     for k in ("state", "tprime", "nsamples"):
         if k in series:
             res[k] = np.asarray(series[k]).item()
     if "E_mean" in series and len(series["E_mean"]):
         res["energy"] = energy(series["E_mean"], series["E_sqr"], LX * LY)
+    if "Czz" in series and "Tpm" in series:
+        res["spin"] = spin(series["Czz"], series["Tpm"], series["Tmp"], LX, LY, family)
+    if "Cnn" in series:
+        res["charge"] = charge(series["Cnn"], LX, LY, family)
+    if "F" in series:
+        d = dwave(series["F"], series["G"], LX, LY)
+        dx, dy = np.divmod(np.arange(LX * LY), LY)
+        res["dwave"] = {"dr": np.stack([dx, dy], 1), **d, "translation_fill": True}
+    if "sz_site" in series:
+        sz = np.asarray(series["sz_site"], np.float64).mean(0)
+        n = np.asarray(series["n_site"], np.float64).mean(0)
+        res["local"] = {"max_abs_sz": float(np.abs(sz).max()), "n_min": float(n.min()), "n_max": float(n.max())}
     return res
 
 
+assemble_energy = assemble
+
+
 def summary(res: dict) -> str:
-    """First lines of the reference's summary (``code/measure/assemble.py:330-342``)."""
+    """The reference's summary lines (``code/measure/assemble.py:330-368``)."""
     lines = [f"{res.get('state', '')} {res['LX']}x{res['LY']}, stripe family {res['family']}"]
     if "energy" in res:
         e = res["energy"]
         lines.append(f"E/N = {e['E_per_site']:.6f} +- {e['E_err']:.1e}, "
                      f"sigma^2/N = {e['var_per_site']:.5f} +- {e['var_err']:.1e} ({e['nsweeps']} sweeps)")
+    if "dwave" in res:
+        d = res["dwave"]
+        lines.append(f"Delta^2_SC = {d['Delta2_SC']:.6f} +- {d['Delta2_SC_err']:.6f}, "
+                     f"Delta_SC = {d['Delta_SC']:.6f} +- {d['Delta_SC_err']:.6f} "
+                     f"(M = {d['n_far']} vectors with |r| >= {d['r_max']:.4f}, {d['nsweeps']} sweeps)")
+    if "spin" in res:
+        s = res["spin"]
+        lines.append(f"M^2_str = {s['M2_str']:.6f} +- {s['M2_str_err']:.6f} at k_alpha = {s['k_alpha'].tolist()}; "
+                     f"zz channel alone {s['M2_zz']:.6f}, C_s(0) = {s['C_s'][0]:.5f}")
+    if "charge" in res:
+        c = res["charge"]
+        lines.append(f"N2(q_c) = {c['N2']:.6f} +- {c['N2_err']:.6f} at q_c = {c['q_c'].tolist()}, "
+                     f"sum_r C_c(r) = {c['C_c'].sum():.1e}")
+    if "local" in res:
+        loc = res["local"]
+        lines.append(f"max |<S^z_i>| = {loc['max_abs_sz']:.1e}, <n_i> in [{loc['n_min']:.4f}, {loc['n_max']:.4f}]")
     return "\n".join(lines)
 
 
 def save_results(res: dict, out_dir: str) -> None:
-    """results.json with non-finite floats as null (``code/measure/assemble.py:383-399``)."""
+    """results.json (scalars, non-finite floats as null) and results.npz (the maps) in out_dir
+    (``code/measure/assemble.py:383-399``)."""
     import json
     import os
 
+    scalars, arrays = {}, {}
+    for key, val in res.items():
+        if isinstance(val, dict):
+            scalars[key] = {}
+            for k, v in val.items():
+                if isinstance(v, np.ndarray) and v.ndim > 0:
+                    arrays[k] = v
+                else:
+                    scalars[key][k] = _json_value(v)
+        else:
+            scalars[key] = _json_value(val)
     with open(os.path.join(out_dir, "results.json"), "w") as fh:
-        json.dump(_json_value(res), fh, indent=1, allow_nan=False)
+        json.dump(scalars, fh, indent=1, allow_nan=False)
+    np.savez(os.path.join(out_dir, "results.npz"), **arrays)
 
 
-def run_energy(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", tprime: float = 0.0,
-               log=print) -> dict:
-    """The energy channel of ``code/measure_orderparams.py:108-174``: every sweep's samples get
-    psi once and E_loc; per-sweep means of E_loc and E_loc^2 go to ``series.npz`` (rewritten every
-    SAVE_EVERY sweeps; sweeps with non-finite values excluded), the final chains to
-    ``spins_last.npz`` in the released-chain format, the assembled energy to ``results.json``.
-    `sampler` is a `sampler.MetropolisSampler`; returns the assembled results."""
+def run_measure(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", tprime: float = 0.0,
+                channels=CHANNELS, nbar: float | None = None, log=print) -> dict:
+    """``code/measure_orderparams.py:108-174``: every sweep's samples get psi once; the selected
+    channels add their per-sweep means to ``series.npz`` (rewritten every SAVE_EVERY sweeps; a sweep
+    with any non-finite value is excluded):
+
+    * energy: E_mean, E_sqr of E_loc;
+    * spin / charge: sz_site, n_site (LX, LY), Czz, Cnn (N,) and the transverse terms Tpm, Tmp
+      (NREF, N) of the reference sites `observables.reference_sites` — sample k serves reference
+      site k mod NREF, so that each site's terms are means over B / NREF samples (the reference
+      evaluates every site on every sample; both estimate the same expectation);
+    * dwave: F (16N,), G (6N,) in the order of F_keys, G_keys, which the series holds.
+
+    The final chains go to ``spins_last.npz`` in the released-chain format, the assembled
+    observables to ``results.json`` / ``results.npz``.  `sampler` is a `sampler.MetropolisSampler`
+    (or anything whose ``sweep()`` returns spins and their LogArray psi); returns the assembled
+    results."""
     import os
+    import time
 
-    from .backflow import Samples
+    import torch
+
+    from . import observables as obs
+    from .backflow import LogArray, Samples
     from .released import pack_spins
 
+    channels = tuple(channels)
+    if not set(channels) <= set(CHANNELS):
+        raise ValueError(f"channels must be a subset of {CHANNELS}, got {channels}")
     os.makedirs(out_dir, exist_ok=True)
     w = state.model.workload
-    series = {"state": name, "tprime": float(tprime), "LX": w.LX, "LY": w.LY, "family": paper_family(tprime),
+    LX, LY, N = w.LX, w.LY, w.N
+    nbar = (w.n_up + w.n_dn) / N if nbar is None else float(nbar)
+    family = paper_family(tprime)
+    series = {"state": name, "tprime": float(tprime), "LX": LX, "LY": LY, "family": family,
               "nsamples": int(sampler.s.shape[0])}
-    rec = {"E_mean": [], "E_sqr": []}
-    kept, excluded = [], []
+    if {"spin", "charge"} & set(channels):
+        obs.stripe_k(LX, LY, family)  # fail early if q is not on the momentum grid
+    if "spin" in channels:
+        refs = obs.reference_sites(LX, LY, NREF)
+        series["REF_SITES"] = np.array(refs)
+    if "charge" in channels:
+        series["nbar"] = nbar
+    if "dwave" in channels:
+        series["F_keys"], series["G_keys"] = obs.dwave_keys(LX, LY)
+    rec, kept, excluded = {}, [], []
     path = os.path.join(out_dir, "series.npz")
 
     def save():
         tmp = os.path.join(out_dir, "series_tmp.npz")
-        np.savez(tmp, **series, **{k: np.asarray(v, np.float64) for k, v in rec.items()},
+        np.savez(tmp, **series, **{k: np.stack(v) for k, v in rec.items()},
                  kept_sweeps=np.array(kept, dtype=np.int32), excluded_sweeps=np.array(excluded, dtype=np.int32))
         os.replace(tmp, path)
 
+    def np64(t):
+        return t.detach().cpu().numpy().astype(np.float64)
+
+    t0 = time.time()
     s = None
     for n in range(nsweeps):
         smp = sampler.sweep()
         s = smp.spins
-        e = H.Oloc(state, Samples(s, smp.psi)).detach().cpu().numpy().astype(np.float64)
-        m, q = float(e.mean()), float((e * e).mean())
-        if not (np.isfinite(m) and np.isfinite(q)):
+        psi = smp.psi
+        if psi is None and {"spin", "dwave"} & set(channels):
+            psi = state(s)
+        cur = {}
+        if "energy" in channels:
+            e = np64(H.Oloc(state, Samples(s, psi)))
+            cur["E_mean"], cur["E_sqr"] = e.mean(), (e * e).mean()
+        if {"spin", "charge"} & set(channels):
+            gz, gn = obs.sz_grid(s, N), obs.dn_grid(s, N, nbar)
+            cur["sz_site"] = np64(gz.mean(0)).reshape(LX, LY)
+            cur["n_site"] = np64(gn.mean(0)).reshape(LX, LY) + nbar
+            if "spin" in channels:
+                cur["Czz"] = np64(obs.autocorr(gz, LX, LY))
+            if "charge" in channels:
+                cur["Cnn"] = np64(obs.autocorr(gn, LX, LY))
+        if "spin" in channels:
+            B = s.shape[0]
+            T = np.zeros((NREF, 2, N))
+            for i, (ax, ay) in enumerate(refs):
+                sel = np.arange(i, B, len(refs))
+                if len(sel) == 0:
+                    raise ValueError(f"{B} samples leave reference site {i} without samples")
+                sel_t = torch.as_tensor(sel, device=s.device)
+                Ti = obs.transverse_local(state, s[sel_t], LogArray(psi.sign[sel_t], psi.logabs[sel_t]),
+                                          ax * LY + ay, LX, LY)
+                T[i] = np64(Ti.mean(0))
+            cur["Tpm"], cur["Tmp"] = T[:, 0], T[:, 1]
+        if "dwave" in channels:
+            F, G = obs.dwave_measure(state, s, psi, LX, LY)
+            cur["F"], cur["G"] = np64(F).reshape(-1), np64(G).reshape(-1)  # flat, in F_keys / G_keys order
+        if not all(np.isfinite(v).all() for v in cur.values()):
             excluded.append(n)
             log(f"sweep {n}: non-finite values, excluded")
             continue
         kept.append(n)
-        rec["E_mean"].append(m)
-        rec["E_sqr"].append(q)
-        log(f"sweep {n}: E/N = {m / w.N:.6f}, sigma^2/N = {(q - m * m) / w.N:.4f}")
+        for k, v in cur.items():
+            rec.setdefault(k, []).append(v)
+        msg = f"sweep {n}: {time.time() - t0:.0f} s"
+        if "energy" in channels:
+            msg += f", E/N = {cur['E_mean'] / N:.6f}, sigma^2/N = {(cur['E_sqr'] - cur['E_mean'] ** 2) / N:.4f}"
+        log(msg)
         if (n + 1) % SAVE_EVERY == 0:
             save()
     save()
     if s is not None:
-        spins = s.detach().cpu().numpy()
-        np.savez(os.path.join(out_dir, "spins_last.npz"), **pack_spins(spins))
-    res = assemble_energy({**series, **{k: np.asarray(v) for k, v in rec.items()}})
+        np.savez(os.path.join(out_dir, "spins_last.npz"), **pack_spins(s.detach().cpu().numpy()))
+    res = assemble({**series, **{k: np.stack(v) for k, v in rec.items()}}) if kept else {}
     if kept:
         log(summary(res))
         save_results(res, out_dir)
     return res
+
+
+def run_energy(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", tprime: float = 0.0,
+               log=print) -> dict:
+    """`run_measure` with the energy channel alone."""
+    return run_measure(state, H, sampler, nsweeps, out_dir, name, tprime, ("energy",), log=log)

@@ -171,6 +171,15 @@ def stripe_k(LX: int, LY: int, family: str):
     return ((hx - nx, hy), (hx + nx, hy), (hx, hy - ny), (hx, hy + ny))
 
 
+def charge_q(LX: int, LY: int, family: str):
+    """Integer momenta of the charge wavevectors +-(2q, 0), +-(0, 2q) (``observables.charge_q``)."""
+    num, den = FAMILIES[family]
+    if (num * LX) % (2 * den) or (num * LY) % (2 * den):
+        raise ValueError(f"q = {num}/{den} pi is not on the momentum grid of {LX}x{LY}")
+    nx, ny = 2 * (num * LX // (2 * den)), 2 * (num * LY // (2 * den))
+    return ((nx, 0), (LX - nx, 0), (0, ny), (0, LY - ny))
+
+
 def structure_factor(C, LX: int, LY: int):
     """S(k) = sum_r e^{ik.r} C(r) on the momentum grid, (..., LX, LY) (``assemble.structure_factor``)."""
     C = np.asarray(C, dtype=np.float64)

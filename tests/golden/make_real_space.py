@@ -4,7 +4,8 @@ tp0_L16_bcs states (``data/supplement/real_space_corr/figS17_real_space_corr_tpm
 maps.npz`` and ``figS12_real_space_corr_tp0_16x16_bcs/maps.npz``: ``Cs``, ``Cc``, ``Cp`` and their
 ``*_err``, indexed by r = (rx, ry) and symmetrized as [C(r) + C(-r)] / 2, 128 sweeps of 8192
 samples; ``Delta_SC2_plateau``, the mean of ``Cp`` over |r| >= ``r_max``, over ``Delta_SC2_nvec``
-displacements) and their stripe order M^2_str and Delta^2_SC of Fig. 4 / Fig. 3 panel c
+displacements; for tpm02 also the per-sweep ``Cs`` and ``Cc`` of ``series.npz`` as ``Cs_series``,
+``Cc_series``) and their stripe order M^2_str and Delta^2_SC of Fig. 4 / Fig. 3 panel c
 (``data/fig4_order_params_tpm02/panel_c_orders.csv``, ``data/fig3_order_params_tp0/
 panel_c_orders.csv``).  Run in the build container:
 ``python tests/golden/make_real_space.py [reference_root]``.  Data only, copied as stored."""
@@ -33,6 +34,9 @@ def main(ref_root="/root/reference"):
         src = os.path.join(ref_root, "data", "supplement", "real_space_corr", fig, "maps.npz")
         with np.load(src, allow_pickle=False) as f:
             out = {k: f[k] for k in MAP_KEYS}
+        if out_name.startswith("real_space_corr_tpm02"):  # the per-sweep series behind the maps
+            with np.load(os.path.join(os.path.dirname(src), "series.npz"), allow_pickle=False) as f:
+                out["Cs_series"], out["Cc_series"] = f["Cs"], f["Cc"]
         with open(os.path.join(ref_root, "data", orders), newline="") as fh:
             for row in csv.DictReader(fh):
                 if row["state"] == label:

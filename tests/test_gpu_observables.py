@@ -145,3 +145,37 @@ def test_released_pair_correlation_matches_published_map(name, fixture):
     d2, e2 = res["Delta2_SC"], res["Delta2_SC_err"]
     print(f"Delta2_SC {d2:.6f} +- {e2:.6f}, published {d2_pub:.6f} +- {d2_err:.6f}", flush=True)
     assert abs(d2 - d2_pub) < 4 * e2 + 3 * d2_err
+
+
+def test_run_measure_all_channels(tmp_path):
+    """`measure.run_measure` end to end on the GPU (8x8, 28 + 28 electrons, 64 chains, 3 sweeps of
+    the Metropolis sampler): the series and results files of all four channels, every value
+    finite, <n_i> = 7/8 on average and E/N in the range of a random-weight state."""
+    import json
+
+    from neuralhubbard_amd import lattice as Lat
+    from neuralhubbard_amd import measure as M
+    from neuralhubbard_amd.backflow import Hubbard, Variational, build_backflow
+    from neuralhubbard_amd.configs import make_params, small_workload
+    from neuralhubbard_amd.sampler import MetropolisSampler
+
+    wk = small_workload(LX=8, LY=8, n_up=28, n_dn=28, tprime=-0.2, channels=8, n_layers=2, seed=5)
+    p = make_params(wk)
+    state = Variational(build_backflow(wk, p))
+    sampler = MetropolisSampler(state, Lat.random_configurations(wk.geom, 64, 3), sweep_steps=64, seed=2,
+                                thermal_steps=128)
+    out = str(tmp_path / "run")
+    print("run_measure: 8x8, 64 chains, 3 sweeps, all channels", flush=True)
+    res = M.run_measure(state, Hubbard(U=wk.U, tprime=wk.tprime), sampler, 3, out, name="small8",
+                        tprime=wk.tprime, log=lambda m: print(m, flush=True))
+    with np.load(os.path.join(out, "series.npz"), allow_pickle=False) as f:
+        series = {k: f[k] for k in f.files}
+    for k in ("E_mean", "E_sqr", "Czz", "Cnn", "Tpm", "Tmp", "F", "G", "sz_site", "n_site"):
+        assert series[k].shape[0] == 3 and np.isfinite(series[k]).all(), k
+    assert abs(series["n_site"].mean() - 0.875) < 1e-12
+    assert -3.0 < res["energy"]["E_per_site"] < 3.0
+    with open(os.path.join(out, "results.json")) as fh:
+        js = json.load(fh)
+    assert set(js) >= {"energy", "spin", "charge", "dwave", "local"}
+    with np.load(os.path.join(out, "results.npz"), allow_pickle=False) as f:
+        assert f["C_p"].shape == (64,) and np.isfinite(f["C_p"]).all()

@@ -149,3 +149,26 @@ def test_dwave_assembly_statistics():
     assert abs(res["Delta2_SC"] - per.mean(0)[far].mean()) < 1e-12
     same = obs.dwave(np.repeat(F[:1], 16, 0), np.repeat(G[:1], 16, 0), LX, LY)
     assert same["Delta2_SC_err"] < 1e-12
+
+
+def test_assembly_of_published_series_reproduces_published_maps():
+    """`measure.jackknife` and `measure.spin` on the reference's own per-sweep C_s (128 sweeps) and
+    C_c (96 sweeps) of tpm02_L16_pfaffian (figS17 series.npz): the symmetrized sweep means are the
+    published maps, the 4-sweep block errors the published error maps (data/supplement/
+    real_space_corr/README.md: "standard errors over blocks of 4 sweeps"), and M^2_str with its
+    jackknife error the published 0.011789(9) of Fig. 4 panel c."""
+    from neuralhubbard_amd import measure as M
+
+    with np.load(os.path.join(GOLDEN, "real_space_corr_tpm02_L16.npz"), allow_pickle=False) as f:
+        pub = {k: f[k] for k in f.files}
+    L = 16
+    for key in ("Cs", "Cc"):
+        X = pub[f"{key}_series"].astype(np.float64)
+        X = 0.5 * (X + np.roll(X[:, ::-1, ::-1], (1, 1), axis=(1, 2)))
+        v, _, errs = M.jackknife(lambda x: x, X.reshape(len(X), -1))
+        np.testing.assert_allclose(v.reshape(L, L), pub[key], atol=1e-9)
+        np.testing.assert_allclose(errs[4].reshape(L, L), pub[f"{key}_err"], rtol=1e-6, atol=1e-12)
+    Cs = pub["Cs_series"].astype(np.float64).reshape(128, -1)
+    sp = M.spin(Cs, np.zeros_like(Cs), np.zeros_like(Cs), L, L, "pi4")
+    assert abs(sp["M2_str"] - float(pub["M2_str"])) < 1e-6
+    assert abs(sp["M2_str_err"] - float(pub["M2_str_err"])) < 1e-6
