@@ -64,7 +64,10 @@ enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1, NH_NET_RESCONV_HEAD = 2 };
  * pfaffian: wt = [4][N][d] spin-block table [uu, ud, du, dd] with the distance envelope
  *           (code/backflow/pf_kernel.py:107-124); base = F0 [2N][2N] antisymmetric or NULL.
  * bcs:      wt = [N][d] (code/backflow/bcs_kernel.py:63-70); base = phi0 [N] or NULL,
- *           added as phi0[r(up_p - dn_q)]. */
+ *           added as phi0[r(up_p - dn_q)].
+ * Sizes: N = LX * LY <= 1024; pfaffian n_up + n_dn even and <= 1024; bcs n_up == n_dn <= 512
+ * (the 32x32 lattice at 7/8 filling has n_up = 448); matrices above n = 64 (pfaffian) or
+ * n = 192 (bcs) live in the workspace. */
 typedef struct nh_model {
   int32_t kind;       /* NH_KIND_* */
   int32_t LX, LY;     /* periodic lattice, both >= 3 */

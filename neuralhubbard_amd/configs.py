@@ -105,6 +105,8 @@ WORKLOADS = {
                       notes="16x16 BCS backflow with base pair function, 8192 configurations"),
     "pf32": Workload("pf32", "pfaffian", 32, 32, 448, 448, -0.2, 8.0, 32, 6, 1024,
                      notes="32x32 Pfaffian backflow, 1024 configurations"),
+    "bcs32": Workload("bcs32", "bcs", 32, 32, 448, 448, 0.0, 8.0, 32, 6, 1024,
+                      notes="32x32 BCS backflow with base pair function (448 x 448 determinants)"),
     # the released tpm02_L16_pfaffian state (code/states.py:114-115): 16 residual blocks of 64
     # channels, d = 32, U = 8, t' = -0.2, 112 + 112 electrons
     "tpm02_L16": Workload("tpm02_L16", "pfaffian", 16, 16, 112, 112, -0.2, 8.0, 64, 16, 8192,

@@ -76,7 +76,7 @@ int check_model(const nh_model* m) {
   if (m->n_up > N || m->n_dn > N) return NH_ERR_SHAPE;
   if (!m->conv_w || !m->conv_b || !m->wt) return NH_ERR_ARG;
   if (m->kind == NH_KIND_PFAFFIAN && ((m->n_up + m->n_dn) % 2 || m->n_up + m->n_dn > 1024)) return NH_ERR_SHAPE;
-  if (m->kind == NH_KIND_BCS && (m->n_up != m->n_dn || m->n_up > 192)) return NH_ERR_SHAPE;
+  if (m->kind == NH_KIND_BCS && (m->n_up != m->n_dn || m->n_up > 512)) return NH_ERR_SHAPE;
   if (m->precision != NH_PREC_FP32 && m->precision != NH_PREC_MIXED) return NH_ERR_ARG;
   if (m->network == NH_NET_CONV && m->precision == NH_PREC_MIXED && (!m->conv_w_lp || !cnn_mfma_supported(m)))
     return NH_ERR_UNSUPPORTED;
@@ -103,7 +103,7 @@ struct ItemBuffers {
   float* Y;
   float* img_sign;
   double* img_logabs;
-  float* gtri;  // Pfaffian triangles above n = 256, one per image of a chunk
+  float* gtri;  // Pfaffian triangles above n = 64 / BCS matrices above n = 192, one per image of a chunk
   int64_t chunk;
 };
 
@@ -111,7 +111,7 @@ static ItemBuffers carve_items(Carver& c, const nh_model* m, int64_t n_items) {
   const int N = m->LX * m->LY;
   const int D = feature_dim(m);
   const int64_t images = n_items * ipi_of(m);
-  const size_t tri = pf_global_tri_floats(m);
+  const size_t tri = image_global_floats(m);
   ItemBuffers b;
   b.chunk = std::min<int64_t>(images, image_chunk(m));
   b.Y = c.take<float>((size_t)b.chunk * 2 * N * D);

@@ -6,7 +6,8 @@
 //   n <= 64:  pf_eliminate / lu_eliminate, one wave, rank-2 (rank-1) updates per step.
 //   n > 64:   pf_blocked2 (blocked Parlett-Reid: left-looking panels, packed pivot keys, two
 //             barriers per step, FP32 MFMA trailing updates) and lu_blocked (blocked LU with
-//             rows in place, one barrier per step, MFMA trailing updates).
+//             rows in place, one barrier per step, MFMA trailing updates).  The BCS image kernel
+//             keeps its matrix in LDS up to n = 192 and in a global workspace above (n <= 512).
 // Parlett-Reid for the Pfaffian follows Wimmer, ACM TOMS 38 (2012) 30 (pfaffian_LTL): with
 // pivot p = A[k+1][k], Pf *= -p, tau_i = A[i][k] / p, v_i = A[i][k+1] and
 // A[i][j] += tau_i v_j - v_i tau_j for i > j >= k + 2.
@@ -544,21 +545,22 @@ constexpr int kLuPanel = 16;  // columns per panel of the blocked LU
 // |pivots| n, parity partials 16.
 __host__ __device__ inline int lu_scratch_words(int n) { return 32 + 3 * ((n + 3) & ~3) + 16; }
 
-// Blocked LU with partial pivoting for sign / log|det| (64 < n <= 256), dense row-major M
-// (leading dimension ld, followed by kWave dummy floats), one thread per row.  Rows are never
+// Blocked LU with partial pivoting for sign / log|det| (64 < n <= min(NT, 2^KB)), dense row-major
+// M (leading dimension ld, followed by kWave dummy floats; LDS or global), one thread per row.  Rows are never
 // moved: row p becomes "done" when it is chosen as pivot, and the permutation's parity is
 // counted at the end from the list of pivot rows.  Panels of P columns:
 //   step s (column j = k0 + s), one barrier: every live row holds its panel entries in
-//   registers; the pivot row p* (largest |M[p][j]| to 2^-15 relative, smallest index among
-//   those; 32-bit keys as in the Pfaffian) is read from LDS, live rows form
+//   registers; the pivot row p* (largest |M[p][j]| to 2^(KB-23) relative, smallest index among
+//   those; 32-bit keys with the row in the low KB bits, as in the Pfaffian) is read from LDS, live rows form
 //   l = M[p][j] / M[p*][j], update their later panel entries M[p][j'] -= l M[p*][j'], store
 //   them (so that any row can be the next pivot) and form the next column's key;
 //   panel end: U12 rows of the panel's pivot rows by forward substitution (one thread per
 //   column), then the trailing update M[i][c] -= sum_q L[i][q] U12[q][c] as 16 x 16 FP32 MFMA
 //   tiles over all rows (rows done earlier hold dead values; the panel's own pivot rows,
 //   read as U12, are not written).
-template <int NT, int P>
+template <int NT, int P, int KB = 8>
 __device__ void lu_blocked(float* M, int n, int ld, uint32_t* scr, float& sign_out, double& log_out) {
+  constexpr uint32_t kIdx = (1u << KB) - 1u;  // low key bits: kIdx - row (n <= 2^KB)
   constexpr int NW = NT / kWave;
   static_assert(NW <= 16 && NW % 4 == 0, "pivot keys: 4, 8 or 16 waves");
   static_assert(P % 4 == 0 && P <= 16, "panel of 4, 8, 12 or 16 columns");
@@ -583,7 +585,7 @@ __device__ void lu_blocked(float* M, int n, int ld, uint32_t* scr, float& sign_o
 #pragma unroll
     for (int q = 0; q < P; ++q) rv[q] = M[(live && k0 + q < n) ? t * ld + k0 + q : dummy];
     {
-      uint32_t key = live ? ((__float_as_uint(fabsf(rv[0])) & 0xFFFFFF00u) | (uint32_t)(255 - t)) : 0u;
+      uint32_t key = live ? ((__float_as_uint(fabsf(rv[0])) & ~kIdx) | (kIdx - (uint32_t)t)) : 0u;
       if (wlo < n) key = wave_max_u32(key);
       if (lane == 0) keys0[wave] = wlo < n ? key : 0u;
     }
@@ -601,7 +603,7 @@ __device__ void lu_blocked(float* M, int n, int ld, uint32_t* scr, float& sign_o
         km = max(km, max(max(kv.x, kv.y), max(kv.z, kv.w)));
       }
       km = __builtin_amdgcn_readfirstlane(km);
-      const int ps = 255 - (int)(km & 0xFFu);
+      const int ps = (int)(kIdx - (km & kIdx));
       const float* prow = M + ps * ld + k0;
       const float piv = prow[s];
       float urow[P];
@@ -634,7 +636,7 @@ __device__ void lu_blocked(float* M, int n, int ld, uint32_t* scr, float& sign_o
 #pragma unroll
         for (int q = s + 1; q < P; ++q) M[(act && k0 + q < n) ? t * ld + k0 + q : dummy] = rv[q];
         if (s + 1 < steps)
-          key = act ? ((__float_as_uint(fabsf(rv[s + 1])) & 0xFFFFFF00u) | (uint32_t)(255 - t)) : 0u;
+          key = act ? ((__float_as_uint(fabsf(rv[s + 1])) & ~kIdx) | (kIdx - (uint32_t)t)) : 0u;
         if (s + 1 < steps) key = wave_max_u32(key);
       }
       if (s + 1 < steps && lane == 0) kbn[wave] = wlo < n ? key : 0u;
@@ -754,6 +756,10 @@ __host__ __device__ inline int pf_elim_floats_lds(int n) {
 // and hide each other's pivot-step latency (measured 0.44 us per 224 x 224 image against
 // 0.58 us with the triangle in LDS and one 1024-thread workgroup per CU).
 constexpr int kLdsTriMaxN = 64;
+// BCS image matrices (n x (n + 1) FP32) stay in LDS up to n = 192 (145 KB); above, up to
+// kGlobalDetMaxN (one LU thread per row of a 512-thread workgroup), in the global workspace.
+constexpr int kLdsDetMaxN = 192;
+constexpr int kGlobalDetMaxN = 512;
 
 // floats of the region used first by the pairing build (features of the occupied modes, one
 // Wt spin block) and then by the blocked elimination (panel vectors, column buffer, pivots)
@@ -908,7 +914,10 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 
 // ------------------------------------------------------------------ BCS image kernel
 
-template <int D, int NT>
+// GM (n > kLdsDetMaxN): the image's n x (n + 1) matrix sits in the global workspace gmat at
+// (img - img0) * gstride floats (followed by kWave dummy floats); LDS holds the features of the
+// occupied modes (then the elimination scratch), the mode list and a flag.
+template <int D, int NT, bool GM>
 __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_up, int n_dn,
                                                   const int8_t* __restrict__ s,
                                                   const int32_t* __restrict__ item_b,
@@ -917,7 +926,8 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
                                                   const float* __restrict__ wt,
                                                   const float* __restrict__ phi0,
                                                   float* __restrict__ img_sign,
-                                                  double* __restrict__ img_logabs) {
+                                                  double* __restrict__ img_logabs,
+                                                  float* __restrict__ gmat, size_t gstride) {
   extern __shared__ float lds[];
   const int img = img0 + blockIdx.x;
   if (img >= img_end) return;
@@ -931,8 +941,15 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   const int ne = n_up + n_dn;
   const int ld = n + 1;
 
-  float* M = lds;
-  float* yu = M + ((n * ld + kWave + 3) & ~3);
+  float* M;
+  float* yu;
+  if constexpr (GM) {
+    M = gmat + (size_t)(img - img0) * gstride;
+    yu = lds;
+  } else {
+    M = lds;
+    yu = M + ((n * ld + kWave + 3) & ~3);
+  }
   float* yd = yu + n * D;
   float* lcol = yd + n * D;
   float* urow = lcol + n;
@@ -992,8 +1009,10 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   __syncthreads();
   float sg;
   double la;
-  if constexpr (NT > kWave)
-    lu_blocked<NT, kLuPanel>(M, n, ld, reinterpret_cast<uint32_t*>(yu), sg, la);  // features are dead
+  if constexpr (GM)
+    lu_blocked<NT, kLuPanel, 9>(M, n, ld, reinterpret_cast<uint32_t*>(yu), sg, la);  // features are dead
+  else if constexpr (NT > kWave)
+    lu_blocked<NT, kLuPanel>(M, n, ld, reinterpret_cast<uint32_t*>(yu), sg, la);
   else
     lu_eliminate<NT>(M, n, ld, lcol, urow, flag, sg, la);
   if (t == 0) {
@@ -1014,19 +1033,25 @@ static size_t pf_lds_bytes(int n, int N, int D) {
 }
 
 int64_t image_chunk(const nh_model* m) {
+  if (m->kind == NH_KIND_BCS) return m->n_up <= kLdsDetMaxN ? 32768 : 1024;  // 0.8 MB per image at n = 448
   const int n = m->n_up + m->n_dn;
-  if (m->kind != NH_KIND_PFAFFIAN || n <= kLdsTriMaxN) return 32768;
+  if (n <= kLdsTriMaxN) return 32768;
   return n <= 256 ? 8192 : 1024;  // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB)
 }
 
-size_t pf_global_tri_floats(const nh_model* m) {
+size_t image_global_floats(const nh_model* m) {
+  if (m->kind == NH_KIND_BCS) {
+    const size_t n = (size_t)m->n_up;
+    return n <= (size_t)kLdsDetMaxN ? 0 : ((n * (n + 1) + kWave + 63) & ~(size_t)63);
+  }
   const int n = m->n_up + m->n_dn;
-  if (m->kind != NH_KIND_PFAFFIAN || n <= kLdsTriMaxN) return 0;
+  if (n <= kLdsTriMaxN) return 0;
   return (size_t)((((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63);
 }
 
 static size_t bcs_lds_bytes(int n, int ne, int D) {
-  return sizeof(float) * (size_t)(((n * (n + 1) + kWave + 3) & ~3) + 2 * n * D + 2 * n) + sizeof(int) * (size_t)(ne + 4);
+  const size_t mat = n > kLdsDetMaxN ? 0 : (size_t)((n * (n + 1) + kWave + 3) & ~3);
+  return sizeof(float) * (mat + (size_t)(2 * n * D + 2 * n)) + sizeof(int) * (size_t)(ne + 4);
 }
 
 constexpr size_t kMaxLds = 160 * 1024;
@@ -1051,24 +1076,32 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
     } else if (n <= 256) {
       hipLaunchKernelGGL((k_image_pf<D, 256, true>), dim3(nimg), dim3(256), lds, st, N, m->LX, m->LY, n,
                          m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
-                         img_logabs, gtri, pf_global_tri_floats(m));
+                         img_logabs, gtri, image_global_floats(m));
     } else {
       hipLaunchKernelGGL((k_image_pf<D, 1024, true>), dim3(nimg), dim3(1024), lds, st, N, m->LX, m->LY, n,
                          m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
-                         img_logabs, gtri, pf_global_tri_floats(m));
+                         img_logabs, gtri, image_global_floats(m));
     }
   } else {
     const int n = m->n_up;
     const size_t lds = bcs_lds_bytes(n, m->n_up + m->n_dn, D);
     if (lds > kMaxLds) return NH_ERR_SHAPE;
     if (n <= 64) {
-      hipLaunchKernelGGL((k_image_bcs<D, 64>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, m->n_up,
+      hipLaunchKernelGGL((k_image_bcs<D, 64, false>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, m->n_up,
                          m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
-                         img_logabs);
+                         img_logabs, nullptr, (size_t)0);
+    } else if (n <= kLdsDetMaxN) {
+      hipLaunchKernelGGL((k_image_bcs<D, 512, false>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, m->n_up,
+                         m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
+                         img_logabs, nullptr, (size_t)0);
+    } else if (n > kGlobalDetMaxN) {
+      return NH_ERR_SHAPE;
+    } else if (!gtri) {
+      return NH_ERR_ARG;
     } else {
-      hipLaunchKernelGGL((k_image_bcs<D, 512>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, m->n_up,
+      hipLaunchKernelGGL((k_image_bcs<D, 512, true>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, m->n_up,
                          m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
-                         img_logabs);
+                         img_logabs, gtri, image_global_floats(m));
     }
   }
   NH_LAUNCH_CHECK();
