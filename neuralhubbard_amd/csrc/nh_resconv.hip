@@ -244,8 +244,9 @@ __global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const flo
 // three taps dxi = 0..2 for all four quarters, hi then lo (24 KB).  The workgroup loads slab
 // k + 1 from global memory into registers while its waves run the MFMAs of slab k from LDS,
 // then writes it to the other LDS buffer; one barrier per slab.  Every weight byte is read from
-// global memory once per image (not once per wave) and its latency hides behind 144 MFMAs per
-// wave.  Host packing (neuralhubbard_amd/mixed.py, pack_resconv_bf16): slab-major
+// global memory once per image (not once per wave) and its latency hides behind the wave's 72
+// MFMAs of the slab (two waves per SIMD: each wave owns four lattice rows and two of the four
+// channel quarters).  Host packing (neuralhubbard_amd/mixed.py, pack_resconv_bf16): slab-major
 // [conv][kh][dyi][plane][dxi][qq][lane][8 BF16] with W[tap = 3 dxi + dyi][32 kh + 8 (l / 16) + e]
 // [16 qq + l % 16].
 
@@ -272,13 +273,16 @@ __device__ inline float rb_gelu(float x) {
 template <int CTRL>
 __device__ inline int rb_rot(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false); }
 
-// act planes = gelu(scale * v), BF16 hi and lo, for the wave's rows x0 .. x0+3
-__device__ inline void rb_store(uint16_t* act, const rc_f32x4 (&v)[4][4], float scale, int x0, int lane, int N) {
+// act planes = gelu(scale * v), BF16 hi and lo, for the wave's rows x0 .. x0+3 and channel
+// quarters q0 .. q0+QN-1
+template <int QN>
+__device__ inline void rb_store(uint16_t* act, const rc_f32x4 (&v)[4][QN], float scale, int x0, int q0, int lane,
+                                int N) {
   const int j = lane & 15, g16 = lane >> 4;
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
+    for (int qq = 0; qq < QN; ++qq) {
       uint32_t hi[4], lo[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -286,7 +290,7 @@ __device__ inline void rb_store(uint16_t* act, const rc_f32x4 (&v)[4][4], float 
         hi[e] = rb_bf16(g);
         lo[e] = rb_bf16(g - __uint_as_float(hi[e] << 16));
       }
-      uint16_t* dst = act + ((x0 + r) * kRcLY + j) * kRbStride + 16 * qq + 4 * g16;
+      uint16_t* dst = act + ((x0 + r) * kRcLY + j) * kRbStride + 16 * (q0 + qq) + 4 * g16;
       *reinterpret_cast<uint2*>(dst) = make_uint2(hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16));
       *reinterpret_cast<uint2*>(dst + N * kRbStride) = make_uint2(lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16));
     }
@@ -299,10 +303,11 @@ constexpr int rb_per_thread() {
   return 2 * kSlabFrag / NT;
 }
 
-// MFMAs of one slab (tap row DYI, K-half kh) on rows x0 .. x0+3; wb = the slab in LDS
-template <int DYI>
-__device__ __forceinline__ void rb_taps(const uint16_t* act, const rb_i32x4* wb, int LX, int N, int x0, int lane, int kh,
-                               rc_f32x4 (&acc)[4][4]) {
+// MFMAs of one slab (tap row DYI, K-half kh) on rows x0 .. x0+3 and quarters q0 .. q0+QN-1;
+// wb = the slab in LDS
+template <int DYI, int QN>
+__device__ __forceinline__ void rb_taps(const uint16_t* act, const rb_i32x4* wb, int LX, int N, int x0, int q0,
+                                        int lane, int kh, rc_f32x4 (&acc)[4][QN]) {
   const int j = lane & 15, g16 = lane >> 4;
   rb_bf16x8 bh[6], bl[6];
 #pragma unroll
@@ -324,8 +329,8 @@ __device__ __forceinline__ void rb_taps(const uint16_t* act, const rb_i32x4* wb,
 #pragma unroll
   for (int dxi = 0; dxi < 3; ++dxi) {
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
-      const int f = (dxi * 4 + qq) * kWave + lane;
+    for (int qq = 0; qq < QN; ++qq) {
+      const int f = (dxi * 4 + q0 + qq) * kWave + lane;
       const rb_bf16x8 wh = __builtin_bit_cast(rb_bf16x8, wb[f]);
       const rb_bf16x8 wl = __builtin_bit_cast(rb_bf16x8, wb[kSlabFrag + f]);
 #pragma unroll
@@ -341,9 +346,10 @@ __device__ __forceinline__ void rb_taps(const uint16_t* act, const rb_i32x4* wb,
 // One slab step: start loading slab k + 1 into registers, run slab k from LDS buffer k & 1,
 // park slab k + 1 in the other buffer, barrier.  The barrier also ends every wave's reads of
 // the conv input.
-template <int NT, int DYI>
+template <int NT, int DYI, int QN>
 __device__ __forceinline__ void rb_step(const uint16_t* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
-                                        int n_slabs, int LX, int N, int x0, int t, int kh, rc_f32x4 (&acc)[4][4]) {
+                                        int n_slabs, int LX, int N, int x0, int q0, int t, int kh,
+                                        rc_f32x4 (&acc)[4][QN]) {
   constexpr int kPer = rb_per_thread<NT>();
   const bool more = k + 1 < n_slabs;
   rb_i32x4 v[kPer];
@@ -352,7 +358,7 @@ __device__ __forceinline__ void rb_step(const uint16_t* act, rb_i32x4* wbuf, con
 #pragma unroll
     for (int i = 0; i < kPer; ++i) v[i] = src[t + i * NT];
   }
-  rb_taps<DYI>(act, wbuf + (k & 1) * 2 * kSlabFrag, LX, N, x0, t & (kWave - 1), kh, acc);
+  rb_taps<DYI, QN>(act, wbuf + (k & 1) * 2 * kSlabFrag, LX, N, x0, q0, t & (kWave - 1), kh, acc);
   if (more) {
     rb_i32x4* dst = wbuf + ((k + 1) & 1) * 2 * kSlabFrag;
 #pragma unroll
@@ -362,22 +368,22 @@ __device__ __forceinline__ void rb_step(const uint16_t* act, rb_i32x4* wbuf, con
   ++k;
 }
 
-template <int NT>
+template <int NT, int QN>
 __device__ __forceinline__ void rb_conv(const uint16_t* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
-                                        int n_slabs, const float* __restrict__ bl, int LX, int N, int x0, int t,
-                                        rc_f32x4 (&acc)[4][4]) {
+                                        int n_slabs, const float* __restrict__ bl, int LX, int N, int x0, int q0,
+                                        int t, rc_f32x4 (&acc)[4][QN]) {
   const int g16 = (t & (kWave - 1)) >> 4;
 #pragma unroll
-  for (int qq = 0; qq < 4; ++qq) {
-    const float4 b4 = *reinterpret_cast<const float4*>(bl + 16 * qq + 4 * g16);
+  for (int qq = 0; qq < QN; ++qq) {
+    const float4 b4 = *reinterpret_cast<const float4*>(bl + 16 * (q0 + qq) + 4 * g16);
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r][qq] = rc_f32x4{b4.x, b4.y, b4.z, b4.w};
   }
 #pragma unroll 1
   for (int kh = 0; kh < 2; ++kh) {
-    rb_step<NT, 0>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, t, kh, acc);
-    rb_step<NT, 1>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, t, kh, acc);
-    rb_step<NT, 2>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, t, kh, acc);
+    rb_step<NT, 0, QN>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, q0, t, kh, acc);
+    rb_step<NT, 1, QN>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, q0, t, kh, acc);
+    rb_step<NT, 2, QN>(act, wbuf, Wsl, k, n_slabs, LX, N, x0, q0, t, kh, acc);
   }
 }
 
@@ -385,8 +391,10 @@ __host__ __device__ constexpr size_t rb_lds_floats(int N) {
   return (size_t)N * kRbStride + 16 * kSlabFrag + 16;  // act planes, two slabs (2 x 2 planes of rb_i32x4), partials
 }
 
-// emb [4][64], Wsl = split-BF16 slabs of the 2 n_blocks convs, Bc [2 n_blocks][64]
-template <int NT>
+// emb [4][64], Wsl = split-BF16 slabs of the 2 n_blocks convs, Bc [2 n_blocks][64].
+// QN channel quarters per wave: QN = 4, one wave per 4 lattice rows (NT = 16 LX); QN = 2, two
+// waves per 4 rows, each with two quarters (NT = 32 LX), so that two waves share each SIMD.
+template <int NT, int QN>
 __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const float* __restrict__ emb,
                                                      const rb_i32x4* __restrict__ Wsl, const float* __restrict__ Bc,
                                                      const float* __restrict__ head, float out_scale,
@@ -394,6 +402,7 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
                                                      const int32_t* __restrict__ item_b,
                                                      const int32_t* __restrict__ item_hop, int img0, int img_end,
                                                      int ipi, float* __restrict__ Y) {
+  static_assert(QN == 4 || QN == 2, "four or two channel quarters per wave");
   extern __shared__ float lds[];
   const int N = LX * kRcLY;
   uint16_t* act = reinterpret_cast<uint16_t*>(lds);                    // hi plane [N][72], lo plane [N][72]
@@ -408,7 +417,8 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
   const int b = item_b ? item_b[item] : item;
   const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
   constexpr int nw = NT / kWave;
-  const int x0 = 4 * wave, j = lane & 15, g16 = lane >> 4;
+  constexpr int nrg = nw * QN / 4;  // row groups of 4 lattice rows (= LX / 4)
+  const int x0 = 4 * (wave % nrg), q0 = QN * (wave / nrg), j = lane & 15, g16 = lane >> 4;
   const int n_slabs = 2 * n_blocks * kSlabsPerConv;
 
   int k = 0;
@@ -416,7 +426,7 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
   for (int i = 0; i < rb_per_thread<NT>(); ++i) wbuf[t + i * NT] = Wsl[t + i * NT];  // slab 0, visible after
                                                                                      // the first barrier
 
-  rc_f32x4 h[4][4];
+  rc_f32x4 h[4][QN];
   {
     const int8_t* sb = s + (size_t)b * 2 * N;
 #pragma unroll
@@ -426,8 +436,8 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
       const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
       const float* e = emb + (up + 2 * dn) * kRcC + 4 * g16;
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const float4 v = *reinterpret_cast<const float4*>(e + 16 * qq);
+      for (int qq = 0; qq < QN; ++qq) {
+        const float4 v = *reinterpret_cast<const float4*>(e + 16 * (q0 + qq));
         h[r][qq] = rc_f32x4{v.x, v.y, v.z, v.w};
       }
     }
@@ -438,7 +448,7 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq)
+      for (int qq = 0; qq < QN; ++qq)
 #pragma unroll
         for (int e = 0; e < 4; ++e) ss = fmaf(h[r][qq][e], h[r][qq][e], ss);
     ss = rc_wave_sum(ss);
@@ -448,26 +458,26 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
 #pragma unroll
     for (int w = 0; w < nw; ++w) tot += part[w];
     const float rinv = 1.0f / sqrtf(fmaf(tot, inv_count, kRmsEps));
-    rb_store(act, h, rinv, x0, lane, N);
+    rb_store<QN>(act, h, rinv, x0, q0, lane, N);
     __syncthreads();
-    rc_f32x4 acc[4][4];
-    rb_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk) * kRcC, LX, N, x0, t, acc);
-    rb_store(act, acc, 1.0f, x0, lane, N);  // the last slab barrier ended every read of act
+    rc_f32x4 acc[4][QN];
+    rb_conv<NT, QN>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk) * kRcC, LX, N, x0, q0, t, acc);
+    rb_store<QN>(act, acc, 1.0f, x0, q0, lane, N);  // the last slab barrier ended every read of act
     __syncthreads();
-    rb_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk + 1) * kRcC, LX, N, x0, t, acc);
+    rb_conv<NT, QN>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk + 1) * kRcC, LX, N, x0, q0, t, acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
+      for (int qq = 0; qq < QN; ++qq) h[r][qq] += acc[r][qq];
   }
   if (head) {  // BCS: the last slab barrier ended every read of act; stage FP32 [site][68] there
     float* xs = lds;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
+      for (int qq = 0; qq < QN; ++qq) {
         const rc_f32x4 v = out_scale * h[r][qq];
-        *reinterpret_cast<float4*>(xs + ((x0 + r) * kRcLY + j) * kRcStride + 16 * qq + 4 * g16) =
+        *reinterpret_cast<float4*>(xs + ((x0 + r) * kRcLY + j) * kRcStride + 16 * (q0 + qq) + 4 * g16) =
             make_float4(v[0], v[1], v[2], v[3]);
       }
     __syncthreads();
@@ -479,9 +489,10 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
   for (int r = 0; r < 4; ++r) {
     const int site = (x0 + r) * kRcLY + j;
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
+    for (int qq = 0; qq < QN; ++qq) {
+      const int qg = q0 + qq;
       const rc_f32x4 v = out_scale * h[r][qq];
-      *reinterpret_cast<float4*>(yo + ((size_t)(qq >> 1) * N + site) * (kRcC / 2) + 16 * (qq & 1) + 4 * g16) =
+      *reinterpret_cast<float4*>(yo + ((size_t)(qg >> 1) * N + site) * (kRcC / 2) + 16 * (qg & 1) + 4 * g16) =
           make_float4(v[0], v[1], v[2], v[3]);
     }
   }
@@ -691,23 +702,18 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
     if (!m->conv_w_lp) return NH_ERR_UNSUPPORTED;
     const size_t lds_bf16 = sizeof(float) * rb_lds_floats(N);
     const rb_i32x4* wsl = reinterpret_cast<const rb_i32x4*>(m->conv_w_lp);
+    // two waves per four lattice rows, two channel quarters each: 1.73 us per 16x16 image against
+    // 1.89 us with one wave per four rows (one wave per SIMD)
+#define NH_RB_LAUNCH(NT)                                                                                      \
+  hipLaunchKernelGGL((k_resconv_bf16<NT, 2>), dim3(nimg), dim3(NT), lds_bf16, st, m->LX, m->n_layers, m->conv_w, \
+                     wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y)
     switch (m->LX) {
-      case 4:
-        hipLaunchKernelGGL(k_resconv_bf16<64>, dim3(nimg), dim3(64), lds_bf16, st, m->LX, m->n_layers, m->conv_w, wsl,
-                           m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
-        break;
-      case 8:
-        hipLaunchKernelGGL(k_resconv_bf16<128>, dim3(nimg), dim3(128), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
-        break;
-      case 12:
-        hipLaunchKernelGGL(k_resconv_bf16<192>, dim3(nimg), dim3(192), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
-        break;
-      default:
-        hipLaunchKernelGGL(k_resconv_bf16<256>, dim3(nimg), dim3(256), lds_bf16, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+      case 4: NH_RB_LAUNCH(128); break;
+      case 8: NH_RB_LAUNCH(256); break;
+      case 12: NH_RB_LAUNCH(384); break;
+      default: NH_RB_LAUNCH(512);
     }
+#undef NH_RB_LAUNCH
     NH_LAUNCH_CHECK();
     return NH_OK;
   }
