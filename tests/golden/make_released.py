@@ -1,5 +1,5 @@
-"""Writes ``<name>.npz`` for the released tpm02_L16_pfaffian and tp0_L16_bcs states as fixtures
-(run in the build container, where the reference checkout is readable:
+"""Writes ``<name>.npz`` for the released tpm02_L16_pfaffian, tp0_L16_pfaffian and tp0_L16_bcs
+states as fixtures (run in the build container, where the reference checkout is readable:
 ``python tests/golden/make_released.py [reference_root] [name ...]``).
 
 A fixture holds data only: the parameters read from ``weights/<name>.eqx`` (leaf stream read
@@ -27,7 +27,7 @@ from neuralhubbard_amd import released as R  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 from oracle.cpu_port import CpuPath  # noqa: E402
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_bcs")
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs")
 CHAINS = 2048  # chains kept for the E/N check
 N_PSI = 8      # chains with oracle amplitudes
 N_FEAT = 2     # chains with oracle features

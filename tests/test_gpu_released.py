@@ -2,18 +2,19 @@
 
 * synthetic residual CNNs on 4x16 and 8x16 lattices against the FP64 oracle: features,
   bare and projected log|psi| and E_loc;
-* the released tpm02_L16_pfaffian and tp0_L16_bcs states (fixtures tests/golden/<name>.npz,
-  written by make_released.py from the reference's weight and chain files): features, log|psi|
-  and E_loc against the FP64 oracle values stored in the fixtures, and the energy per site over
-  the released chains against the paper: E/N = -0.74533(1), sigma^2/N = 0.0272(5)
-  (code/states.py:114-115) and E/N = -0.75878(2), sigma^2/N = 0.0255(20) (:104-105).
+* the released tpm02_L16_pfaffian, tp0_L16_pfaffian and tp0_L16_bcs states (fixtures
+  tests/golden/<name>.npz, written by make_released.py from the reference's weight and chain
+  files): features, log|psi| and E_loc against the FP64 oracle values stored in the fixtures, and
+  the energy per site over the released chains against the paper: E/N = -0.74533(1),
+  sigma^2/N = 0.0272(5) (code/states.py:114-115), E/N = -0.75813(1), sigma^2/N = 0.0259(2)
+  (:106-107) and E/N = -0.75878(2), sigma^2/N = 0.0255(20) (:104-105).
 
 Both precisions of the network: FP32 MFMA (with the LDS weight slabs, and reading the weights
 directly: "fp32-direct") and split-BF16 MFMA ("mixed").  Tolerances: features
 1e-4 (FP32) or 1e-3 (split BF16) relative to their largest magnitude, log|psi| as described in
 the small-lattice test and 1e-3 at 16x16 (the bar for the 16x16 Pfaffian), signs exact, E_loc
-1e-4 per site.  The E/N check allows four standard errors of the
-chain mean, sigma^2/N a factor 1.5 either way."""
+1e-4 per site.  The E/N check (2048 fixture chains split-BF16, 512 FP32) allows four standard
+errors of the chain mean, sigma^2/N (2048 chains) a factor 1.5 either way."""
 
 import os
 
@@ -100,7 +101,7 @@ def test_small_resconv_matches_oracle(kind, LX, n_el, blocks, prec):
     np.testing.assert_allclose(el / wk.N, er / wk.N, rtol=0, atol=(1e-4 if LX == 4 else 1e-3) * (10 if mixed else 1))
 
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_bcs")
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs")
 
 
 def _released(name="tpm02_L16_pfaffian"):
@@ -145,13 +146,16 @@ def test_released_state_energy_per_site(name, prec):
     from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow
 
     info, wk, p, S, _ = _released(name)
-    K = 512
+    # E_loc / N has heavy tails (tp0_L16_pfaffian chain 353 sits at -0.951, the FP64 oracle
+    # agrees), so sigma^2/N is checked over all 2048 fixture chains, in the faster split-BF16
+    # form; the FP32 form checks E/N over the first 512
+    K = len(S) if prec == "mixed" else 512
     dev = build_backflow(wk, p, prec)
     state = Variational(dev, symm=True)
     H = Hubbard(U=wk.U, tprime=wk.tprime, chunk=64)
     el = []
-    for b0 in range(0, K, 128):
-        s = torch.tensor(S[b0:b0 + 128], device="cuda")
+    for b0 in range(0, K, 256):
+        s = torch.tensor(S[b0:b0 + 256], device="cuda")
         el.append(H.Oloc(state, Samples(s, state(s))).cpu().numpy())
         print(f"{prec}: E_loc of chains {b0}..{b0 + len(s)}", flush=True)
     e = np.concatenate(el) / wk.N
@@ -161,4 +165,5 @@ def test_released_state_energy_per_site(name, prec):
     print(f"E/N = {mean:.5f} +- {err:.5f} (paper {info.E_per_site}), sigma^2/N = {var_per_site:.4f} "
           f"(paper {info.var_per_site})", flush=True)
     assert abs(mean - info.E_per_site) < 4 * err + info.E_err
-    assert info.var_per_site / 1.5 < var_per_site < 1.5 * info.var_per_site
+    if K == len(S):
+        assert info.var_per_site / 1.5 < var_per_site < 1.5 * info.var_per_site

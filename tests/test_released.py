@@ -1,6 +1,6 @@
 """Released-state loaders and the residual CNN restatement on the CPU.
 
-* the fixtures tests/golden/tpm02_L16_pfaffian.npz and tp0_L16_bcs.npz against the reference's
+* the fixtures tests/golden/tpm02_L16_pfaffian.npz, tp0_L16_pfaffian.npz and tp0_L16_bcs.npz against the reference's
   own files when the reference checkout is present (md5 of the weight file, code/states.py:104,
   114; every leaf; the packed chains bit for bit, code/states.py:328-352), and the parameter
   count 1,208,320 of code/states.py:104, 114 without it;
@@ -10,7 +10,7 @@
   recomputation (drift guard);
 * the fixture's FP64 E_loc of released chains against the paper's E/N: one chain's E_loc / N
   has a spread of sqrt(sigma^2/N / N) around it (0.0103 around -0.74533, code/states.py:114-115;
-  0.0100 around -0.75878, :104-105).
+  0.0100 around -0.75878, :104-105; 0.0101 around -0.75813, :106-107).
 """
 
 import hashlib
@@ -28,8 +28,8 @@ REF_WEIGHTS = "/root/reference/weights"
 NAME = "tpm02_L16_pfaffian"
 
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_bcs")
-PAD = {"tpm02_L16_pfaffian": 1791, "tp0_L16_bcs": 1854}  # zero pads of the weight files
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs")
+PAD = {"tpm02_L16_pfaffian": 1791, "tp0_L16_pfaffian": 1791, "tp0_L16_bcs": 1854}  # zero pads of the weight files
 
 
 def fixture(name=NAME):
