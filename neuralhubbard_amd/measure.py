@@ -204,8 +204,8 @@ def run_measure(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", t
 
     * energy: E_mean, E_sqr of E_loc;
     * spin / charge: sz_site, n_site (LX, LY), Czz, Cnn (N,) and the transverse terms Tpm, Tmp
-      (NREF, N) of the reference sites `observables.reference_sites` — sample k serves reference
-      site k mod NREF, so that each site's terms are means over B / NREF samples (the reference
+      (NREF, N) of the reference sites `observables.reference_sites` (NREF of them, fewer on
+      lattices narrower than NREF) — sample k serves reference site k mod NREF, so that each site's terms are means over B / NREF samples (the reference
       evaluates every site on every sample; both estimate the same expectation);
     * dwave: F (16N,), G (6N,) in the order of F_keys, G_keys, which the series holds.
 
@@ -275,7 +275,7 @@ def run_measure(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", t
                 cur["Cnn"] = np64(obs.autocorr(gn, LX, LY))
         if "spin" in channels:
             B = s.shape[0]
-            T = np.zeros((NREF, 2, N))
+            T = np.zeros((len(refs), 2, N))
             for i, (ax, ay) in enumerate(refs):
                 sel = np.arange(i, B, len(refs))
                 if len(sel) == 0:
