@@ -46,6 +46,7 @@ if ROOT not in sys.path:
 METRIC = "samples/s (log-psi+E_loc, 16x16 Pfaffian): mixed+surrogate vs plain FP32"
 PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_image_pf_pf16_hbm_traffic.md")
+TRAFFIC_SUMMARY_RESCONV = os.path.join(ROOT, "profiles", "k_resconv_bf16_tpm02_L16_hbm_traffic.md")
 
 
 def parse():
@@ -78,11 +79,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(images_per_launch):
-    """HBM bytes per launch of k_image_pf: the committed rocprofv3 counter summary's bytes per
-    image (FETCH_SIZE + WRITE_SIZE, copy-calibrated) times this run's images per launch."""
+def read_traffic(images_per_launch, summary=TRAFFIC_SUMMARY):
+    """HBM bytes per launch of the dominant kernel: the committed rocprofv3 counter summary's
+    bytes per image (FETCH_SIZE + WRITE_SIZE, copy-calibrated) times this run's images per launch."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(summary) as f:
             m = re.search(r"traffic_bytes_per_image:\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) * images_per_launch if m else None
     except OSError:
@@ -260,7 +261,11 @@ def main():
     launches, ms = r["kinds"][dom]
     achieved = flops[dom] / (ms * 1e-3)
     arith = "bf16" if (head == "mixed-da" and dom == "features") else "fp32"
-    traffic = read_traffic(images / max(1, launches)) if (dom == "pairing+pfaffian" and not resconv) else None
+    traffic = None
+    if dom == "pairing+pfaffian" and not resconv:
+        traffic = read_traffic(images / max(1, launches))
+    elif dom == "features" and resconv and arith == "bf16":
+        traffic = read_traffic(images / max(1, launches), TRAFFIC_SUMMARY_RESCONV)
     cnn_kernel = ("k_resconv_bf16" if arith == "bf16" else "k_resconv_fs") if resconv else "k_cnn_mfma"
     roof = {"bound": "mfma", "kernel": "k_image_pf" if dom == "pairing+pfaffian" else cnn_kernel,
             "achieved": achieved / 1e12, "peak": PEAK[arith] / 1e12, "unit": "TFLOP/s",
