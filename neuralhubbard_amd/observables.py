@@ -244,8 +244,9 @@ def _apply_ops_sign(occ, ops):
 def dwave_local(state, spins, psi, LX: int, LY: int):
     """Per-sample local values (F (B, 4, 4N), G (B, 3, N, 2)) of the d-wave terms at R = (0, 0):
     F[b, 2 l + oL, 2 (2 site + axis) + oR] for left bond l (0: +x, 1: +y) and every right bond
-    (site, axis), G[b, li, j, s] for the left sites li = (0,0), (1,0), (0,1); diagonal terms from the
-    occupations (``dwave_packed.PackedDwave.measure``)."""
+    (site, axis), G[b, li, j, s] for the left sites li = (0,0), (0,1), (1,0); diagonal terms from the
+    occupations (``dwave_packed.PackedDwave.measure``).  Flattened, F and G follow the reference's
+    ``F_keys`` and ``G_keys`` (`dwave_keys`)."""
     torch = _torch()
     N = LX * LY
     s = spins
@@ -317,6 +318,8 @@ def dwave_measure(state, spins, psi, LX: int, LY: int, chunk: int = 512):
     from .backflow import LogArray
 
     B = spins.shape[0]
+    if B == 0:
+        raise ValueError("no samples to measure")
     F = G = 0.0
     for b0 in range(0, B, chunk):
         sl = slice(b0, min(B, b0 + chunk))
