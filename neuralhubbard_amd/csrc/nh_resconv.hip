@@ -6,10 +6,12 @@
 // (network table of the reference: data/supplement/tableS1_network/network_settings.json:21-56;
 // quantax ResConv as built at code/backflow/models.py:319-329; features at models.py:163-167).
 //
-// Lattices with LY == 16 sites per row and LX % 4 == 0, LX <= 16.  One block of 16 LX threads
-// evaluates one image; wave w owns lattice rows 4w .. 4w+3 for all 64 channels: the residual
-// stream h stays in registers in the MFMA result layout (lane l, register q -> site
-// y = 4 (l / 16) + q, channel 16 quarter + l % 16).  A lattice row of 16 sites is the M = 16
+// Lattices with LY == 16 sites per row and LX % 4 == 0, LX <= 16.  One block evaluates one
+// image.  In k_resconv (16 LX threads) wave w owns lattice rows 4w .. 4w+3 for all 64 channels;
+// the slab kernels k_resconv_fs / k_resconv_bf16 (32 LX threads) give those rows to two waves
+// with two channel quarters each, so that two waves share every SIMD.  The residual stream h
+// stays in registers in the MFMA result layout (lane l, register q -> site y = 4 (l / 16) + q,
+// channel 16 quarter + l % 16).  A lattice row of 16 sites is the M = 16
 // dimension of an MFMA tile, so the y-shift of a 3x3 tap is a DPP rotation inside rows of 16
 // lanes (periodic for free) and the x-shift reads the neighbouring lattice row.  Each conv
 // input sits in LDS as [site][64 + 4 pad], channel c at (c % 4) * 16 + c / 4, so a lane's 8
@@ -62,20 +64,26 @@ __device__ inline void rc_head(const float* xs, const float* __restrict__ head, 
   }
 }
 
-// act[site][pos(c)] = gelu(scale * v) for the wave's rows x0 .. x0+3
-__device__ inline void rc_store(float* act, const rc_f32x4 (&v)[4][4], float scale, int x0, int lane) {
+// act[site][pos(c)] = gelu(scale * v) for the wave's rows x0 .. x0+3 and channel quarters
+// q0 .. q0+QN-1
+template <int QN>
+__device__ inline void rc_store_q(float* act, const rc_f32x4 (&v)[4][QN], float scale, int x0, int q0, int lane) {
   const int j = lane & 15, g16 = lane >> 4;
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
-      const int pos = rc_pos(16 * qq + j);
+    for (int qq = 0; qq < QN; ++qq) {
+      const int pos = rc_pos(16 * (q0 + qq) + j);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int site = (x0 + r) * kRcLY + 4 * g16 + q;
         act[site * kRcStride + pos] = gelu_tanh(scale * v[r][qq][q]);
       }
     }
+}
+
+__device__ inline void rc_store(float* act, const rc_f32x4 (&v)[4][4], float scale, int x0, int lane) {
+  rc_store_q<4>(act, v, scale, x0, 0, lane);
 }
 
 // acc = bias + periodic conv3x3 of act (weights [tap][c_in][c_out]) on rows x0 .. x0+3
@@ -505,9 +513,9 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
 // [16 qq + l % 16] at element kk (neuralhubbard_amd/mixed.py, pack_resconv_f32; 24 KB, the same
 // size as a split-BF16 slab), double-buffered in LDS, the next slab in flight during the MFMAs.
 
-template <int DYI>
-__device__ __forceinline__ void rs_taps(const float* act, const float4* wb, int LX, int x0, int lane, int kh,
-                                        rc_f32x4 (&acc)[4][4]) {
+template <int DYI, int QN>
+__device__ __forceinline__ void rs_taps(const float* act, const float4* wb, int LX, int x0, int q0, int lane, int kh,
+                                        rc_f32x4 (&acc)[4][QN]) {
   const int j = lane & 15, g16 = lane >> 4;
   float fs[6][8];
 #pragma unroll
@@ -528,8 +536,8 @@ __device__ __forceinline__ void rs_taps(const float* act, const float4* wb, int 
 #pragma unroll
   for (int dxi = 0; dxi < 3; ++dxi) {
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
-      const float4* w = wb + ((dxi * 4 + qq) * kWave + lane) * 2;
+    for (int qq = 0; qq < QN; ++qq) {
+      const float4* w = wb + ((dxi * 4 + q0 + qq) * kWave + lane) * 2;
       const float4 w0 = w[0], w1 = w[1];
       const float wv[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
 #pragma unroll
@@ -541,9 +549,10 @@ __device__ __forceinline__ void rs_taps(const float* act, const float4* wb, int 
   }
 }
 
-template <int NT, int DYI>
+template <int NT, int DYI, int QN>
 __device__ __forceinline__ void rs_step(const float* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
-                                        int n_slabs, int LX, int x0, int t, int kh, rc_f32x4 (&acc)[4][4]) {
+                                        int n_slabs, int LX, int x0, int q0, int t, int kh,
+                                        rc_f32x4 (&acc)[4][QN]) {
   constexpr int kPer = rb_per_thread<NT>();
   const bool more = k + 1 < n_slabs;
   rb_i32x4 v[kPer];
@@ -552,8 +561,8 @@ __device__ __forceinline__ void rs_step(const float* act, rb_i32x4* wbuf, const 
 #pragma unroll
     for (int i = 0; i < kPer; ++i) v[i] = src[t + i * NT];
   }
-  rs_taps<DYI>(act, reinterpret_cast<const float4*>(wbuf + (k & 1) * 2 * kSlabFrag), LX, x0, t & (kWave - 1), kh,
-               acc);
+  rs_taps<DYI, QN>(act, reinterpret_cast<const float4*>(wbuf + (k & 1) * 2 * kSlabFrag), LX, x0, q0,
+                   t & (kWave - 1), kh, acc);
   if (more) {
     rb_i32x4* dst = wbuf + ((k + 1) & 1) * 2 * kSlabFrag;
 #pragma unroll
@@ -563,22 +572,22 @@ __device__ __forceinline__ void rs_step(const float* act, rb_i32x4* wbuf, const 
   ++k;
 }
 
-template <int NT>
+template <int NT, int QN>
 __device__ __forceinline__ void rs_conv(const float* act, rb_i32x4* wbuf, const rb_i32x4* __restrict__ Wsl, int& k,
-                                        int n_slabs, const float* __restrict__ bl, int LX, int x0, int t,
-                                        rc_f32x4 (&acc)[4][4]) {
+                                        int n_slabs, const float* __restrict__ bl, int LX, int x0, int q0, int t,
+                                        rc_f32x4 (&acc)[4][QN]) {
   const int j = t & 15;
 #pragma unroll
-  for (int qq = 0; qq < 4; ++qq) {
-    const float bv = bl[16 * qq + j];
+  for (int qq = 0; qq < QN; ++qq) {
+    const float bv = bl[16 * (q0 + qq) + j];
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r][qq] = rc_f32x4{bv, bv, bv, bv};
   }
 #pragma unroll 1
   for (int kh = 0; kh < 2; ++kh) {
-    rs_step<NT, 0>(act, wbuf, Wsl, k, n_slabs, LX, x0, t, kh, acc);
-    rs_step<NT, 1>(act, wbuf, Wsl, k, n_slabs, LX, x0, t, kh, acc);
-    rs_step<NT, 2>(act, wbuf, Wsl, k, n_slabs, LX, x0, t, kh, acc);
+    rs_step<NT, 0, QN>(act, wbuf, Wsl, k, n_slabs, LX, x0, q0, t, kh, acc);
+    rs_step<NT, 1, QN>(act, wbuf, Wsl, k, n_slabs, LX, x0, q0, t, kh, acc);
+    rs_step<NT, 2, QN>(act, wbuf, Wsl, k, n_slabs, LX, x0, q0, t, kh, acc);
   }
 }
 
@@ -586,7 +595,8 @@ __host__ __device__ constexpr size_t rs_lds_floats(int N) {
   return (size_t)N * kRcStride + 16 * kSlabFrag + 16;  // activations, two slabs, partials
 }
 
-template <int NT>
+// QN channel quarters per wave, as in k_resconv_bf16 (QN = 2: two waves per four lattice rows).
+template <int NT, int QN>
 __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const float* __restrict__ emb,
                                                    const rb_i32x4* __restrict__ Wsl, const float* __restrict__ Bc,
                                                    const float* __restrict__ head, float out_scale,
@@ -594,6 +604,7 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
                                                    const int32_t* __restrict__ item_b,
                                                    const int32_t* __restrict__ item_hop, int img0, int img_end,
                                                    int ipi, float* __restrict__ Y) {
+  static_assert(QN == 4 || QN == 2, "four or two channel quarters per wave");
   extern __shared__ float lds[];
   const int N = LX * kRcLY;
   float* act = lds;
@@ -608,13 +619,14 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
   const int b = item_b ? item_b[item] : item;
   const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
   constexpr int nw = NT / kWave;
-  const int x0 = 4 * wave, j = lane & 15, g16 = lane >> 4;
+  constexpr int nrg = nw * QN / 4;  // row groups of 4 lattice rows (= LX / 4)
+  const int x0 = 4 * (wave % nrg), q0 = QN * (wave / nrg), j = lane & 15, g16 = lane >> 4;
   const int n_slabs = 2 * n_blocks * kSlabsPerConv;
   int k = 0;
 #pragma unroll
   for (int i = 0; i < rb_per_thread<NT>(); ++i) wbuf[t + i * NT] = Wsl[t + i * NT];  // slab 0
 
-  rc_f32x4 h[4][4];
+  rc_f32x4 h[4][QN];
   {
     const int8_t* sb = s + (size_t)b * 2 * N;
 #pragma unroll
@@ -626,7 +638,7 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
         const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
         const float* e = emb + (up + 2 * dn) * kRcC + j;
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) h[r][qq][q] = e[16 * qq];
+        for (int qq = 0; qq < QN; ++qq) h[r][qq][q] = e[16 * (q0 + qq)];
       }
   }
   const float inv_count = 1.0f / (float)(kRcC * N);
@@ -635,7 +647,7 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq)
+      for (int qq = 0; qq < QN; ++qq)
 #pragma unroll
         for (int q = 0; q < 4; ++q) ss = fmaf(h[r][qq][q], h[r][qq][q], ss);
     ss = rc_wave_sum(ss);
@@ -645,26 +657,26 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
 #pragma unroll
     for (int w = 0; w < nw; ++w) tot += part[w];
     const float rinv = 1.0f / sqrtf(fmaf(tot, inv_count, kRmsEps));
-    rc_store(act, h, rinv, x0, lane);
+    rc_store_q<QN>(act, h, rinv, x0, q0, lane);
     __syncthreads();
-    rc_f32x4 acc[4][4];
-    rs_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk) * kRcC, LX, x0, t, acc);
-    rc_store(act, acc, 1.0f, x0, lane);  // the last slab barrier ended every read of act
+    rc_f32x4 acc[4][QN];
+    rs_conv<NT, QN>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk) * kRcC, LX, x0, q0, t, acc);
+    rc_store_q<QN>(act, acc, 1.0f, x0, q0, lane);  // the last slab barrier ended every read of act
     __syncthreads();
-    rs_conv<NT>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk + 1) * kRcC, LX, x0, t, acc);
+    rs_conv<NT, QN>(act, wbuf, Wsl, k, n_slabs, Bc + (2 * blk + 1) * kRcC, LX, x0, q0, t, acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) h[r][qq] += acc[r][qq];
+      for (int qq = 0; qq < QN; ++qq) h[r][qq] += acc[r][qq];
   }
   if (head) {  // BCS: the last slab barrier ended every read of act
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq)
+      for (int qq = 0; qq < QN; ++qq)
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          act[((x0 + r) * kRcLY + 4 * g16 + q) * kRcStride + 16 * qq + j] = out_scale * h[r][qq][q];
+          act[((x0 + r) * kRcLY + 4 * g16 + q) * kRcStride + 16 * (q0 + qq) + j] = out_scale * h[r][qq][q];
     __syncthreads();
     rc_head(act, head, N, Y + (size_t)(img - img0) * (2 * N * kRcC));
     return;
@@ -673,11 +685,12 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq)
+    for (int qq = 0; qq < QN; ++qq)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int site = (x0 + r) * kRcLY + 4 * g16 + q;
-        yo[((size_t)(qq >> 1) * N + site) * (kRcC / 2) + 16 * (qq & 1) + j] = out_scale * h[r][qq][q];
+        const int qg = q0 + qq;
+        yo[((size_t)(qg >> 1) * N + site) * (kRcC / 2) + 16 * (qg & 1) + j] = out_scale * h[r][qq][q];
       }
 }
 
@@ -720,23 +733,18 @@ int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int
   if (m->conv_w_mfma) {  // FP32 weight slabs
     const size_t lds_fs = sizeof(float) * rs_lds_floats(N);
     const rb_i32x4* wsl = reinterpret_cast<const rb_i32x4*>(m->conv_w_mfma);
+    // two waves per four lattice rows, as in the split-BF16 kernel: 5.05 us per 16x16 image
+    // against 5.43 us with one wave per four rows
+#define NH_RS_LAUNCH(NT)                                                                                     \
+  hipLaunchKernelGGL((k_resconv_fs<NT, 2>), dim3(nimg), dim3(NT), lds_fs, st, m->LX, m->n_layers, m->conv_w, \
+                     wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y)
     switch (m->LX) {
-      case 4:
-        hipLaunchKernelGGL(k_resconv_fs<64>, dim3(nimg), dim3(64), lds_fs, st, m->LX, m->n_layers, m->conv_w, wsl,
-                           m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
-        break;
-      case 8:
-        hipLaunchKernelGGL(k_resconv_fs<128>, dim3(nimg), dim3(128), lds_fs, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
-        break;
-      case 12:
-        hipLaunchKernelGGL(k_resconv_fs<192>, dim3(nimg), dim3(192), lds_fs, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
-        break;
-      default:
-        hipLaunchKernelGGL(k_resconv_fs<256>, dim3(nimg), dim3(256), lds_fs, st, m->LX, m->n_layers, m->conv_w,
-                           wsl, m->conv_b, head, scale, s, item_b, item_hop, img0, img_end, ipi, Y);
+      case 4: NH_RS_LAUNCH(128); break;
+      case 8: NH_RS_LAUNCH(256); break;
+      case 12: NH_RS_LAUNCH(384); break;
+      default: NH_RS_LAUNCH(512);
     }
+#undef NH_RS_LAUNCH
     NH_LAUNCH_CHECK();
     return NH_OK;
   }
