@@ -67,7 +67,8 @@ enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1, NH_NET_RESCONV_HEAD = 2 };
  *           added as phi0[r(up_p - dn_q)].
  * Sizes: N = LX * LY <= 1024; pfaffian n_up + n_dn even and <= 1024; bcs n_up == n_dn <= 512
  * (the 32x32 lattice at 7/8 filling has n_up = 448); matrices above n = 64 (pfaffian) or
- * n = 192 (bcs) live in the workspace. */
+ * n = 192 (bcs, and from n = 65 where the features take most of the LDS, d = 64) live in the
+ * workspace. */
 typedef struct nh_model {
   int32_t kind;       /* NH_KIND_* */
   int32_t LX, LY;     /* periodic lattice, both >= 3 */

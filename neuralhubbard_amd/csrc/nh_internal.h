@@ -27,8 +27,9 @@ bool resconv_supported(const nh_model* m);
 
 // Pairing block + Pfaffian (or determinant) of images [img0, img_end), writing the signed
 // log of eta_g(s) psi(g.s) per image into img_sign / img_logabs (indexed by global image).
-// Pfaffians above n = 64 keep their triangle, BCS determinants above n = 192 their matrix, in
-// gtri, image_global_floats(m) floats per image of the chunk (NULL otherwise).
+// Pfaffians above n = 64 keep their triangle, BCS determinants above n = 192 (or with d = 64
+// features) their matrix, in gtri, image_global_floats(m) floats per image of the chunk (NULL
+// otherwise).
 int launch_image_amplitude(hipStream_t st, const nh_model* m, const int8_t* s,
                            const int32_t* item_b, const int32_t* item_hop, int img0, int img_end,
                            int ipi, const float* Y, float* img_sign, double* img_logabs, float* gtri);

@@ -7,7 +7,8 @@
 //   n > 64:   pf_blocked2 (blocked Parlett-Reid: left-looking panels, packed pivot keys, two
 //             barriers per step, FP32 MFMA trailing updates) and lu_blocked (blocked LU with
 //             rows in place, one barrier per step, MFMA trailing updates).  The BCS image kernel
-//             keeps its matrix in LDS up to n = 192 and in a global workspace above (n <= 512).
+//             keeps its matrix in LDS up to n = 192 (unless that leaves one workgroup per CU) and
+//             in a global workspace otherwise (n <= 512).
 // Parlett-Reid for the Pfaffian follows Wimmer, ACM TOMS 38 (2012) 30 (pfaffian_LTL): with
 // pivot p = A[k+1][k], Pf *= -p, tau_i = A[i][k] / p, v_i = A[i][k+1] and
 // A[i][j] += tau_i v_j - v_i tau_j for i > j >= k + 2.
@@ -914,7 +915,7 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 
 // ------------------------------------------------------------------ BCS image kernel
 
-// GM (n > kLdsDetMaxN): the image's n x (n + 1) matrix sits in the global workspace gmat at
+// GM (bcs_global_matrix): the image's n x (n + 1) matrix sits in the global workspace gmat at
 // (img - img0) * gstride floats (followed by kWave dummy floats); LDS holds the features of the
 // occupied modes (then the elimination scratch), the mode list and a flag.
 template <int D, int NT, bool GM>
@@ -1032,8 +1033,24 @@ static size_t pf_lds_bytes(int n, int N, int D) {
          sizeof(int) * (size_t)(2 * n + 4 + 2 * 16 + 2);
 }
 
+static size_t bcs_lds_bytes(int n, int ne, int D, bool gm) {
+  const size_t mat = gm ? 0 : (size_t)((n * (n + 1) + kWave + 3) & ~3);
+  return sizeof(float) * (mat + (size_t)(2 * n * D + 2 * n)) + sizeof(int) * (size_t)(ne + 4);
+}
+
+// BCS determinant matrix in the global workspace rather than in LDS: above n = 192, and where the
+// LDS-resident form would leave room for only one workgroup per CU (the d = 64 features of the
+// released BCS state: 2.75 us per 16x16 image with the matrix in the workspace against 3.02 us;
+// at d = 16 the LDS form keeps two workgroups per CU and is faster, 0.86 against 1.02 us).
+static bool bcs_global_matrix(const nh_model* m) {
+  const int n = m->n_up;
+  if (n > kLdsDetMaxN) return true;
+  return n > 64 && 2 * bcs_lds_bytes(n, m->n_up + m->n_dn, feature_dim(m), false) > 160 * 1024;
+}
+
 int64_t image_chunk(const nh_model* m) {
-  if (m->kind == NH_KIND_BCS) return m->n_up <= kLdsDetMaxN ? 32768 : 1024;  // 0.8 MB per image at n = 448
+  if (m->kind == NH_KIND_BCS)
+    return !bcs_global_matrix(m) ? 32768 : (m->n_up <= 256 ? 8192 : 1024);  // 0.8 MB per image at n = 448
   const int n = m->n_up + m->n_dn;
   if (n <= kLdsTriMaxN) return 32768;
   return n <= 256 ? 8192 : 1024;  // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB)
@@ -1042,16 +1059,11 @@ int64_t image_chunk(const nh_model* m) {
 size_t image_global_floats(const nh_model* m) {
   if (m->kind == NH_KIND_BCS) {
     const size_t n = (size_t)m->n_up;
-    return n <= (size_t)kLdsDetMaxN ? 0 : ((n * (n + 1) + kWave + 63) & ~(size_t)63);
+    return !bcs_global_matrix(m) ? 0 : ((n * (n + 1) + kWave + 63) & ~(size_t)63);
   }
   const int n = m->n_up + m->n_dn;
   if (n <= kLdsTriMaxN) return 0;
   return (size_t)((((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63);
-}
-
-static size_t bcs_lds_bytes(int n, int ne, int D) {
-  const size_t mat = n > kLdsDetMaxN ? 0 : (size_t)((n * (n + 1) + kWave + 3) & ~3);
-  return sizeof(float) * (mat + (size_t)(2 * n * D + 2 * n)) + sizeof(int) * (size_t)(ne + 4);
 }
 
 constexpr size_t kMaxLds = 160 * 1024;
@@ -1084,13 +1096,14 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
     }
   } else {
     const int n = m->n_up;
-    const size_t lds = bcs_lds_bytes(n, m->n_up + m->n_dn, D);
+    const bool gm = bcs_global_matrix(m);
+    const size_t lds = bcs_lds_bytes(n, m->n_up + m->n_dn, D, gm);
     if (lds > kMaxLds) return NH_ERR_SHAPE;
     if (n <= 64) {
       hipLaunchKernelGGL((k_image_bcs<D, 64, false>), dim3(nimg), dim3(64), lds, st, N, m->LX, m->LY, m->n_up,
                          m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
                          img_logabs, nullptr, (size_t)0);
-    } else if (n <= kLdsDetMaxN) {
+    } else if (!gm) {
       hipLaunchKernelGGL((k_image_bcs<D, 512, false>), dim3(nimg), dim3(512), lds, st, N, m->LX, m->LY, m->n_up,
                          m->n_dn, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
                          img_logabs, nullptr, (size_t)0);
