@@ -315,3 +315,82 @@ def run_energy(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", tp
                log=print) -> dict:
     """`run_measure` with the energy channel alone."""
     return run_measure(state, H, sampler, nsweeps, out_dir, name, tprime, ("energy",), log=log)
+
+
+# ------------------------------------------------------------------------ command line
+
+def _load_state_files(name: str, weights: str, spins: str | None):
+    """(workload, params, chains) of a released state from its ``.eqx`` weight file (and
+    ``_spins.npz`` chains), or from a parameter ``.npz`` holding the same arrays and the packed
+    chains (``conv_w``, ``conv_b``, ``W``, ``theta``, ``emb``, optional ``head`` / ``phi0``,
+    ``bits``, ``shape``)."""
+    from .configs import Params
+    from .released import STATES, load_released, load_spins, unpack_spins
+
+    info = STATES[name]
+    if weights.endswith(".eqx"):
+        wk, params = load_released(name, weights)
+        chains = None
+    else:
+        with np.load(weights, allow_pickle=False) as f:
+            g = {k: f[k] for k in f.files}
+        wk = info.workload()
+        params = Params(list(g["conv_w"]), list(g["conv_b"]), g["W"], float(g["theta"]), None, g.get("phi0"),
+                        g["emb"], g.get("head"))
+        chains = unpack_spins(g["bits"], g["shape"]) if "bits" in g else None
+    if spins is not None:
+        chains = load_spins(spins, info)
+    if chains is None:
+        raise ValueError("no initial chains: give --spins (weights/<state>_spins.npz)")
+    return wk, params, chains
+
+
+def main(argv=None) -> dict:
+    """The command line of ``code/measure_orderparams.py`` on a released state:
+
+        python -m neuralhubbard_amd.measure --state tpm02_L16_pfaffian \\
+            --weights weights/tpm02_L16_pfaffian.eqx --spins weights/tpm02_L16_pfaffian_spins.npz \\
+            --out runs/tpm02_L16_pfaffian
+
+    The chains start at the released chains (repeated or truncated to --nsamples), are
+    thermalised for --thermal x N exchange proposals and measured every --sweep-steps x N
+    proposals (defaults 50 and 4, the paper's values for the 16x16 states, ``code/states.py:99-126``).
+    Writes series.npz, spins_last.npz, results.json and results.npz into --out."""
+    import argparse
+
+    import torch
+
+    from .backflow import Hubbard, Variational, build_backflow
+    from .released import STATES
+    from .sampler import MetropolisSampler
+
+    p = argparse.ArgumentParser(description="Measure energy, spin, charge and d-wave correlations of a released state.")
+    p.add_argument("--state", required=True, choices=sorted(STATES))
+    p.add_argument("--weights", required=True, help="weights/<state>.eqx, or a parameter .npz with packed chains")
+    p.add_argument("--spins", help="weights/<state>_spins.npz (default: the chains of a parameter .npz)")
+    p.add_argument("--out", required=True, help="output folder")
+    p.add_argument("--nsamples", type=int, default=8192, help="chains, one sample each per sweep")
+    p.add_argument("--nsweeps", type=int, default=64)
+    p.add_argument("--thermal", type=float, default=50.0, help="thermalisation proposals per chain / N")
+    p.add_argument("--sweep-steps", type=float, default=4.0, help="proposals per chain between sweeps / N")
+    p.add_argument("--channels", default=",".join(CHANNELS), help="comma-separated subset of " + ",".join(CHANNELS))
+    p.add_argument("--precision", default="mixed", choices=["fp32", "mixed"])
+    p.add_argument("--seed", type=int, default=42)
+    args = p.parse_args(argv)
+    channels = tuple(c for c in args.channels.split(",") if c)
+    if not set(channels) <= set(CHANNELS):
+        p.error(f"--channels must be a subset of {','.join(CHANNELS)}")
+    info = STATES[args.state]
+    wk, params, chains = _load_state_files(args.state, args.weights, args.spins)
+    state = Variational(build_backflow(wk, params, args.precision))
+    start = np.resize(chains, (args.nsamples, 2 * wk.N))
+    print(f"{info.name}: {wk.LX}x{wk.LY}, {args.nsamples} chains, {args.precision}; thermalising", flush=True)
+    sampler = MetropolisSampler(state, torch.as_tensor(start), int(round(args.sweep_steps * wk.N)), seed=args.seed,
+                                thermal_steps=int(round(args.thermal * wk.N)))
+    H = Hubbard(U=info.U, t=1.0, tprime=info.tprime)
+    return run_measure(state, H, sampler, args.nsweeps, args.out, name=info.name, tprime=info.tprime,
+                       channels=channels)
+
+
+if __name__ == "__main__":
+    main()

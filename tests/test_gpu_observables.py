@@ -179,3 +179,24 @@ def test_run_measure_all_channels(tmp_path):
     assert set(js) >= {"energy", "spin", "charge", "dwave", "local"}
     with np.load(os.path.join(out, "results.npz"), allow_pickle=False) as f:
         assert f["C_p"].shape == (64,) and np.isfinite(f["C_p"]).all()
+
+
+def test_measure_command_line_on_released_state(tmp_path):
+    """`python -m neuralhubbard_amd.measure` on the released tpm02_L16_pfaffian state (fixture
+    parameters and chains; 16 chains, 2 sweeps of N / 2 proposals, every channel): the files of
+    code/measure_orderparams.py and an E/N near the paper's -0.74533."""
+    import json
+
+    from neuralhubbard_amd import measure as M
+
+    path = os.path.join(GOLDEN, "tpm02_L16_pfaffian.npz")
+    out = str(tmp_path / "run")
+    print("measure command line: tpm02_L16_pfaffian, 16 chains, 2 sweeps", flush=True)
+    res = M.main(["--state", "tpm02_L16_pfaffian", "--weights", path, "--out", out, "--nsamples", "16",
+                  "--nsweeps", "2", "--thermal", "0.5", "--sweep-steps", "0.5"])
+    for f in ("series.npz", "spins_last.npz", "results.json", "results.npz"):
+        assert os.path.exists(os.path.join(out, f)), f
+    with open(os.path.join(out, "results.json")) as fh:
+        js = json.load(fh)
+    assert set(js) >= {"energy", "spin", "charge", "dwave", "local"}
+    assert -0.80 < res["energy"]["E_per_site"] < -0.70

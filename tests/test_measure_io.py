@@ -175,3 +175,16 @@ def test_spin_assembly_matches_direct_sums():
     Sk = obs.structure_factor(Czz.mean(0), 8, 8)
     assert c["N2"] == pytest.approx((Sk[2, 0] + Sk[6, 0] + Sk[0, 2] + Sk[0, 6]) / 4 / 64, rel=1e-10)
     assert [tuple(k) for k in r["k_alpha"]] == [(3, 4), (5, 4), (4, 3), (4, 5)]
+
+
+def test_command_line_loads_released_fixture_and_rejects_bad_channels():
+    """`measure.main`'s loader reads a parameter .npz with packed chains (the fixture format) and
+    the argument check of code/measure_orderparams.py:44-48 rejects unknown channels."""
+    from neuralhubbard_amd.released import STATES
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tpm02_L16_pfaffian.npz")
+    wk, params, chains = M._load_state_files("tpm02_L16_pfaffian", path, None)
+    assert wk.network == "resconv" and len(params.conv_w) == 32 and chains.shape == (2048, 512)
+    assert np.all((chains[:, :256] > 0).sum(1) == STATES["tpm02_L16_pfaffian"].n_up)
+    with pytest.raises(SystemExit):
+        M.main(["--state", "tpm02_L16_pfaffian", "--weights", path, "--out", "x", "--channels", "energy,magic"])
