@@ -52,11 +52,13 @@ enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1, NH_NET_RESCONV_HEAD = 2 };
  *   n_layers blocks h <- h + conv3x3(gelu_tanh(conv3x3(gelu_tanh(rmsnorm(h))))) with
  *   rmsnorm(h) = h / sqrt(mean over all channels and sites of h^2 + 1e-5), then
  *   h / sqrt(n_layers + 1).  conv_w = emb [4][C] followed by the 2 n_layers convs,
- *   conv_b [2 n_layers][C].  C == 64, LY == 16, LX in {4, 8, 12, 16}.  NH_PREC_FP32 runs FP32
+ *   conv_b [2 n_layers][C].  C == 64; on LY == 16, LX in {4, 8, 12, 16} NH_PREC_FP32 runs FP32
  *   MFMA; NH_PREC_MIXED runs split-BF16 MFMA (FP32 accumulation) on conv_w_lp, the BF16
  *   A-fragments of the convs (layout in neuralhubbard_amd/mixed.py, pack_resconv_bf16).
  *   conv_w_mfma, when given, holds the FP32 convs as B-fragment slabs (pack_resconv_f32) that
- *   the FP32 kernel streams through LDS; NULL reads conv_w directly.
+ *   the FP32 kernel streams through LDS; NULL reads conv_w directly.  Other lattices
+ *   (N <= 1024, no head) run the network layer by layer in FP32 on the VALU with the
+ *   activations in the workspace, in both precisions.
  * network NH_NET_RESCONV_HEAD: the same network followed by the BCS states' linear head
  *   (code/backflow/models.py:261-270): y[k][site][a] = sum_c head[k][a][c] x[c][site], x the
  *   scaled output, d = C features per spin; conv_w ends with the head as [k][c][a] (2 x C x C);
@@ -131,6 +133,12 @@ int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const 
 /* ---- a4: backflow features  (replaces PfaffianBackflow.features, code/backflow/models.py:163-167)
  * s int8 [batch][2N] -> y float [batch][2][N][d]  (no projection; the identity image). */
 int nh_features(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* y);
+/* The same with a workspace: the residual CNN on lattices outside its tile kernels (LY != 16 or
+ * LX > 16, e.g. the released 24x24 state) runs layer by layer with its activations there, and
+ * nh_features returns NH_ERR_UNSUPPORTED for it.  The query returns 256 bytes where none is needed. */
+size_t nh_features_workspace_size(const nh_model* model, int32_t batch);
+int nh_features_ws(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* y, void* workspace,
+                   size_t workspace_bytes);
 
 /* ---- a12 + model __call__: projected amplitude  (replaces quantax Variational.__call__ over
  * PfaffianBackflow/BCSBackflow.__call__, code/backflow/models.py:208-211, 294-302,

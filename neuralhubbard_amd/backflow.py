@@ -271,8 +271,13 @@ class _Backflow:
         w = self.workload
         y = torch.empty((s.shape[0], 2, w.N, w.d), dtype=torch.float32, device="cuda")
         m = self.struct(False)
-        _lib.check(_lib.load().nh_features(_lib.stream_handle(), ctypes.byref(m), s.shape[0], _lib.ptr(s),
-                                           _lib.ptr(y)), "nh_features")
+        lib = _lib.load()
+        need = lib.nh_features_workspace_size(ctypes.byref(m), s.shape[0])
+        if need == 0:
+            raise _lib.NhError("invalid model description")
+        ws = torch.empty(int(need), dtype=torch.uint8, device="cuda")
+        _lib.check(lib.nh_features_ws(_lib.stream_handle(), ctypes.byref(m), s.shape[0], _lib.ptr(s), _lib.ptr(y),
+                                      _lib.ptr(ws), ws.numel()), "nh_features_ws")
         return y
 
     def __call__(self, s) -> LogArray:

@@ -104,6 +104,7 @@ struct ItemBuffers {
   float* img_sign;
   double* img_logabs;
   float* gtri;  // Pfaffian triangles above n = 64 / BCS matrices above n = 192, one per image of a chunk
+  float* net;   // residual-CNN activations of a chunk on lattices outside its tile kernels (or NULL)
   int64_t chunk;
 };
 
@@ -116,6 +117,8 @@ static ItemBuffers carve_items(Carver& c, const nh_model* m, int64_t n_items) {
   b.chunk = std::min<int64_t>(images, image_chunk(m));
   b.Y = c.take<float>((size_t)b.chunk * 2 * N * D);
   b.gtri = tri ? c.take<float>((size_t)b.chunk * tri) : nullptr;
+  const size_t net = resconv_scratch_floats(m, b.chunk);
+  b.net = net ? c.take<float>(net) : nullptr;
   b.img_sign = c.take<float>((size_t)images);
   b.img_logabs = c.take<double>((size_t)images);
   return b;
@@ -131,7 +134,7 @@ static int run_items(hipStream_t st, const nh_model* m, const int8_t* s, const i
     int rc;
     {
       ProfScope ps(st, NH_PROF_FEATURES);
-      rc = launch_cnn(st, m, s, item_b, item_hop, (int)i0, (int)i1, ipi, b.Y);
+      rc = launch_cnn(st, m, s, item_b, item_hop, (int)i0, (int)i1, ipi, b.Y, b.net);
     }
     if (rc != NH_OK) return rc;
     {

@@ -142,9 +142,9 @@ static int launch_c(hipStream_t st, const nh_model* m, const int8_t* s, const in
 }
 
 int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
-               const int32_t* item_hop, int img0, int img_end, int ipi, float* Y) {
+               const int32_t* item_hop, int img0, int img_end, int ipi, float* Y, float* scratch) {
   if (m->network == NH_NET_RESCONV || m->network == NH_NET_RESCONV_HEAD)
-    return launch_resconv(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);
+    return launch_resconv(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, scratch);
   if (m->precision == NH_PREC_MIXED)
     return launch_cnn_mfma(st, m, true, s, item_b, item_hop, img0, img_end, ipi, Y);
   if (m->conv_w_mfma && cnn_mfma_supported(m))
@@ -165,4 +165,20 @@ extern "C" int nh_features(void* stream, const nh_model* model, int32_t batch, c
   int rc = nh::check_model(model);
   if (rc != NH_OK) return rc;
   return nh::launch_cnn((hipStream_t)stream, model, s, nullptr, nullptr, 0, batch, 1, y);
+}
+
+extern "C" size_t nh_features_workspace_size(const nh_model* model, int32_t batch) {
+  if (!model || batch < 0 || nh::check_model(model) != NH_OK) return 0;
+  return sizeof(float) * nh::resconv_scratch_floats(model, batch) + 256;
+}
+
+extern "C" int nh_features_ws(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* y,
+                              void* workspace, size_t workspace_bytes) {
+  if (!model || !s || !y || batch < 0) return NH_ERR_ARG;
+  int rc = nh::check_model(model);
+  if (rc != NH_OK) return rc;
+  const size_t need = sizeof(float) * nh::resconv_scratch_floats(model, batch);
+  if (need && (!workspace || workspace_bytes < need)) return NH_ERR_WORKSPACE;
+  return nh::launch_cnn((hipStream_t)stream, model, s, nullptr, nullptr, 0, batch, 1, y,
+                        need ? static_cast<float*>(workspace) : nullptr);
 }

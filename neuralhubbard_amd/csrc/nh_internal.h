@@ -16,14 +16,18 @@ int cnn_slots(int N);
 // Features of images [img0, img_end) of an item list (ipi images per item) into
 // Y[(img - img0)][2N][d].
 int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
-               const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
+               const int32_t* item_hop, int img0, int img_end, int ipi, float* Y,
+               float* scratch = nullptr);
 int launch_cnn_mfma(hipStream_t st, const nh_model* m, bool bf16, const int8_t* s, const int32_t* item_b,
                     const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
 bool cnn_mfma_supported(const nh_model* m);
 // the residual CNN of the released states (network == NH_NET_RESCONV)
 int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
-                   const int32_t* item_hop, int img0, int img_end, int ipi, float* Y);
+                   const int32_t* item_hop, int img0, int img_end, int ipi, float* Y, float* scratch);
 bool resconv_supported(const nh_model* m);
+// floats of workspace the residual CNN needs for `images` images (lattices outside the tile
+// kernels run layer by layer with two activation buffers; 0 otherwise)
+size_t resconv_scratch_floats(const nh_model* m, int64_t images);
 
 // Pairing block + Pfaffian (or determinant) of images [img0, img_end), writing the signed
 // log of eta_g(s) psi(g.s) per image into img_sign / img_logabs (indexed by global image).

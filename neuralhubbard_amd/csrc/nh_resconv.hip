@@ -694,17 +694,184 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
       }
 }
 
+// ---------------------------------------------------------------- other lattices
+//
+// Lattices outside the tile kernels (LY != 16, or more than 16 rows) run the network layer by
+// layer over a chunk of images, FP32 on the VALU, with the activations in the workspace.  The
+// residual stream H of image i is kept as [half][site][32] (channel c at half c / 32), the layout
+// of the features y[spin][site][a], so that the scaled output lands in Y in place; A and B are
+// two [site][channel] activation buffers of the same size and rinv one float per image.
+
+__device__ inline int rg_at(int N, int site, int c) { return ((c >> 5) * N + site) * 32 + (c & 31); }
+
+// H[i] = emb[occupation class] per site (zero for unused item slots)
+__global__ void __launch_bounds__(256) k_rg_embed(int LX, int LY, const int8_t* __restrict__ s,
+                                                  const int32_t* __restrict__ item_b,
+                                                  const int32_t* __restrict__ item_hop, int img0, int ipi,
+                                                  const float* __restrict__ emb, float* __restrict__ H) {
+  const int N = LX * LY;
+  const int i = blockIdx.y, img = img0 + i;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= N * kRcC) return;
+  const int site = e / kRcC, c = e % kRcC;
+  const int item = img / ipi;
+  const int grp = (ipi == kGroup) ? (img & 7) : 0;
+  const int32_t hop = item_hop ? item_hop[item] : kHopNone;
+  float v = 0.f;
+  if (hop != kHopInvalid) {
+    const int b = item_b ? item_b[item] : item;
+    const int8_t* sb = s + (size_t)b * 2 * N;
+    const int up = item_occ(sb, hop, group_map(site, grp, N, LX, LY)) > 0;
+    const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, LY)) > 0;
+    v = emb[(up + 2 * dn) * kRcC + c];
+  }
+  H[(size_t)i * N * kRcC + rg_at(N, site, c)] = v;
+}
+
+// rinv[i] = 1 / sqrt(mean of H[i]^2 over channels and sites + eps), one block per image
+__global__ void __launch_bounds__(256) k_rg_rinv(int N, const float* __restrict__ H, float* __restrict__ rinv) {
+  __shared__ float part[4];
+  const float* h = H + (size_t)blockIdx.x * N * kRcC;
+  float ss = 0.f;
+  for (int e = threadIdx.x; e < N * kRcC; e += 256) ss = fmaf(h[e], h[e], ss);
+  ss = rc_wave_sum(ss);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float tot = part[0] + part[1] + part[2] + part[3];
+    rinv[blockIdx.x] = 1.0f / sqrtf(fmaf(tot, 1.0f / (float)(kRcC * N), kRmsEps));
+  }
+}
+
+// A[i][site][c] = gelu(scale_i X[i]); X in the H layout (HL) or [site][c]
+template <bool HL>
+__global__ void __launch_bounds__(256) k_rg_act(int N, int nimg, const float* __restrict__ X,
+                                                const float* __restrict__ rinv, float* __restrict__ A) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)nimg * N * kRcC) return;
+  const int i = (int)(e / ((size_t)N * kRcC));
+  const int r = (int)(e - (size_t)i * N * kRcC);
+  const int site = r / kRcC, c = r % kRcC;
+  const float x = X[(size_t)i * N * kRcC + (HL ? rg_at(N, site, c) : r)];
+  A[e] = gelu_tanh(rinv ? rinv[i] * x : x);
+}
+
+constexpr int kRgSites = 16;  // sites per conv block: 4 groups of 4 sites x 64 output channels
+
+// Out = bias + periodic conv3x3 of A (W [tap][c_in][c_out], tap = (dx + 1) * 3 + dy + 1, input at
+// (x + dx, y + dy)) for the block's 16 sites of image blockIdx.y; RES: Out (H layout) += the
+// result in place, otherwise Out is [site][c].
+template <bool RES>
+__global__ void __launch_bounds__(256) k_rg_conv(int LX, int LY, const float* __restrict__ A,
+                                                 const float* __restrict__ W, const float* __restrict__ bias,
+                                                 float* __restrict__ Out) {
+  __shared__ float wl[kRcC * kRcC];
+  __shared__ float al[kRgSites][kRcC];
+  const int N = LX * LY;
+  const int i = blockIdx.y;
+  const int s0 = blockIdx.x * kRgSites;
+  const int co = threadIdx.x & (kRcC - 1), sg = threadIdx.x >> 6;
+  const float* Ai = A + (size_t)i * N * kRcC;
+  float acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = bias[co];
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dx = tap / 3 - 1, dy = tap % 3 - 1;
+    __syncthreads();  // the previous tap's reads of wl / al are done
+    for (int e = threadIdx.x; e < kRcC * kRcC; e += 256) wl[e] = W[tap * kRcC * kRcC + e];
+    for (int e = threadIdx.x; e < kRgSites * kRcC; e += 256) {
+      const int ls = e / kRcC, c = e % kRcC, site = s0 + ls;
+      float v = 0.f;
+      if (site < N) {
+        const int x = site / LY, y = site % LY;
+        const int xn = (x + dx + LX) % LX, yn = (y + dy + LY) % LY;
+        v = Ai[(xn * LY + yn) * kRcC + c];
+      }
+      al[ls][c] = v;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int ci = 0; ci < kRcC; ++ci) {
+      const float w = wl[ci * kRcC + co];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = fmaf(al[4 * sg + j][ci], w, acc[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int site = s0 + 4 * sg + j;
+    if (site >= N) continue;
+    if (RES) {
+      float* o = Out + (size_t)i * N * kRcC + rg_at(N, site, co);
+      *o += acc[j];
+    } else {
+      Out[(size_t)i * N * kRcC + site * kRcC + co] = acc[j];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_rg_scale(size_t n, float scale, float* __restrict__ H) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) H[e] *= scale;
+}
+
+bool resconv_tiled(const nh_model* m) {
+  return m->LY == kRcLY && m->LX >= 4 && m->LX <= 16 && m->LX % 4 == 0;
+}
+
 }  // namespace
 
 bool resconv_supported(const nh_model* m) {
-  return m->channels == kRcC && m->LY == kRcLY && m->LX >= 4 && m->LX <= 16 && m->LX % 4 == 0 && m->n_layers >= 1;
+  if (m->channels != kRcC || m->n_layers < 1) return false;
+  if (resconv_tiled(m)) return true;
+  return m->network == NH_NET_RESCONV;  // other lattices: no linear head
+}
+
+size_t resconv_scratch_floats(const nh_model* m, int64_t images) {
+  if ((m->network != NH_NET_RESCONV && m->network != NH_NET_RESCONV_HEAD) || resconv_tiled(m)) return 0;
+  return (size_t)images * (2 * (size_t)m->LX * m->LY * kRcC + 1);
+}
+
+// the layer-by-layer network of lattices outside the tile kernels
+static int launch_resconv_layers(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
+                                 const int32_t* item_hop, int img0, int img_end, int ipi, float* Y,
+                                 float* scratch) {
+  if (!scratch) return NH_ERR_UNSUPPORTED;  // nh_features carries no workspace
+  const int nimg = img_end - img0;
+  const int N = m->LX * m->LY;
+  const size_t per = (size_t)N * kRcC;
+  float* H = Y;  // [image][half][site][32] = the feature layout
+  float* A = scratch;
+  float* B = A + (size_t)nimg * per;
+  float* rinv = B + (size_t)nimg * per;
+  const float* emb = m->conv_w;
+  const float* Wc = m->conv_w + kRcClasses * kRcC;
+  const unsigned ge = (unsigned)((per + 255) / 256);
+  const unsigned ga = (unsigned)(((size_t)nimg * per + 255) / 256);
+  const dim3 gc((unsigned)((N + kRgSites - 1) / kRgSites), (unsigned)nimg);
+  hipLaunchKernelGGL(k_rg_embed, dim3(ge, nimg), dim3(256), 0, st, m->LX, m->LY, s, item_b, item_hop, img0, ipi,
+                     emb, H);
+  for (int blk = 0; blk < m->n_layers; ++blk) {
+    const float* W1 = Wc + (size_t)(2 * blk) * 9 * kRcC * kRcC;
+    hipLaunchKernelGGL(k_rg_rinv, dim3(nimg), dim3(256), 0, st, N, H, rinv);
+    hipLaunchKernelGGL(k_rg_act<true>, dim3(ga), dim3(256), 0, st, N, nimg, H, rinv, A);
+    hipLaunchKernelGGL(k_rg_conv<false>, gc, dim3(256), 0, st, m->LX, m->LY, A, W1, m->conv_b + (2 * blk) * kRcC, B);
+    hipLaunchKernelGGL(k_rg_act<false>, dim3(ga), dim3(256), 0, st, N, nimg, B, nullptr, A);
+    hipLaunchKernelGGL(k_rg_conv<true>, gc, dim3(256), 0, st, m->LX, m->LY, A, W1 + 9 * kRcC * kRcC,
+                       m->conv_b + (2 * blk + 1) * kRcC, H);
+  }
+  const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
+  hipLaunchKernelGGL(k_rg_scale, dim3(ga), dim3(256), 0, st, (size_t)nimg * per, scale, H);
+  NH_LAUNCH_CHECK();
+  return NH_OK;
 }
 
 int launch_resconv(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
-                   const int32_t* item_hop, int img0, int img_end, int ipi, float* Y) {
+                   const int32_t* item_hop, int img0, int img_end, int ipi, float* Y, float* scratch) {
   if (!resconv_supported(m)) return NH_ERR_UNSUPPORTED;
   const int nimg = img_end - img0;
   if (nimg <= 0) return NH_OK;
+  if (!resconv_tiled(m)) return launch_resconv_layers(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, scratch);
   const int N = m->LX * kRcLY;
   const size_t lds = sizeof(float) * ((size_t)N * kRcStride + 16);
   const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));

@@ -7,8 +7,9 @@ Reference interfaces mirrored here (ChenAo-Phys/NeuralHubbard):
   them with ``np.load(allow_pickle=False)``, so nothing in the file is executed.
 * ``weights/<name>_spins.npz`` (``code/states.py:328-352``): ``bits = np.packbits(spins > 0,
   axis=-1)`` and ``shape = (nchains, 2N)``; `unpack_spins` / `pack_spins` / `load_spins`.
-* the state table of ``code/states.py:99-128`` for the 16x16 backflow states this library
-  evaluates (`STATES`), with the projected E/N and sigma^2/N of the paper.
+* the state table of ``code/states.py:99-128`` for the backflow states this library evaluates
+  (`STATES`: the three 16x16 states and the 24x24 Pfaffian state), with the projected E/N and
+  sigma^2/N of the paper.
 
 `released_params` maps the leaves of a ``PfaffianBackflow`` (``code/backflow/models.py:121-151``:
 ``net`` = quantax ResConv with 16 blocks of 64 channels and a final bias, ``W`` (3, N, 32),
@@ -67,6 +68,8 @@ STATES = {
                   -0.75813, 1e-5, 0.0259, 0.0002),
         StateInfo("tpm02_L16_pfaffian", "pfaffian", 16, -0.2, "aef48e8b7c10cfa183f8c0e8dcf84d06", 1_208_320,
                   -0.74533, 1e-5, 0.0272, 0.0005),
+        StateInfo("tpm02_L24_pfaffian", "pfaffian", 24, -0.2, "17653481d380416e6df7033ba7902aee", 1_241_088,
+                  -0.74585, 1e-5, 0.0257, 0.0003),
     )
 }
 

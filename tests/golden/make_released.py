@@ -27,38 +27,43 @@ from neuralhubbard_amd import released as R  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 from oracle.cpu_port import CpuPath  # noqa: E402
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs")
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs", "tpm02_L24_pfaffian")
 CHAINS = 2048  # chains kept for the E/N check
 N_PSI = 8      # chains with oracle amplitudes
 N_FEAT = 2     # chains with oracle features
 N_ELOC = 2     # chains with oracle E_loc
+# the 24x24 state: fewer chains (its E/N check is short) and no oracle E_loc (one chain's ~2300
+# connected configurations x 8 images of a 1.4 GFLOP network take the C restatement minutes)
+SIZES = {"tpm02_L24_pfaffian": dict(chains=256, n_psi=2, n_feat=1, n_eloc=0)}
 
 
 def make(NAME, ref_root):
     wdir = os.path.join(ref_root, "weights")
     info = R.STATES[NAME]
+    sz = {**dict(chains=CHAINS, n_psi=N_PSI, n_feat=N_FEAT, n_eloc=N_ELOC), **SIZES.get(NAME, {})}
     wk, p = R.load_released(NAME, os.path.join(wdir, f"{NAME}.eqx"))
-    S = R.load_spins(os.path.join(wdir, f"{NAME}_spins.npz"), info)[:CHAINS]
+    S = R.load_spins(os.path.join(wdir, f"{NAME}_spins.npz"), info)[:sz["chains"]]
     m = O.Model(wk, p)
-    feats = m.features_batch(S[:N_FEAT])
+    feats = m.features_batch(S[:sz["n_feat"]])
     cp = CpuPath(wk, p, bits=64, threads=os.cpu_count())
-    sg, la = cp.psi(S[:N_PSI])
+    sg, la = cp.psi(S[:sz["n_psi"]])
     bare = CpuPath(wk, p, bits=64, project=False, threads=os.cpu_count())
-    bsg, bla = bare.psi(S[:N_PSI])
+    bsg, bla = bare.psi(S[:sz["n_psi"]])
     t0 = time.time()
-    el = cp.local_energy(S[:N_ELOC], sg[:N_ELOC], la[:N_ELOC])
-    print(f"oracle E_loc of {N_ELOC} chains: {el} ({time.time() - t0:.0f} s)", flush=True)
+    n_el = sz["n_eloc"]
+    el = cp.local_energy(S[:n_el], sg[:n_el], la[:n_el]) if n_el else np.zeros(0)
+    print(f"oracle E_loc of {n_el} chains: {el} ({time.time() - t0:.0f} s)", flush=True)
     extra = {}
-    if info.ansatz == "pfaffian":
+    if info.ansatz == "pfaffian" and n_el:
         # the same with the up and down embedding vectors exchanged (a plausible convention
         # mistake), to show that E/N against the paper discriminates it
         q = copy.copy(p)
         q.emb = np.ascontiguousarray(p.emb[:, [0, 2, 1, 3]])
         cq = CpuPath(wk, q, bits=64, threads=os.cpu_count())
-        qs, ql = cq.psi(S[:N_ELOC])
-        extra["eloc_swapped_embedding"] = cq.local_energy(S[:N_ELOC], qs, ql)
+        qs, ql = cq.psi(S[:n_el])
+        extra["eloc_swapped_embedding"] = cq.local_energy(S[:n_el], qs, ql)
         print(f"... with up/down embeddings exchanged: {extra['eloc_swapped_embedding']}", flush=True)
-    else:
+    elif info.ansatz == "bcs":
         extra["head"], extra["phi0"] = p.head, p.phi0
     out = os.path.join(HERE, f"{NAME}.npz")
     np.savez_compressed(

@@ -28,8 +28,9 @@ REF_WEIGHTS = "/root/reference/weights"
 NAME = "tpm02_L16_pfaffian"
 
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs")
-PAD = {"tpm02_L16_pfaffian": 1791, "tp0_L16_pfaffian": 1791, "tp0_L16_bcs": 1854}  # zero pads of the weight files
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs", "tpm02_L24_pfaffian")
+PAD = {"tpm02_L16_pfaffian": 1791, "tp0_L16_pfaffian": 1791, "tp0_L16_bcs": 1854,
+       "tpm02_L24_pfaffian": 3839}  # zero pads of the weight files
 
 
 def fixture(name=NAME):
@@ -42,11 +43,12 @@ def fixture(name=NAME):
 @pytest.mark.parametrize("name", NAMES)
 def test_fixture_parameter_count_and_chains(name):
     f, p = fixture(name)
-    assert R.n_params(p, pad=PAD[name]) == R.STATES[name].nparams == 1_208_320
+    info = R.STATES[name]
+    assert R.n_params(p, pad=PAD[name]) == info.nparams
     S = R.unpack_spins(f["bits"], f["shape"])
-    N = R.STATES[name].N
-    assert S.shape == (2048, 2 * N) and S.dtype == np.int8
-    assert np.all((S[:, :N] > 0).sum(1) == 112) and np.all((S[:, N:] > 0).sum(1) == 112)
+    N = info.N
+    assert S.shape == (2048 if info.L == 16 else 256, 2 * N) and S.dtype == np.int8
+    assert np.all((S[:, :N] > 0).sum(1) == info.n_up) and np.all((S[:, N:] > 0).sum(1) == info.n_up)
     rec = R.pack_spins(S)
     np.testing.assert_array_equal(rec["bits"], f["bits"])
 
