@@ -916,8 +916,9 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 // ------------------------------------------------------------------ BCS image kernel
 
 // GM (bcs_global_matrix): the image's n x (n + 1) matrix sits in the global workspace gmat at
-// (img - img0) * gstride floats (followed by kWave dummy floats); LDS holds the features of the
-// occupied modes (then the elimination scratch), the mode list and a flag.
+// (img - img0) * gstride floats (followed by kWave dummy floats); LDS holds the LU scratch, the
+// features of the down modes (each row's up-mode features are read from Y), the mode list and a
+// flag.
 template <int D, int NT, bool GM>
 __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_up, int n_dn,
                                                   const int8_t* __restrict__ s,
@@ -943,15 +944,20 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   const int ld = n + 1;
 
   float* M;
-  float* yu;
+  float* yu;   // staged features of the up modes (LDS form; GM reads them from Y)
+  float* scr;  // blocked-LU scratch
+  float* yd;
   if constexpr (GM) {
     M = gmat + (size_t)(img - img0) * gstride;
-    yu = lds;
+    yu = nullptr;
+    scr = lds;
+    yd = lds + ((lu_scratch_words(n) + 3) & ~3);
   } else {
     M = lds;
     yu = M + ((n * ld + kWave + 3) & ~3);
+    scr = yu;  // the features are dead once the matrix is built
+    yd = yu + n * D;
   }
-  float* yd = yu + n * D;
   float* lcol = yd + n * D;
   float* urow = lcol + n;
   int* modes = reinterpret_cast<int*>(urow + n);
@@ -977,7 +983,8 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   const float* Yi = Y + (size_t)(img - img0) * (two_n * D);
   for (int e = t; e < n * (D / 4); e += NT) {
     const int p = e / (D / 4), a4 = e - p * (D / 4);
-    reinterpret_cast<float4*>(yu)[e] = reinterpret_cast<const float4*>(Yi + (size_t)rows[p] * D)[a4];
+    if constexpr (!GM)
+      reinterpret_cast<float4*>(yu)[e] = reinterpret_cast<const float4*>(Yi + (size_t)rows[p] * D)[a4];
     reinterpret_cast<float4*>(yd)[e] = reinterpret_cast<const float4*>(Yi + (size_t)cols[p] * D)[a4];
   }
   __syncthreads();
@@ -985,7 +992,7 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   for (int p = wave; p < n; p += NW) {
     float yp[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) yp[a] = yu[p * D + a];
+    for (int a = 0; a < D; ++a) yp[a] = GM ? Yi[(size_t)rows[p] * D + a] : yu[p * D + a];
     const int sitep = rows[p];
     for (int q = lane; q < n; q += kWave) {
       const int siteq = cols[q] - N;
@@ -1011,9 +1018,9 @@ __global__ void __launch_bounds__(NT) k_image_bcs(int N, int LX, int LY, int n_u
   float sg;
   double la;
   if constexpr (GM)
-    lu_blocked<NT, kLuPanel, 9>(M, n, ld, reinterpret_cast<uint32_t*>(yu), sg, la);  // features are dead
+    lu_blocked<NT, kLuPanel, 9>(M, n, ld, reinterpret_cast<uint32_t*>(scr), sg, la);
   else if constexpr (NT > kWave)
-    lu_blocked<NT, kLuPanel>(M, n, ld, reinterpret_cast<uint32_t*>(yu), sg, la);
+    lu_blocked<NT, kLuPanel>(M, n, ld, reinterpret_cast<uint32_t*>(scr), sg, la);
   else
     lu_eliminate<NT>(M, n, ld, lcol, urow, flag, sg, la);
   if (t == 0) {
@@ -1034,8 +1041,10 @@ static size_t pf_lds_bytes(int n, int N, int D) {
 }
 
 static size_t bcs_lds_bytes(int n, int ne, int D, bool gm) {
-  const size_t mat = gm ? 0 : (size_t)((n * (n + 1) + kWave + 3) & ~3);
-  return sizeof(float) * (mat + (size_t)(2 * n * D + 2 * n)) + sizeof(int) * (size_t)(ne + 4);
+  // GM: LU scratch + the down modes' features; otherwise the matrix + both spins' features
+  const size_t front = gm ? (size_t)((lu_scratch_words(n) + 3) & ~3) + (size_t)n * D
+                          : (size_t)((n * (n + 1) + kWave + 3) & ~3) + (size_t)2 * n * D;
+  return sizeof(float) * (front + (size_t)(2 * n)) + sizeof(int) * (size_t)(ne + 4);
 }
 
 // BCS determinant matrix in the global workspace rather than in LDS: above n = 192, and where the

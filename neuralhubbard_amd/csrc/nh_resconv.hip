@@ -810,6 +810,25 @@ __global__ void __launch_bounds__(256) k_rg_conv(int LX, int LY, const float* __
   }
 }
 
+// BCS linear head: Y[i][k][site][dd] = sum_c head[k][c][dd] scale H[i][c at site] (head [k][c][dd])
+__global__ void __launch_bounds__(256) k_rg_head(int N, float scale, const float* __restrict__ H,
+                                                 const float* __restrict__ head, float* __restrict__ Y) {
+  __shared__ float xs[4][kRcC];
+  const int i = blockIdx.y;
+  const int ls = threadIdx.x >> 6, dd = threadIdx.x & (kRcC - 1);
+  const int site = blockIdx.x * 4 + ls;
+  xs[ls][dd] = site < N ? scale * H[(size_t)i * N * kRcC + rg_at(N, site, dd)] : 0.f;
+  __syncthreads();
+  if (site >= N) return;
+  for (int k = 0; k < 2; ++k) {
+    const float* hk = head + (size_t)k * kRcC * kRcC + dd;
+    float acc = 0.f;
+#pragma unroll 8
+    for (int c = 0; c < kRcC; ++c) acc = fmaf(hk[c * kRcC], xs[ls][c], acc);
+    Y[(size_t)i * 2 * N * kRcC + ((size_t)k * N + site) * kRcC + dd] = acc;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_rg_scale(size_t n, float scale, float* __restrict__ H) {
   const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (e < n) H[e] *= scale;
@@ -822,14 +841,15 @@ bool resconv_tiled(const nh_model* m) {
 }  // namespace
 
 bool resconv_supported(const nh_model* m) {
-  if (m->channels != kRcC || m->n_layers < 1) return false;
-  if (resconv_tiled(m)) return true;
-  return m->network == NH_NET_RESCONV;  // other lattices: no linear head
+  return m->channels == kRcC && m->n_layers >= 1;
 }
 
 size_t resconv_scratch_floats(const nh_model* m, int64_t images) {
   if ((m->network != NH_NET_RESCONV && m->network != NH_NET_RESCONV_HEAD) || resconv_tiled(m)) return 0;
-  return (size_t)images * (2 * (size_t)m->LX * m->LY * kRcC + 1);
+  // two activation buffers (and, with the BCS head, the residual stream: Y then holds 2 x 64
+  // features per site) and one float per image
+  const size_t bufs = m->network == NH_NET_RESCONV_HEAD ? 3 : 2;
+  return (size_t)images * (bufs * (size_t)m->LX * m->LY * kRcC + 1);
 }
 
 // the layer-by-layer network of lattices outside the tile kernels
@@ -840,10 +860,11 @@ static int launch_resconv_layers(hipStream_t st, const nh_model* m, const int8_t
   const int nimg = img_end - img0;
   const int N = m->LX * m->LY;
   const size_t per = (size_t)N * kRcC;
-  float* H = Y;  // [image][half][site][32] = the feature layout
+  const bool head = m->network == NH_NET_RESCONV_HEAD;
   float* A = scratch;
   float* B = A + (size_t)nimg * per;
-  float* rinv = B + (size_t)nimg * per;
+  float* H = head ? B + (size_t)nimg * per : Y;  // without the head: [image][half][site][32] = Y's layout
+  float* rinv = (head ? H : B) + (size_t)nimg * per;
   const float* emb = m->conv_w;
   const float* Wc = m->conv_w + kRcClasses * kRcC;
   const unsigned ge = (unsigned)((per + 255) / 256);
@@ -861,7 +882,12 @@ static int launch_resconv_layers(hipStream_t st, const nh_model* m, const int8_t
                        m->conv_b + (2 * blk + 1) * kRcC, H);
   }
   const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
-  hipLaunchKernelGGL(k_rg_scale, dim3(ga), dim3(256), 0, st, (size_t)nimg * per, scale, H);
+  if (head) {
+    const float* hw = m->conv_w + kRcClasses * kRcC + (size_t)2 * m->n_layers * 9 * kRcC * kRcC;
+    hipLaunchKernelGGL(k_rg_head, dim3((unsigned)((N + 3) / 4), (unsigned)nimg), dim3(256), 0, st, N, scale, H, hw, Y);
+  } else {
+    hipLaunchKernelGGL(k_rg_scale, dim3(ga), dim3(256), 0, st, (size_t)nimg * per, scale, H);
+  }
   NH_LAUNCH_CHECK();
   return NH_OK;
 }

@@ -8,8 +8,8 @@ Reference interfaces mirrored here (ChenAo-Phys/NeuralHubbard):
 * ``weights/<name>_spins.npz`` (``code/states.py:328-352``): ``bits = np.packbits(spins > 0,
   axis=-1)`` and ``shape = (nchains, 2N)``; `unpack_spins` / `pack_spins` / `load_spins`.
 * the state table of ``code/states.py:99-128`` for the backflow states this library evaluates
-  (`STATES`: the three 16x16 states and the 24x24 Pfaffian state), with the projected E/N and
-  sigma^2/N of the paper.
+  (`STATES`: the three 16x16 states, the 24x24 Pfaffian state and the 32x32 BCS state), with the
+  projected E/N and sigma^2/N of the paper.
 
 `released_params` maps the leaves of a ``PfaffianBackflow`` (``code/backflow/models.py:121-151``:
 ``net`` = quantax ResConv with 16 blocks of 64 channels and a final bias, ``W`` (3, N, 32),
@@ -70,6 +70,8 @@ STATES = {
                   -0.74533, 1e-5, 0.0272, 0.0005),
         StateInfo("tpm02_L24_pfaffian", "pfaffian", 24, -0.2, "17653481d380416e6df7033ba7902aee", 1_241_088,
                   -0.74585, 1e-5, 0.0257, 0.0003),
+        StateInfo("tp0_L32_bcs", "bcs", 32, 0.0, "e45a997c3202fae9df7c4f38bd7e6acc", 1_257_472,
+                  -0.75617, 1e-5, 0.046, 0.010),
     )
 }
 

@@ -27,14 +27,16 @@ from neuralhubbard_amd import released as R  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 from oracle.cpu_port import CpuPath  # noqa: E402
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs", "tpm02_L24_pfaffian")
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs", "tpm02_L24_pfaffian", "tp0_L32_bcs")
 CHAINS = 2048  # chains kept for the E/N check
 N_PSI = 8      # chains with oracle amplitudes
 N_FEAT = 2     # chains with oracle features
 N_ELOC = 2     # chains with oracle E_loc
-# the 24x24 state: fewer chains (its E/N check is short) and no oracle E_loc (one chain's ~2300
-# connected configurations x 8 images of a 1.4 GFLOP network take the C restatement minutes)
-SIZES = {"tpm02_L24_pfaffian": dict(chains=256, n_psi=2, n_feat=1, n_eloc=0)}
+# the 24x24 and 32x32 states: fewer chains (their E/N checks are short) and no oracle E_loc (one
+# chain's ~2000 connected configurations x 8 images of a 1.4-2.4 GFLOP network take the C
+# restatement many minutes)
+SIZES = {"tpm02_L24_pfaffian": dict(chains=256, n_psi=2, n_feat=1, n_eloc=0),
+         "tp0_L32_bcs": dict(chains=64, n_psi=2, n_feat=1, n_eloc=0)}
 
 
 def make(NAME, ref_root):

@@ -151,7 +151,7 @@ def _released(name="tpm02_L16_pfaffian"):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
-@pytest.mark.parametrize("name", NAMES + ("tpm02_L24_pfaffian",))
+@pytest.mark.parametrize("name", NAMES + ("tpm02_L24_pfaffian", "tp0_L32_bcs"))
 def test_released_state_matches_oracle(name, prec):
     from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow
 
@@ -219,6 +219,28 @@ def test_released_24x24_energy_per_site():
         s = torch.tensor(S[b0:b0 + 8], device="cuda")
         el.append(H.Oloc(state, Samples(s, state(s))).cpu().numpy())
         print(f"24x24: E_loc of chains {b0}..{b0 + 8}", flush=True)
+    e = np.concatenate(el) / wk.N
+    assert np.all(np.isfinite(e))
+    mean, err = e.mean(), e.std(ddof=1) / np.sqrt(len(e))
+    print(f"E/N = {mean:.5f} +- {err:.5f} (paper {info.E_per_site})", flush=True)
+    assert abs(mean - info.E_per_site) < 4 * err + info.E_err
+
+
+def test_released_32x32_bcs_energy_per_site():
+    """The released tp0_L32_bcs state (32x32, 448 + 448 electrons, 448 x 448 determinants in the
+    workspace, the residual CNN and its linear head layer by layer): E/N over the first 16
+    fixture chains against the paper's -0.75617(1) (code/states.py:110-111), four standard
+    errors of the chain mean."""
+    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow
+
+    info, wk, p, S, _ = _released("tp0_L32_bcs")
+    state = Variational(build_backflow(wk, p, "fp32"))
+    H = Hubbard(U=wk.U, tprime=wk.tprime, chunk=4)
+    el = []
+    for b0 in range(0, 16, 4):
+        s = torch.tensor(S[b0:b0 + 4], device="cuda")
+        el.append(H.Oloc(state, Samples(s, state(s))).cpu().numpy())
+        print(f"32x32 BCS: E_loc of chains {b0}..{b0 + 4}", flush=True)
     e = np.concatenate(el) / wk.N
     assert np.all(np.isfinite(e))
     mean, err = e.mean(), e.std(ddof=1) / np.sqrt(len(e))

@@ -55,9 +55,10 @@ def test_residual_cnn_model_checks():
         assert lib.nh_log_psi_workspace_size(ctypes.byref(other), 64) > 64 * 8 * 3 * N * 64 * 4
         assert lib.nh_features_workspace_size(ctypes.byref(other), 64) >= 64 * (2 * N * 64 + 1) * 4
     assert lib.nh_features_workspace_size(ctypes.byref(ok), 64) == 256  # tile kernels: none needed
-    H = _lib.NH_NET_RESCONV_HEAD
+    # the BCS head off the tiles keeps the residual stream in the workspace too
+    bcs32 = _lib.NhModel(1, 32, 32, 448, 448, 64, 16, 0, 1, _lib.NH_NET_RESCONV_HEAD, 16, 16, 16, 0, 0)
+    assert lib.nh_features_workspace_size(ctypes.byref(bcs32), 8) >= 8 * (3 * 1024 * 64 + 1) * 4
     for bad in (_lib.NhModel(0, 16, 16, 112, 112, 32, 16, 0, 1, R, 16, 16, 16, 0, 0),   # C != 64
-                _lib.NhModel(1, 24, 24, 252, 252, 64, 16, 0, 1, H, 16, 16, 16, 0, 0),   # head off the tiles
                 _lib.NhModel(0, 40, 40, 700, 700, 64, 16, 0, 1, R, 16, 16, 16, 0, 0),   # N > 1024
                 _lib.NhModel(0, 16, 16, 112, 112, 64, 0, 0, 1, R, 16, 16, 16, 0, 0),    # no blocks
                 _lib.NhModel(0, 16, 16, 112, 112, 64, 16, 1, 1, R, 16, 16, 16, 0, 0),    # mixed, no BF16 copy

@@ -28,9 +28,10 @@ REF_WEIGHTS = "/root/reference/weights"
 NAME = "tpm02_L16_pfaffian"
 
 
-NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs", "tpm02_L24_pfaffian")
+NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs", "tpm02_L24_pfaffian", "tp0_L32_bcs")
 PAD = {"tpm02_L16_pfaffian": 1791, "tp0_L16_pfaffian": 1791, "tp0_L16_bcs": 1854,
-       "tpm02_L24_pfaffian": 3839}  # zero pads of the weight files
+       "tpm02_L24_pfaffian": 3839, "tp0_L32_bcs": 1854}  # zero pads of the weight files
+CHAINS = {16: 2048, 24: 256, 32: 64}  # released chains kept in the fixtures
 
 
 def fixture(name=NAME):
@@ -47,7 +48,7 @@ def test_fixture_parameter_count_and_chains(name):
     assert R.n_params(p, pad=PAD[name]) == info.nparams
     S = R.unpack_spins(f["bits"], f["shape"])
     N = info.N
-    assert S.shape == (2048 if info.L == 16 else 256, 2 * N) and S.dtype == np.int8
+    assert S.shape == (CHAINS[info.L], 2 * N) and S.dtype == np.int8
     assert np.all((S[:, :N] > 0).sum(1) == info.n_up) and np.all((S[:, N:] > 0).sum(1) == info.n_up)
     rec = R.pack_spins(S)
     np.testing.assert_array_equal(rec["bits"], f["bits"])
