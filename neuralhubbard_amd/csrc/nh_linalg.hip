@@ -1062,7 +1062,9 @@ int64_t image_chunk(const nh_model* m) {
     return !bcs_global_matrix(m) ? 32768 : (m->n_up <= 256 ? 8192 : 1024);  // 0.8 MB per image at n = 448
   const int n = m->n_up + m->n_dn;
   if (n <= kLdsTriMaxN) return 32768;
-  return n <= 256 ? 8192 : 1024;  // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB)
+  // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB); at n = 224, 32768 images
+  // per launch (4.3 GB of workspace) give 5 % more E_loc throughput than 8192
+  return n <= 256 ? 32768 : 1024;
 }
 
 size_t image_global_floats(const nh_model* m) {
