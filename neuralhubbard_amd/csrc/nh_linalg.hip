@@ -1059,7 +1059,7 @@ static bool bcs_global_matrix(const nh_model* m) {
 
 int64_t image_chunk(const nh_model* m) {
   if (m->kind == NH_KIND_BCS)
-    return !bcs_global_matrix(m) ? 32768 : (m->n_up <= 256 ? 8192 : 1024);  // 0.8 MB per image at n = 448
+    return !bcs_global_matrix(m) || m->n_up <= 256 ? 32768 : 1024;  // 0.8 MB per image at n = 448
   const int n = m->n_up + m->n_dn;
   if (n <= kLdsTriMaxN) return 32768;
   // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB); at n = 224, 32768 images
