@@ -125,6 +125,14 @@ def test_argument_errors_without_gpu():
     assert lib.nh_slogpf(None, 4, 7, None, None, None) == _lib.NH_ERR_ARG
     assert lib.nh_slogpf(None, 1, 300, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16)) == _lib.NH_ERR_SHAPE
     assert lib.nh_slogdet(None, 1, 500, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16)) == _lib.NH_ERR_SHAPE
+    # the layer-by-layer network needs its workspace: too small a one is refused before any launch
+    R = _lib.NH_NET_RESCONV
+    big = _lib.NhModel(0, 24, 24, 252, 252, 64, 16, 0, 1, R, 16, 16, 16, 0, 0)
+    p = ctypes.c_void_p(256)
+    need = lib.nh_features_workspace_size(ctypes.byref(big), 2)
+    assert need >= 2 * (2 * 576 * 64 + 1) * 4
+    assert lib.nh_features_ws(None, ctypes.byref(big), 2, p, p, p, need // 2) == _lib.NH_ERR_WORKSPACE
+    assert lib.nh_features_ws(None, ctypes.byref(big), 2, p, p, None, need) == _lib.NH_ERR_WORKSPACE
 
 
 def test_host_tables_match_oracle():
