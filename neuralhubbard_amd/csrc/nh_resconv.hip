@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
 // ---------------------------------------------------------------- other lattices
 //
 // Lattices outside the tile kernels (LY != 16, or more than 16 rows) run the network layer by
-// layer over a chunk of images, FP32 on the VALU, with the activations in the workspace.  The
+// layer over a chunk of images, FP32 (MFMA convolutions), with the activations in the workspace.  The
 // residual stream H of image i is kept as [half][site][32] (channel c at half c / 32), the layout
 // of the features y[spin][site][a], so that the scaled output lands in Y in place; A and B are
 // two [site][channel] activation buffers of the same size and rinv one float per image.
@@ -757,29 +757,40 @@ __global__ void __launch_bounds__(256) k_rg_act(int N, int nimg, const float* __
 }
 
 // Out = bias + periodic conv3x3 of A (W [tap][c_in][c_out], tap = (dx + 1) * 3 + dy + 1, input at
-// (x + dx, y + dy)) for the block's 4 SPT sites of image blockIdx.y (4 groups of SPT sites x 64
-// output channels, one output per thread and site); RES: Out (H layout) += the result in place,
-// otherwise Out is [site][c].
-template <bool RES, int SPT>
-__global__ void __launch_bounds__(256) k_rg_conv(int LX, int LY, const float* __restrict__ A,
-                                                 const float* __restrict__ W, const float* __restrict__ bias,
-                                                 float* __restrict__ Out) {
-  constexpr int S = 4 * SPT;
-  __shared__ float wl[kRcC * kRcC];
-  __shared__ float al[S][kRcC];
+// (x + dx, y + dy)); RES: Out (H layout) += the result in place, otherwise Out is [site][c].
+// FP32 MFMA (v_mfma_f32_16x16x4_f32): a block owns 32 sites (two M tiles of 16) x 64 output
+// channels (four N tiles) of image blockIdx.y, one tap's weights and neighbour activations staged
+// in LDS; wave w computes tiles (M = w / 2, N = 2 (w % 2) + {0,1})
+// over K = 9 taps x 64 input channels.  A fragment: lane l holds site 16 M + l % 16, channel
+// 4 kk + l / 16 of the tap's neighbour rows; B fragment: lane l holds W[tap][4 kk + l / 16]
+// [16 N + l % 16]; result lane l, register r -> site 16 M + 4 (l / 16) + r, channel 16 N + l % 16.
+// LDS rows are padded by 4 floats so that both fragment reads are conflict-free.
+constexpr int kRgPad = kRcC + 4;
+
+template <bool RES>
+__global__ void __launch_bounds__(256) k_rg_conv_mfma(int LX, int LY, const float* __restrict__ A,
+                                                      const float* __restrict__ W, const float* __restrict__ bias,
+                                                      float* __restrict__ Out) {
+  __shared__ float wl[kRcC * kRgPad];
+  __shared__ float al[32 * kRgPad];
   const int N = LX * LY;
   const int i = blockIdx.y;
-  const int s0 = blockIdx.x * S;
-  const int co = threadIdx.x & (kRcC - 1), sg = threadIdx.x >> 6;
+  const int s0 = blockIdx.x * 32;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int mt = wave >> 1, nt0 = 2 * (wave & 1);
+  const int m = lane & 15, k = lane >> 4;
   const float* Ai = A + (size_t)i * N * kRcC;
-  float acc[SPT];
+  rc_f32x4 acc[2];
 #pragma unroll
-  for (int j = 0; j < SPT; ++j) acc[j] = bias[co];
+  for (int q = 0; q < 2; ++q) {
+    const float bv = bias[16 * (nt0 + q) + m];
+    acc[q] = rc_f32x4{bv, bv, bv, bv};
+  }
   for (int tap = 0; tap < 9; ++tap) {
     const int dx = tap / 3 - 1, dy = tap % 3 - 1;
     __syncthreads();  // the previous tap's reads of wl / al are done
-    for (int e = threadIdx.x; e < kRcC * kRcC; e += 256) wl[e] = W[tap * kRcC * kRcC + e];
-    for (int e = threadIdx.x; e < S * kRcC; e += 256) {
+    for (int e = threadIdx.x; e < kRcC * kRcC; e += 256) wl[(e / kRcC) * kRgPad + e % kRcC] = W[tap * kRcC * kRcC + e];
+    for (int e = threadIdx.x; e < 32 * kRcC; e += 256) {
       const int ls = e / kRcC, c = e % kRcC, site = s0 + ls;
       float v = 0.f;
       if (site < N) {
@@ -787,25 +798,32 @@ __global__ void __launch_bounds__(256) k_rg_conv(int LX, int LY, const float* __
         const int xn = (x + dx + LX) % LX, yn = (y + dy + LY) % LY;
         v = Ai[(xn * LY + yn) * kRcC + c];
       }
-      al[ls][c] = v;
+      al[ls * kRgPad + c] = v;
     }
     __syncthreads();
-#pragma unroll 8
-    for (int ci = 0; ci < kRcC; ++ci) {
-      const float w = wl[ci * kRcC + co];
+#pragma unroll 4
+    for (int kk = 0; kk < kRcC / 4; ++kk) {
+      const float a = al[(16 * mt + m) * kRgPad + 4 * kk + k];
 #pragma unroll
-      for (int j = 0; j < SPT; ++j) acc[j] = fmaf(al[SPT * sg + j][ci], w, acc[j]);
+      for (int q = 0; q < 2; ++q) {
+        const float b = wl[(4 * kk + k) * kRgPad + 16 * (nt0 + q) + m];
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+      }
     }
   }
 #pragma unroll
-  for (int j = 0; j < SPT; ++j) {
-    const int site = s0 + SPT * sg + j;
-    if (site >= N) continue;
-    if (RES) {
-      float* o = Out + (size_t)i * N * kRcC + rg_at(N, site, co);
-      *o += acc[j];
-    } else {
-      Out[(size_t)i * N * kRcC + site * kRcC + co] = acc[j];
+  for (int q = 0; q < 2; ++q) {
+    const int co = 16 * (nt0 + q) + m;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int site = s0 + 16 * mt + 4 * k + r;
+      if (site >= N) continue;
+      if (RES) {
+        float* o = Out + (size_t)i * N * kRcC + rg_at(N, site, co);
+        *o += acc[q][r];
+      } else {
+        Out[(size_t)i * N * kRcC + site * kRcC + co] = acc[q][r];
+      }
     }
   }
 }
@@ -869,19 +887,19 @@ static int launch_resconv_layers(hipStream_t st, const nh_model* m, const int8_t
   const float* Wc = m->conv_w + kRcClasses * kRcC;
   const unsigned ge = (unsigned)((per + 255) / 256);
   const unsigned ga = (unsigned)(((size_t)nimg * per + 255) / 256);
-  // 8 sites per thread (32 per block): 47.6 us per 24x24 image against 52.8 (4) and 62.2 (16)
-  constexpr int kSpt = 8;
-  const dim3 gc((unsigned)((N + 4 * kSpt - 1) / (4 * kSpt)), (unsigned)nimg);
+  // 32 sites per block on FP32 MFMA: 43.4 us per 24x24 image (VALU FMA: 48 us with 32 sites per
+  // block, 52.8 with 16, 62.2 with 64)
+  const dim3 gc((unsigned)((N + 31) / 32), (unsigned)nimg);
   hipLaunchKernelGGL(k_rg_embed, dim3(ge, nimg), dim3(256), 0, st, m->LX, m->LY, s, item_b, item_hop, img0, ipi,
                      emb, H);
   for (int blk = 0; blk < m->n_layers; ++blk) {
     const float* W1 = Wc + (size_t)(2 * blk) * 9 * kRcC * kRcC;
     hipLaunchKernelGGL(k_rg_rinv, dim3(nimg), dim3(256), 0, st, N, H, rinv);
     hipLaunchKernelGGL(k_rg_act<true>, dim3(ga), dim3(256), 0, st, N, nimg, H, rinv, A);
-    hipLaunchKernelGGL((k_rg_conv<false, kSpt>), gc, dim3(256), 0, st, m->LX, m->LY, A, W1,
+    hipLaunchKernelGGL((k_rg_conv_mfma<false>), gc, dim3(256), 0, st, m->LX, m->LY, A, W1,
                        m->conv_b + (2 * blk) * kRcC, B);
     hipLaunchKernelGGL(k_rg_act<false>, dim3(ga), dim3(256), 0, st, N, nimg, B, nullptr, A);
-    hipLaunchKernelGGL((k_rg_conv<true, kSpt>), gc, dim3(256), 0, st, m->LX, m->LY, A, W1 + 9 * kRcC * kRcC,
+    hipLaunchKernelGGL((k_rg_conv_mfma<true>), gc, dim3(256), 0, st, m->LX, m->LY, A, W1 + 9 * kRcC * kRcC,
                        m->conv_b + (2 * blk + 1) * kRcC, H);
   }
   const float scale = (float)(1.0 / sqrt((double)m->n_layers + 1.0));
