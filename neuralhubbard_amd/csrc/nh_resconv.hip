@@ -828,6 +828,99 @@ __global__ void __launch_bounds__(256) k_rg_conv_mfma(int LX, int LY, const floa
   }
 }
 
+// Split-BF16 form of the same conv (the mixed setting): activations as BF16 hi and lo planes
+// [image][plane][site][64], weights the split-BF16 A fragments of the tile kernel's slabs
+// (conv_w_lp, [conv][kh][dyi][plane][dxi][qq][lane][8]).  A block owns 32 sites (two N tiles of
+// 16) x 64 output channels; wave w computes quarter w (M tile) for both site tiles with
+// v_mfma_f32_16x16x32_bf16, lo*hi + hi*lo + hi*hi per product.  B fragment: lane l holds the 8
+// channels 32 kh + 8 (l / 16) .. + 7 of site 16 nt + l % 16 (one 16-byte LDS read per plane);
+// result lane l, register r -> channel 16 w + 4 (l / 16) + r, site 16 nt + l % 16.
+
+// A16[i] = split BF16 of gelu(scale_i X[i]); X in the H layout (HL) or [site][c]
+template <bool HL>
+__global__ void __launch_bounds__(256) k_rg_act_bf16(int N, int nimg, const float* __restrict__ X,
+                                                     const float* __restrict__ rinv, uint16_t* __restrict__ A16) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)nimg * N * kRcC) return;
+  const int i = (int)(e / ((size_t)N * kRcC));
+  const int r = (int)(e - (size_t)i * N * kRcC);
+  const int site = r / kRcC, c = r % kRcC;
+  const float x = X[(size_t)i * N * kRcC + (HL ? rg_at(N, site, c) : r)];
+  const float g = rb_gelu(rinv ? rinv[i] * x : x);
+  const uint32_t hi = rb_bf16(g);
+  const uint32_t lo = rb_bf16(g - __uint_as_float(hi << 16));
+  uint16_t* a = A16 + (size_t)i * 2 * N * kRcC;
+  a[r] = (uint16_t)hi;
+  a[(size_t)N * kRcC + r] = (uint16_t)lo;
+}
+
+template <bool RES>
+__global__ void __launch_bounds__(256) k_rg_conv_bf16(int LX, int LY, const uint16_t* __restrict__ A16,
+                                                      const rb_i32x4* __restrict__ Wl, const float* __restrict__ bias,
+                                                      float* __restrict__ Out) {
+  __shared__ uint16_t ah[32 * kRbStride];
+  __shared__ uint16_t alo[32 * kRbStride];
+  const int N = LX * LY;
+  const int i = blockIdx.y;
+  const int s0 = blockIdx.x * 32;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int j16 = lane & 15, g16 = lane >> 4;
+  const uint16_t* Ai = A16 + (size_t)i * 2 * N * kRcC;
+  rc_f32x4 acc[2];
+  {
+    const float4 b4 = *reinterpret_cast<const float4*>(bias + 16 * w + 4 * g16);
+    acc[0] = acc[1] = rc_f32x4{b4.x, b4.y, b4.z, b4.w};
+  }
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dxi = tap / 3, dyi = tap % 3;
+    const int dx = dxi - 1, dy = dyi - 1;
+    __syncthreads();  // the previous tap's reads of ah / alo are done
+    // 32 sites x 8 chunks of 8 channels x 2 planes, 16 bytes each: two per thread
+    for (int e = threadIdx.x; e < 32 * 8 * 2; e += 256) {
+      const int pl = e >> 8, ls = (e >> 3) & 31, ch = e & 7, site = s0 + ls;
+      int4 v = make_int4(0, 0, 0, 0);
+      if (site < N) {
+        const int x = site / LY, y = site % LY;
+        const int xn = (x + dx + LX) % LX, yn = (y + dy + LY) % LY;
+        v = *reinterpret_cast<const int4*>(Ai + (size_t)pl * N * kRcC + (xn * LY + yn) * kRcC + 8 * ch);
+      }
+      *reinterpret_cast<int4*>((pl ? alo : ah) + ls * kRbStride + 8 * ch) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh) {
+      // A fragments of (kh, dyi, dxi, quarter w), hi and lo planes
+      const rb_i32x4* base = Wl + ((size_t)((kh * 3 + dyi) * 2) * 3 * 4) * kWave;
+      const rb_bf16x8 wh = __builtin_bit_cast(rb_bf16x8, base[((0 * 3 + dxi) * 4 + w) * kWave + lane]);
+      const rb_bf16x8 wlo = __builtin_bit_cast(rb_bf16x8, base[((1 * 3 + dxi) * 4 + w) * kWave + lane]);
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        const int off = (16 * nt + j16) * kRbStride + 32 * kh + 8 * g16;
+        const rb_bf16x8 bh = __builtin_bit_cast(rb_bf16x8, *reinterpret_cast<const int4*>(ah + off));
+        const rb_bf16x8 bl = __builtin_bit_cast(rb_bf16x8, *reinterpret_cast<const int4*>(alo + off));
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlo, bh, acc[nt], 0, 0, 0);
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bl, acc[nt], 0, 0, 0);
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bh, acc[nt], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    const int site = s0 + 16 * nt + j16;
+    if (site >= N) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = 16 * w + 4 * g16 + r;
+      if (RES) {
+        float* o = Out + (size_t)i * N * kRcC + rg_at(N, site, co);
+        *o += acc[nt][r];
+      } else {
+        Out[(size_t)i * N * kRcC + site * kRcC + co] = acc[nt][r];
+      }
+    }
+  }
+}
+
 // BCS linear head: Y[i][k][site][dd] = sum_c head[k][c][dd] scale H[i][c at site] (head [k][c][dd])
 __global__ void __launch_bounds__(256) k_rg_head(int N, float scale, const float* __restrict__ H,
                                                  const float* __restrict__ head, float* __restrict__ Y) {
@@ -892,9 +985,23 @@ static int launch_resconv_layers(hipStream_t st, const nh_model* m, const int8_t
   const dim3 gc((unsigned)((N + 31) / 32), (unsigned)nimg);
   hipLaunchKernelGGL(k_rg_embed, dim3(ge, nimg), dim3(256), 0, st, m->LX, m->LY, s, item_b, item_hop, img0, ipi,
                      emb, H);
+  // the mixed setting: split-BF16 convolutions on the BF16 slabs of the tile kernel
+  const bool bf16 = m->precision == NH_PREC_MIXED && m->conv_w_lp;
+  const rb_i32x4* Wlp = reinterpret_cast<const rb_i32x4*>(m->conv_w_lp);
+  uint16_t* A16 = reinterpret_cast<uint16_t*>(A);  // hi and lo planes fill the FP32 buffer's bytes
   for (int blk = 0; blk < m->n_layers; ++blk) {
     const float* W1 = Wc + (size_t)(2 * blk) * 9 * kRcC * kRcC;
     hipLaunchKernelGGL(k_rg_rinv, dim3(nimg), dim3(256), 0, st, N, H, rinv);
+    if (bf16) {
+      const rb_i32x4* L1 = Wlp + (size_t)(2 * blk) * kSlabsPerConv * 2 * kSlabFrag;
+      hipLaunchKernelGGL(k_rg_act_bf16<true>, dim3(ga), dim3(256), 0, st, N, nimg, H, rinv, A16);
+      hipLaunchKernelGGL((k_rg_conv_bf16<false>), gc, dim3(256), 0, st, m->LX, m->LY, A16, L1,
+                         m->conv_b + (2 * blk) * kRcC, B);
+      hipLaunchKernelGGL(k_rg_act_bf16<false>, dim3(ga), dim3(256), 0, st, N, nimg, B, nullptr, A16);
+      hipLaunchKernelGGL((k_rg_conv_bf16<true>), gc, dim3(256), 0, st, m->LX, m->LY, A16,
+                         L1 + kSlabsPerConv * 2 * kSlabFrag, m->conv_b + (2 * blk + 1) * kRcC, H);
+      continue;
+    }
     hipLaunchKernelGGL(k_rg_act<true>, dim3(ga), dim3(256), 0, st, N, nimg, H, rinv, A);
     hipLaunchKernelGGL((k_rg_conv_mfma<false>), gc, dim3(256), 0, st, m->LX, m->LY, A, W1,
                        m->conv_b + (2 * blk) * kRcC, B);

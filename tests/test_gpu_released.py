@@ -104,9 +104,9 @@ def test_small_resconv_matches_oracle(kind, LX, n_el, blocks, prec):
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 @pytest.mark.parametrize("LX,LY,n_el,blocks", [(4, 6, 8, 2), (6, 6, 14, 2), (20, 4, 30, 1)])
 def test_other_lattice_resconv_matches_oracle(LX, LY, n_el, blocks, prec):
-    """The residual CNN on lattices outside the tile kernels (layer by layer, FP32 on the VALU, in
-    both precision settings): features through nh_features_ws, bare and projected log|psi| and
-    E_loc against the FP64 oracle, tolerances as in the 16-row test."""
+    """The residual CNN on lattices outside the tile kernels (layer by layer: FP32 MFMA, or split
+    BF16 MFMA in the mixed setting): features through nh_features_ws, bare and projected log|psi|
+    and E_loc against the FP64 oracle, tolerances as in the 16-row test."""
     from neuralhubbard_amd.backflow import Hubbard, LogArray, Samples, Variational, build_backflow
     from neuralhubbard_amd.configs import make_configurations, make_params, small_workload
 
@@ -115,15 +115,17 @@ def test_other_lattice_resconv_matches_oracle(LX, LY, n_el, blocks, prec):
     p = make_params(wk)
     S = make_configurations(wk, 4)
     print(f"resconv {LX}x{LY} n_el={n_el} blocks={blocks} {prec}", flush=True)
+    mixed = prec == "mixed"
     ref = O.Model(wk, p)
     dev = build_backflow(wk, p, prec)
     y = dev.features(torch.tensor(S, device="cuda")).cpu().numpy()
     yr = ref.features_batch(S)
-    np.testing.assert_allclose(y, yr, rtol=0, atol=1e-4 * np.abs(yr).max())
+    print("max |d y| / max |y|", np.abs(y - yr).max() / np.abs(yr).max(), flush=True)
+    np.testing.assert_allclose(y, yr, rtol=0, atol=(1e-3 if mixed else 1e-4) * np.abs(yr).max())
     for symm in (False, True):
         sg, la = Variational(dev, symm=symm)(torch.tensor(S, device="cuda")).numpy()
         rs, rl = ref.psi_batch(S, project=symm)
-        tol = 1e-4 + 30 * _sensitivity(wk, p, S, symm, rel=1e-7)
+        tol = 1e-4 + 30 * _sensitivity(wk, p, S, symm, rel=1e-5 if mixed else 1e-7)
         print(f"project={symm}: |d log psi| {np.abs(la - rl)}, tolerance {tol}", flush=True)
         np.testing.assert_array_equal(sg, rs)
         assert np.all(np.abs(la - rl) <= tol)
@@ -133,7 +135,7 @@ def test_other_lattice_resconv_matches_oracle(LX, LY, n_el, blocks, prec):
     el = Hubbard(U=wk.U, tprime=wk.tprime, chunk=2).Oloc(Variational(dev), Samples(torch.tensor(S, device="cuda"), psi))
     el = el.cpu().numpy()
     print(f"E_loc / N {el / wk.N}, oracle {er / wk.N}", flush=True)
-    np.testing.assert_allclose(el / wk.N, er / wk.N, rtol=0, atol=1e-4)
+    np.testing.assert_allclose(el / wk.N, er / wk.N, rtol=0, atol=1e-4 * (10 if mixed else 1))
 
 
 NAMES = ("tpm02_L16_pfaffian", "tp0_L16_pfaffian", "tp0_L16_bcs")
