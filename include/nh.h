@@ -57,8 +57,8 @@ enum { NH_NET_CONV = 0, NH_NET_RESCONV = 1, NH_NET_RESCONV_HEAD = 2 };
  *   A-fragments of the convs (layout in neuralhubbard_amd/mixed.py, pack_resconv_bf16).
  *   conv_w_mfma, when given, holds the FP32 convs as B-fragment slabs (pack_resconv_f32) that
  *   the FP32 kernel streams through LDS; NULL reads conv_w directly.  Other lattices
- *   (N <= 1024) run the network (and the BCS head) layer by layer in FP32 (MFMA convolutions)
- *   with the activations in the workspace, in both precisions.
+ *   (N <= 1024) run the network (and the BCS head) layer by layer with the activations in the
+ *   workspace: FP32 MFMA convolutions, or split-BF16 MFMA on conv_w_lp under NH_PREC_MIXED.
  * network NH_NET_RESCONV_HEAD: the same network followed by the BCS states' linear head
  *   (code/backflow/models.py:261-270): y[k][site][a] = sum_c head[k][a][c] x[c][site], x the
  *   scaled output, d = C features per spin; conv_w ends with the head as [k][c][a] (2 x C x C);

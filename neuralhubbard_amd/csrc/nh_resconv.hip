@@ -980,8 +980,8 @@ static int launch_resconv_layers(hipStream_t st, const nh_model* m, const int8_t
   const float* Wc = m->conv_w + kRcClasses * kRcC;
   const unsigned ge = (unsigned)((per + 255) / 256);
   const unsigned ga = (unsigned)(((size_t)nimg * per + 255) / 256);
-  // 32 sites per block on FP32 MFMA: 43.4 us per 24x24 image (VALU FMA: 48 us with 32 sites per
-  // block, 52.8 with 16, 62.2 with 64)
+  // 32 sites per block on FP32 MFMA: 37.9 us per 24x24 image (VALU FMA: 48 us with 32 sites per
+  // block, 52.8 with 16, 62.2 with 64); split BF16 12.5 us
   const dim3 gc((unsigned)((N + 31) / 32), (unsigned)nimg);
   hipLaunchKernelGGL(k_rg_embed, dim3(ge, nimg), dim3(256), 0, st, m->LX, m->LY, s, item_b, item_hop, img0, ipi,
                      emb, H);
