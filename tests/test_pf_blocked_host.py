@@ -1,0 +1,78 @@
+"""The blocked Pfaffian pf_blocked2 (neuralhubbard_amd/csrc/nh_pf_blocked.h) run on the CPU under
+a host emulation of its workgroup (tests/host/pf_blocked_host.cpp: one OS thread per GPU thread,
+the wave builtins as exchanges between barriers of 64 lanes), built with AddressSanitizer and the
+alignment check: every access stays inside the triangle and the scratch the kernels allot, no
+value read from scratch that was never written reaches the result (the scratch starts as NaN),
+readfirstlane only sees wave-uniform values, and (sign, log|Pf|) matches the FP64 oracle.  Covers
+the panel widths in use (8 steps, 4 in the LDS form above n = 240) and wider ones (12, 16), at the
+sizes of the image kernels (n = 224 for 16x16 at 112 + 112 electrons) and the standalone
+nh_slogpf form (1024 threads)."""
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CASES = [(256, 8, 224), (256, 4, 130), (256, 12, 224), (256, 16, 256), (256, 12, 66), (1024, 8, 224),
+         (1024, 4, 256)]
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    exe = str(tmp_path_factory.mktemp("pfh") / "pf_blocked_host")
+    cmd = ["g++", "-std=c++20", "-O1", "-g", "-fsanitize=address,alignment", "-fno-sanitize-recover=all",
+           "-fno-strict-aliasing", "-pthread", "-Wno-unknown-pragmas",
+           "-I", os.path.join(ROOT, "neuralhubbard_amd", "csrc"),
+           os.path.join(HERE, "host", "pf_blocked_host.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def _run(exe, tmp_path, NT, P, A):
+    n = A.shape[0]
+    i, j = np.tril_indices(n, -1)  # row-major strict lower triangle: row i at i (i - 1) / 2
+    path = str(tmp_path / f"tri_{NT}_{P}_{n}.bin")
+    A[i, j].astype(np.float32).tofile(path)
+    r = subprocess.run([exe, str(NT), str(P), str(n), path], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    sg, la = r.stdout.split()
+    return float(sg), float(la)
+
+
+@pytest.mark.parametrize("NT,P,n", CASES)
+def test_pf_blocked_host_matches_oracle(harness, tmp_path, NT, P, n):
+    rng = np.random.default_rng(1000 * P + n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    A = A - A.T
+    sg, la = _run(harness, tmp_path, NT, P, A)
+    rs, rl = O.slogpf(A.astype(np.float64))
+    assert np.isfinite(la), "a NaN scratch value reached the result"
+    assert sg == rs
+    assert abs(la - rl) < 1e-4 * n  # FP32 elimination, FP64 log-sum
+
+
+def test_pf_blocked_host_pins_a_known_pfaffian(harness, tmp_path):
+    """Block-diagonal [[0, d_k], [-d_k, 0]] blocks, rows and columns permuted at random: Pf is
+    sgn(perm) prod d_k, a value fixed without the oracle, reached only through pivoting."""
+    n = 130
+    rng = np.random.default_rng(7)
+    d = rng.uniform(0.5, 2.0, n // 2) * rng.choice([-1.0, 1.0], n // 2)
+    B = np.zeros((n, n))
+    B[np.arange(0, n, 2), np.arange(1, n, 2)] = d
+    B -= B.T
+    perm = rng.permutation(n)
+    A = np.zeros_like(B)
+    A[np.ix_(perm, perm)] = B  # A[perm[i]][perm[j]] = B[i][j], Pf A = sgn(perm) Pf B
+    sgn_perm = np.linalg.det(np.eye(n)[perm])
+    sg, la = _run(harness, tmp_path, 256, 8, A.astype(np.float32))
+    assert sg == np.sign(sgn_perm) * np.sign(np.prod(d))
+    assert abs(la - np.sum(np.log(np.abs(d.astype(np.float32).astype(np.float64))))) < 1e-4
