@@ -1,4 +1,5 @@
-// Host emulation of the blocked Pfaffian pf_blocked2 (neuralhubbard_amd/csrc/nh_pf_blocked.h):
+// Host emulation of the blocked Pfaffian pf_blocked2 (neuralhubbard_amd/csrc/nh_pf_blocked.h) and
+// the blocked LU lu_blocked (nh_lu_blocked.h):
 // one OS thread per GPU thread of the workgroup, __syncthreads as a barrier of all of them, and
 // the wave-level builtins the routine uses (readfirstlane, the wave max, shfl_xor and the
 // 16x16x4 FP32 MFMA) as exchanges through a per-wave buffer between barriers of its 64 lanes.
@@ -8,6 +9,9 @@
 //
 //   pf_blocked_host NT P n tri.bin   ->  prints "sign logabs" of the packed strict lower triangle
 //                                        (n (n - 1) / 2 float32, row i at i (i - 1) / 2)
+//   pf_blocked_host lu NT n mat.bin   ->  "sign logabs" of det of the row-major n x n float32
+//                                        matrix, held as the BCS kernels hold it (leading
+//                                        dimension n + 1, 16-column panels, 9-bit row keys)
 
 #include <algorithm>
 #include <barrier>
@@ -129,6 +133,38 @@ inline int toff(int i, int j) { return ((i * (i - 1)) >> 1) + j; }
 
 #include "nh_pf_blocked.h"
 
+#include "nh_lu_blocked.h"
+
+template <int NT>
+static void run_lu(int n, const std::vector<float>& in, float& sg, double& la) {
+  const int ld = n + 1;
+  float* M = static_cast<float*>(std::malloc(sizeof(float) * ((size_t)n * ld + kWave)));
+  for (size_t i = 0; i < (size_t)n * ld + kWave; ++i) M[i] = 0.f;  // padding column and dummies
+  for (int r = 0; r < n; ++r) std::memcpy(M + (size_t)r * ld, in.data() + (size_t)r * n, sizeof(float) * n);
+  const int nscr = lu_scratch_words(n);
+  uint32_t* scr = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * nscr));
+  for (int i = 0; i < nscr; ++i) scr[i] = 0x7fc00000u;  // NaN
+  std::barrier<> block(NT);
+  std::vector<EmuWave> waves(NT / kWave);
+  g_block = &block;
+  g_waves = waves.data();
+  std::vector<std::thread> th;
+  for (int t = 0; t < NT; ++t)
+    th.emplace_back([&, t] {
+      threadIdx.x = t;
+      float s;
+      double l;
+      lu_blocked<NT, 16, 9>(M, n, ld, scr, s, l);
+      if (t == 0) {
+        sg = s;
+        la = l;
+      }
+    });
+  for (auto& x : th) x.join();
+  std::free(scr);
+  std::free(M);
+}
+
 template <int NT, int P>
 static void run(int n, const std::vector<float>& in, float& sg, double& la) {
   const size_t ntri = (size_t)n * (n - 1) / 2;
@@ -159,7 +195,24 @@ static void run(int n, const std::vector<float>& in, float& sg, double& la) {
   std::free(tri);
 }
 
+static int main_lu(int argc, char** argv) {
+  const int NT = std::atoi(argv[2]), n = std::atoi(argv[3]);
+  if (n < 1 || n > NT || n > 512) return 2;
+  std::vector<float> in((size_t)n * n);
+  FILE* f = std::fopen(argv[4], "rb");
+  if (!f || std::fread(in.data(), sizeof(float), in.size(), f) != in.size()) return 2;
+  std::fclose(f);
+  float sg = 0.f;
+  double la = 0.0;
+  if (NT == 256) run_lu<256>(n, in, sg, la);
+  else if (NT == 512) run_lu<512>(n, in, sg, la);
+  else return 2;
+  std::printf("%.1f %.17g\n", sg, la);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 5 && std::strcmp(argv[1], "lu") == 0) return main_lu(argc, argv);
   if (argc != 5) {
     std::fprintf(stderr, "usage: %s NT P n tri.bin\n", argv[0]);
     return 2;

@@ -1,12 +1,14 @@
-"""The blocked Pfaffian pf_blocked2 (neuralhubbard_amd/csrc/nh_pf_blocked.h) run on the CPU under
-a host emulation of its workgroup (tests/host/pf_blocked_host.cpp: one OS thread per GPU thread,
-the wave builtins as exchanges between barriers of 64 lanes), built with AddressSanitizer and the
-alignment check: every access stays inside the triangle and the scratch the kernels allot, no
-value read from scratch that was never written reaches the result (the scratch starts as NaN),
-readfirstlane only sees wave-uniform values, and (sign, log|Pf|) matches the FP64 oracle.  Covers
-the panel widths in use (8 steps, 4 in the LDS form above n = 240) and wider ones (12, 16), at the
-sizes of the image kernels (n = 224 for 16x16 at 112 + 112 electrons) and the standalone
-nh_slogpf form (1024 threads)."""
+"""The blocked Pfaffian pf_blocked2 (neuralhubbard_amd/csrc/nh_pf_blocked.h) and the blocked LU
+lu_blocked (nh_lu_blocked.h) run on the CPU under a host emulation of their workgroup
+(tests/host/pf_blocked_host.cpp: one OS thread per GPU thread, the wave builtins as exchanges
+between barriers of 64 lanes), built with AddressSanitizer and the alignment check: every access
+stays inside the matrix and the scratch the kernels allot, no value read from scratch that was
+never written reaches the result (the scratch starts as NaN), readfirstlane only sees
+wave-uniform values, and (sign, log|Pf|) matches the FP64 oracle, (sign, log|det|) LAPACK.
+Pfaffian: the panel widths in use (8 steps, 4 in the LDS form above n = 240) and wider ones (12,
+16), at the sizes of the image kernels (n = 224 for 16x16 at 112 + 112 electrons) and the
+standalone nh_slogpf form (1024 threads).  LU: the BCS kernels' 16-column panels at n = 100 to
+448 (the 32x32 BCS state)."""
 
 import os
 import shutil
@@ -75,4 +77,41 @@ def test_pf_blocked_host_pins_a_known_pfaffian(harness, tmp_path):
     sgn_perm = np.linalg.det(np.eye(n)[perm])
     sg, la = _run(harness, tmp_path, 256, 8, A.astype(np.float32))
     assert sg == np.sign(sgn_perm) * np.sign(np.prod(d))
+    assert abs(la - np.sum(np.log(np.abs(d.astype(np.float32).astype(np.float64))))) < 1e-4
+
+
+def _run_lu(exe, tmp_path, NT, A):
+    n = A.shape[0]
+    path = str(tmp_path / f"mat_{NT}_{n}.bin")
+    np.ascontiguousarray(A, dtype=np.float32).tofile(path)
+    r = subprocess.run([exe, "lu", str(NT), str(n), path], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    sg, la = r.stdout.split()
+    return float(sg), float(la)
+
+
+@pytest.mark.parametrize("NT,n", [(256, 100), (256, 256), (512, 130), (512, 448)])
+def test_lu_blocked_host_matches_lapack(harness, tmp_path, NT, n):
+    """lu_blocked (the BCS determinant: 16-column panels, rows kept in place, 9-bit row keys) at
+    the LDS sizes and at n = 448 of the 32x32 BCS state, against LAPACK's slogdet in FP64."""
+    rng = np.random.default_rng(n + NT)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    sg, la = _run_lu(harness, tmp_path, NT, A)
+    rs, rl = np.linalg.slogdet(A.astype(np.float64))
+    assert np.isfinite(la), "a NaN scratch value reached the result"
+    assert sg == rs
+    assert abs(la - rl) < 1e-4 * n
+
+
+def test_lu_blocked_host_pins_a_permuted_diagonal(harness, tmp_path):
+    """det of a row-permuted diagonal matrix is sgn(perm) prod d: reached only through pivoting,
+    and its sign only through the parity count of the pivot rows."""
+    n = 200
+    rng = np.random.default_rng(11)
+    d = rng.uniform(0.5, 2.0, n) * rng.choice([-1.0, 1.0], n)
+    perm = rng.permutation(n)
+    A = np.zeros((n, n))
+    A[perm, np.arange(n)] = d  # column j holds d_j in row perm[j]
+    sg, la = _run_lu(harness, tmp_path, 256, A.astype(np.float32))
+    assert sg == np.linalg.det(np.eye(n)[perm]) * np.sign(np.prod(d))
     assert abs(la - np.sum(np.log(np.abs(d.astype(np.float32).astype(np.float64))))) < 1e-4
