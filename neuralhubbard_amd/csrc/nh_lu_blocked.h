@@ -2,6 +2,7 @@
 // workgroup, and its scratch size.  A fragment, not a standalone header: it is included inside
 // namespace nh of nh_linalg.hip after kWave, wave_max_u32, wave_sum_f64 and nh_f32x4, and by the
 // host emulation under tests/host, which supplies those and the wave-level builtins itself.
+#pragma once
 
 constexpr int kLuPanel = 16;  // columns per panel of the blocked LU
 

@@ -2,6 +2,7 @@
 // scratch size.  A fragment, not a standalone header: it is included inside namespace nh of
 // nh_linalg.hip after kWave, toff, wave_max_u32 and nh_f32x4, and by the host emulation under
 // tests/host, which supplies those and the wave-level builtins itself.
+#pragma once
 
 // Scratch floats of pf_blocked2: panel vectors row-interleaved UW (n + 16) x 2P and q-major
 // UT, WT P x (n + 16) each, column buffer n + 4, pivot keys 16 words, pivots n / 2 + 4.
