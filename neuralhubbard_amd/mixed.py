@@ -3,9 +3,11 @@ stack, C == 64 for the residual CNN, `pack_resconv_bf16`).
 
 FP32 (v_mfma_f32_16x16x4_f32): for layer l, tap, output half hf, K-step kk and lane:
     W[tap][ci = 4 kk + lane // 16][co = 16 hf + lane % 16]
-BF16 (v_mfma_f32_16x16x32_bf16): for layer l, tap, half hf, lane, j < 8:
-    W[tap][ci = 8 (lane // 16) + j][co = 16 hf + lane % 16]
-  stored as split BF16: the hi parts of all layers, then the lo parts, W - hi rounded.
+FP16 (v_mfma_f32_16x16x32_f16): for layer l, tap, half hf, lane, j < 8:
+    LP_SCALE * W[tap][ci = 8 (lane // 16) + j][co = 16 hf + lane % 16]
+  stored as split FP16: the hi parts of all layers, then the lo parts, LP_SCALE * W - hi
+  rounded.  The power-of-two pre-scale keeps the lo parts of the small conv weights FP16
+  normals; the kernel divides it out of each layer's sum exactly.
 Layer 0 has 2 input channels; the missing ones are zero.  Tap = (dx + 1) * 3 + (dy + 1).
 """
 
@@ -14,6 +16,7 @@ from __future__ import annotations
 import numpy as np
 
 C = 32
+LP_SCALE = 64.0  # kLpScale of nh_cnn_mfma.hip
 
 
 def _tap_major(params):
@@ -47,8 +50,9 @@ def bf16_bits(x: np.ndarray) -> np.ndarray:
     return u.astype(np.uint16)
 
 
-def pack_bf16(params) -> np.ndarray:
-    """Split-BF16 fragments: hi parts of every layer, then lo parts (W = hi + lo + O(2^-16 W))."""
+def pack_f16(params) -> np.ndarray:
+    """Split-FP16 fragments of LP_SCALE * W: hi parts of every layer, then lo parts, as uint16
+    bit patterns (LP_SCALE * W = hi + lo + O(2^-22 |W|))."""
     W = _tap_major(params)
     lane = np.arange(64)
     j = np.arange(8)
@@ -57,10 +61,10 @@ def pack_bf16(params) -> np.ndarray:
     for hf in range(2):
         co = (16 * hf + lane % 16)[:, None]  # (64, 1)
         out[:, :, hf] = W[:, :, ci, co]
-    hi = bf16_bits(out.reshape(-1))
-    hif = (hi.astype(np.uint32) << 16).view(np.float32)
-    lo = bf16_bits(out.reshape(-1) - hif)
-    return np.concatenate([hi, lo])
+    x = out.reshape(-1) * np.float32(LP_SCALE)
+    hi = x.astype(np.float16)
+    lo = (x - hi.astype(np.float32)).astype(np.float16)
+    return np.concatenate([hi.view(np.uint16), lo.view(np.uint16)])
 
 
 def pack_resconv_bf16(params) -> np.ndarray:
@@ -114,4 +118,4 @@ def pack_lowp(model):
 
     if model.workload.network == "resconv":
         return torch.as_tensor(pack_resconv_bf16(model.params).view(np.int16), device="cuda")
-    return torch.as_tensor(pack_bf16(model.params).view(np.int16), device="cuda")
+    return torch.as_tensor(pack_f16(model.params).view(np.int16), device="cuda")

@@ -80,6 +80,32 @@ def test_pack_resconv_layout():
     assert b[3 * C + 40] == p.conv_b[3][40]
 
 
+def test_pack_f16_layout():
+    """Split-FP16 fragments of the conv stack: hi + lo = 2^6 W to FP16's 2^-22 relative (plus half
+    the FP16 subnormal spacing 2^-25 of the lo part, 2^-31 in W), in the
+    v_mfma_f32_16x16x32_f16 A layout [layer][tap][half][lane][j] -> W[tap][8 (lane // 16) + j][16 half + lane % 16]."""
+    from neuralhubbard_amd.mixed import LP_SCALE, pack_f16
+
+    wk = small_workload("pfaffian", LX=4, LY=16, n_up=12, n_dn=12, channels=32, n_layers=2)
+    p = make_params(wk)
+    packed = pack_f16(p)
+    L = wk.n_layers + 1
+    assert packed.size == 2 * L * 9 * 2 * 64 * 8
+    f = packed.view(np.float16).astype(np.float64).reshape(2, L, 9, 2, 64, 8)
+    rec = (f[0] + f[1]) / LP_SCALE
+    lane = np.arange(64)
+    for l in range(L):
+        w = np.asarray(p.conv_w[l], np.float64)  # (co, ci, 3, 3)
+        for tap in (0, 4, 8):
+            for hf in range(2):
+                ci = 8 * (lane // 16)[:, None] + np.arange(8)[None, :]
+                co = (16 * hf + lane % 16)[:, None]
+                want = np.zeros((64, 8))
+                ok = ci < w.shape[1]
+                want[ok] = w[np.broadcast_to(co, ci.shape)[ok], ci[ok], tap // 3, tap % 3]
+                assert np.all(np.abs(rec[l, tap, hf] - want) <= 2.0**-21 * np.abs(want) + 2.0**-31)
+
+
 def test_pack_resconv_bf16_layout():
     from neuralhubbard_amd.mixed import pack_resconv_bf16
 
