@@ -1,13 +1,13 @@
-"""Statistics of a run of sweeps (host side, float64) and the reference's measurement script with
-its file formats.
+"""Statistics of a run of sweeps (host side, float64) and the measurement driver with the file
+formats of the reference's measurement script.
 
-Mirrors ``code/measure/assemble.py:48-97`` (``BLOCK_SIZES``, ``MIN_BLOCKS``, ``jackknife``,
-``energy``): the per-sweep means of E_loc and E_loc^2 are grouped in blocks of 1, 2, 4 and 8
-consecutive sweeps (an incomplete last block is dropped); every block size that leaves at least
-8 blocks gives a leave-one-block-out jackknife error, and the quoted error is the largest.
-`run_measure` writes ``series.npz``, ``spins_last.npz``, ``results.json`` and ``results.npz`` as
-``code/measure_orderparams.py`` and ``code/measure/assemble.py`` do, for the energy, spin, charge
-and d-wave channels; `assemble` turns a series into E/N, M^2_str, N2(q_c), C_p(r) and
+Error bars are the reference's definition (``code/measure/assemble.py:48-97``): per-sweep
+means are grouped into blocks of 1, 2, 4 and 8 consecutive sweeps (an incomplete last block is
+dropped); each block size leaving at least 8 blocks gives a delete-one-block jackknife error, and
+the quoted error is the largest of them.  `run_measure` writes ``series.npz``,
+``spins_last.npz``, ``results.json`` and ``results.npz`` with the reference's keys
+(``code/measure_orderparams.py``, ``code/measure/assemble.py:296-411``) for the energy, spin,
+charge and d-wave channels; `assemble` turns a series into E/N, M^2_str, N2(q_c), C_p(r) and
 Delta^2_SC.
 """
 
@@ -19,9 +19,21 @@ BLOCK_SIZES = (1, 2, 4, 8)
 MIN_BLOCKS = 8
 
 
+def delete_one_means(x, b: int):
+    """Means over all blocks but one, one row per deleted block, for blocks of b consecutive rows
+    of x (the incomplete tail dropped): (nb, ...)."""
+    x = np.asarray(x, np.float64)
+    nb = x.shape[0] // b
+    used = x[: nb * b]
+    block_sums = np.add.reduceat(used, np.arange(0, nb * b, b), axis=0)
+    return (used.sum(axis=0) - block_sums) / ((nb - 1) * b)
+
+
 def jackknife(f, *series):
-    """(f(means), max error over block sizes, {block size: error}) of per-sweep series."""
-    xs = [np.asarray(s, dtype=np.float64) for s in series]
+    """(f of the sweep means, error, {block size: error}) of per-sweep series (leading axis =
+    sweeps); f may return arrays, the error is the element-wise maximum over the admissible block
+    sizes (NaN where none is admissible)."""
+    xs = [np.asarray(x, np.float64) for x in series]
     value = f(*(x.mean(axis=0) for x in xs))
     errs = {}
     n = xs[0].shape[0]
@@ -29,22 +41,28 @@ def jackknife(f, *series):
         nb = n // b
         if nb < MIN_BLOCKS:
             continue
-        blocks = [x[:nb * b].reshape((nb, b) + x.shape[1:]).mean(axis=1) for x in xs]
-        # leave-one-block-out means of every series, shape (nb, ...)
-        loo = [(blk.sum(axis=0)[None] - blk) / (nb - 1) for blk in blocks]
-        reps = np.array([f(*(l[i] for l in loo)) for i in range(nb)])
-        errs[b] = np.sqrt(nb - 1) * reps.std(axis=0)
-    error = np.max(np.stack(list(errs.values())), axis=0) if errs else np.full(np.shape(value), np.nan)
+        reps = np.stack([np.asarray(f(*cols), np.float64)
+                         for cols in zip(*(delete_one_means(x, b) for x in xs))])
+        dev = reps - reps.mean(axis=0)
+        errs[b] = np.sqrt((nb - 1) / nb * np.sum(dev * dev, axis=0))
+    error = np.maximum.reduce(list(errs.values())) if errs else np.full(np.shape(value), np.nan)
     return value, error, errs
 
 
+def _per_block(errs: dict) -> dict:
+    return {int(b): float(np.asarray(e)) for b, e in errs.items()}
+
+
 def energy(E_mean, E_sqr, N: int) -> dict:
-    """E/N and sigma^2/N with jackknife errors from per-sweep means of E_loc and E_loc^2."""
-    e, e_err, e_errs = jackknife(lambda m: m / N, E_mean)
-    v, v_err, v_errs = jackknife(lambda m, q: (q - m * m) / N, E_mean, E_sqr)
-    return {"E_per_site": float(e), "E_err": float(e_err), "var_per_site": float(v), "var_err": float(v_err),
-            "E_err_blocks": {int(b): float(x) for b, x in e_errs.items()},
-            "var_err_blocks": {int(b): float(x) for b, x in v_errs.items()}, "nsweeps": int(len(E_mean))}
+    """E/N and sigma^2/N = (<E^2> - <E>^2) / N with their errors, from per-sweep means of E_loc and
+    E_loc^2 (keys of the reference's results.json)."""
+    E_mean, E_sqr = np.asarray(E_mean, np.float64), np.asarray(E_sqr, np.float64)
+    e, e_err, e_blk = jackknife(lambda m: m / N, E_mean)
+    var, var_err, var_blk = jackknife(lambda m, q: (q - m * m) / N, E_mean, E_sqr)
+    out = {"E_per_site": e, "E_err": e_err, "var_per_site": var, "var_err": var_err}
+    out = {k: float(v) for k, v in out.items()}
+    out.update(E_err_blocks=_per_block(e_blk), var_err_blocks=_per_block(var_blk), nsweeps=int(E_mean.shape[0]))
+    return out
 
 
 def within_error_bars(a: dict, b: dict, k: float = 2.0) -> bool:
@@ -60,88 +78,75 @@ SAVE_EVERY = 4  # sweeps between rewrites of series.npz (code/measure_orderparam
 
 
 def paper_family(tprime: float) -> str:
-    """Stripe family label the reference stores with a series (``observables.paper_family``)."""
+    """Stripe family label stored with a series: pi/8 stripes at t' = 0, pi/4 otherwise."""
     return "pi8" if tprime == 0 else "pi4"
 
 
-def _json_value(v):
-    if isinstance(v, dict):
-        return {str(k): _json_value(x) for k, x in v.items()}
-    if isinstance(v, np.generic):
-        v = v.item()
-    if isinstance(v, float) and not np.isfinite(v):
-        return None
-    return v
-
-
-def _blocks(errs: dict) -> dict:
-    return {int(b): float(np.asarray(x)) for b, x in errs.items()}
-
-
 def spin(Czz, Tpm, Tmp, LX: int, LY: int, family: str) -> dict:
-    """C_s(r), S(k) and M^2_str (M2_zz: the zz part alone) from per-sweep Czz (S, N) and the
-    transverse terms Tpm, Tmp (S, N) or (S, NREF, N), reference sites averaged
-    (``assemble.spin_corr_series`` / ``assemble.spin``)."""
-    from .observables import stripe_k, structure_factor
+    """Spin correlation C_s(r) = C_zz(r) + (T+-(r) + T-+(r)) / 2 per sweep (transverse terms of
+    several reference sites averaged), its structure factor, and the stripe order
+    M^2_str = (1/4N) sum_alpha S(k_alpha) with the zz part alone as M2_zz."""
+    from .observables import stripe_k, stripe_weights, structure_factor
 
     Czz = np.asarray(Czz, np.float64)
-    Tpm, Tmp = np.asarray(Tpm, np.float64), np.asarray(Tmp, np.float64)
-    if Tpm.ndim == 3:
-        Tpm, Tmp = Tpm.mean(1), Tmp.mean(1)
+    Tpm, Tmp = (np.asarray(t, np.float64) for t in (Tpm, Tmp))
+    if Tpm.ndim == 3:  # (sweeps, reference sites, N)
+        Tpm, Tmp = Tpm.mean(axis=1), Tmp.mean(axis=1)
     Cs = Czz + 0.5 * (Tpm + Tmp)
-    N = LX * LY
-    ks = stripe_k(LX, LY, family)
-
-    def fam(Sk):
-        return np.mean([Sk[..., kx % LX, ky % LY] for kx, ky in ks], axis=0) / N
-
-    v, e, errs = jackknife(lambda x: x, fam(structure_factor(Cs, LX, LY)))
-    vz, ez, _ = jackknife(lambda x: x, fam(structure_factor(Czz, LX, LY)))
-    C, C_err, _ = jackknife(lambda x: x, Cs)
-    return {"C_s": C, "C_s_err": C_err, "S_k": structure_factor(C, LX, LY), "k_alpha": np.array(ks),
-            "family": family, "M2_str": float(v), "M2_str_err": float(e), "M2_str_err_blocks": _blocks(errs),
-            "M2_zz": float(vz), "M2_zz_err": float(ez), "nsweeps": int(len(Cs))}
+    w = stripe_weights(LX, LY, stripe_k(LX, LY, family))  # M^2 = w . C, linear in C
+    m2, m2_err, m2_blk = jackknife(lambda c: c @ w, Cs)
+    mzz, mzz_err, _ = jackknife(lambda c: c @ w, Czz)
+    C, C_err, _ = jackknife(lambda c: c, Cs)
+    return {"C_s": C, "C_s_err": C_err, "S_k": structure_factor(C, LX, LY), "k_alpha": np.array(stripe_k(LX, LY, family)),
+            "family": family, "M2_str": float(m2), "M2_str_err": float(m2_err), "M2_str_err_blocks": _per_block(m2_blk),
+            "M2_zz": float(mzz), "M2_zz_err": float(mzz_err), "nsweeps": int(Cs.shape[0])}
 
 
 def charge(Cnn, LX: int, LY: int, family: str) -> dict:
-    """C_c(r), N(q) and N2 = (1/4N) sum_alpha N(q_alpha) at the charge wavevectors
-    (``assemble.charge``)."""
-    from .observables import charge_q, structure_factor
+    """Charge correlation C_c(r) per sweep, its structure factor N(q) and
+    N2 = (1/4N) sum_alpha N(q_alpha) at the charge wavevectors."""
+    from .observables import charge_q, stripe_weights, structure_factor
 
     Cnn = np.asarray(Cnn, np.float64)
-    qs = charge_q(LX, LY, family)
-    C, C_err, _ = jackknife(lambda x: x, Cnn)
-    Nq = structure_factor(Cnn, LX, LY)
-    n2 = np.mean([Nq[..., qx % LX, qy % LY] for qx, qy in qs], axis=0) / (LX * LY)
-    v, e, _ = jackknife(lambda x: x, n2)
-    return {"C_c": C, "C_c_err": C_err, "N_q": structure_factor(C, LX, LY), "q_c": np.array(qs),
-            "N2": float(v), "N2_err": float(e), "nsweeps": int(len(Cnn))}
+    w = stripe_weights(LX, LY, charge_q(LX, LY, family))
+    n2, n2_err, _ = jackknife(lambda c: c @ w, Cnn)
+    C, C_err, _ = jackknife(lambda c: c, Cnn)
+    return {"C_c": C, "C_c_err": C_err, "N_q": structure_factor(C, LX, LY), "q_c": np.array(charge_q(LX, LY, family)),
+            "N2": float(n2), "N2_err": float(n2_err), "nsweeps": int(Cnn.shape[0])}
+
+
+def _local(series, LX, LY, family):
+    sz = np.asarray(series["sz_site"], np.float64).mean(axis=0)
+    n = np.asarray(series["n_site"], np.float64).mean(axis=0)
+    return {"max_abs_sz": float(np.abs(sz).max()), "n_min": float(n.min()), "n_max": float(n.max())}
+
+
+def _dwave(series, LX, LY, family):
+    from .observables import dwave
+
+    dx, dy = np.divmod(np.arange(LX * LY), LY)
+    return {"dr": np.stack([dx, dy], 1), **dwave(series["F"], series["G"], LX, LY), "translation_fill": True}
+
+
+# result key, the series keys it needs, how it is assembled
+_ASSEMBLERS = (
+    ("energy", ("E_mean", "E_sqr"), lambda s, LX, LY, fam: energy(s["E_mean"], s["E_sqr"], LX * LY)),
+    ("spin", ("Czz", "Tpm", "Tmp"), lambda s, LX, LY, fam: spin(s["Czz"], s["Tpm"], s["Tmp"], LX, LY, fam)),
+    ("charge", ("Cnn",), lambda s, LX, LY, fam: charge(s["Cnn"], LX, LY, fam)),
+    ("dwave", ("F", "G"), _dwave),
+    ("local", ("sz_site", "n_site"), _local),
+)
 
 
 def assemble(series: dict) -> dict:
-    """Every observable a series holds (``code/measure/assemble.py:301-327``)."""
-    from .observables import dwave
-
+    """Every observable a series (the content of series.npz) holds."""
     LX, LY = int(series["LX"]), int(series["LY"])
     family = str(series["family"])
     res = {"LX": LX, "LY": LY, "family": family}
-    for k in ("state", "tprime", "nsamples"):
-        if k in series:
-            res[k] = np.asarray(series[k]).item()
-    if "E_mean" in series and len(series["E_mean"]):
-        res["energy"] = energy(series["E_mean"], series["E_sqr"], LX * LY)
-    if "Czz" in series and "Tpm" in series:
-        res["spin"] = spin(series["Czz"], series["Tpm"], series["Tmp"], LX, LY, family)
-    if "Cnn" in series:
-        res["charge"] = charge(series["Cnn"], LX, LY, family)
-    if "F" in series:
-        d = dwave(series["F"], series["G"], LX, LY)
-        dx, dy = np.divmod(np.arange(LX * LY), LY)
-        res["dwave"] = {"dr": np.stack([dx, dy], 1), **d, "translation_fill": True}
-    if "sz_site" in series:
-        sz = np.asarray(series["sz_site"], np.float64).mean(0)
-        n = np.asarray(series["n_site"], np.float64).mean(0)
-        res["local"] = {"max_abs_sz": float(np.abs(sz).max()), "n_min": float(n.min()), "n_max": float(n.max())}
+    res.update({k: np.asarray(series[k]).item() for k in ("state", "tprime", "nsamples") if k in series})
+    for key, needs, make in _ASSEMBLERS:
+        if all(k in series for k in needs) and len(series[needs[0]]):
+            res[key] = make(series, LX, LY, family)
     return res
 
 
@@ -149,51 +154,67 @@ assemble_energy = assemble
 
 
 def summary(res: dict) -> str:
-    """The reference's summary lines (``code/measure/assemble.py:330-368``)."""
-    lines = [f"{res.get('state', '')} {res['LX']}x{res['LY']}, stripe family {res['family']}"]
+    """Human-readable summary of assembled results, one line per observable."""
+    def pm(d, k, err, fmt):
+        return f"{d[k]:{fmt}} +- {d[err]:.1g}"
+
+    out = [f"[{res.get('state', '') or 'state'}] {res['LX']}x{res['LY']} lattice, stripe family {res['family']}"]
     if "energy" in res:
         e = res["energy"]
-        lines.append(f"E/N = {e['E_per_site']:.6f} +- {e['E_err']:.1e}, "
-                     f"sigma^2/N = {e['var_per_site']:.5f} +- {e['var_err']:.1e} ({e['nsweeps']} sweeps)")
-    if "dwave" in res:
-        d = res["dwave"]
-        lines.append(f"Delta^2_SC = {d['Delta2_SC']:.6f} +- {d['Delta2_SC_err']:.6f}, "
-                     f"Delta_SC = {d['Delta_SC']:.6f} +- {d['Delta_SC_err']:.6f} "
-                     f"(M = {d['n_far']} vectors with |r| >= {d['r_max']:.4f}, {d['nsweeps']} sweeps)")
+        out.append(f"  energy   E/N {pm(e, 'E_per_site', 'E_err', '.6f')}  var/N {pm(e, 'var_per_site', 'var_err', '.5f')}  "
+                   f"[{e['nsweeps']} sweeps]")
     if "spin" in res:
-        s = res["spin"]
-        lines.append(f"M^2_str = {s['M2_str']:.6f} +- {s['M2_str_err']:.6f} at k_alpha = {s['k_alpha'].tolist()}; "
-                     f"zz channel alone {s['M2_zz']:.6f}, C_s(0) = {s['C_s'][0]:.5f}")
+        sp = res["spin"]
+        out.append(f"  spin     M2_str {pm(sp, 'M2_str', 'M2_str_err', '.6f')}  M2_zz {sp['M2_zz']:.6f}  C_s(0) {sp['C_s'][0]:.5f}  "
+                   f"k_alpha {sp['k_alpha'].tolist()}")
     if "charge" in res:
         c = res["charge"]
-        lines.append(f"N2(q_c) = {c['N2']:.6f} +- {c['N2_err']:.6f} at q_c = {c['q_c'].tolist()}, "
-                     f"sum_r C_c(r) = {c['C_c'].sum():.1e}")
+        out.append(f"  charge   N2 {pm(c, 'N2', 'N2_err', '.6f')}  sum_r C_c {c['C_c'].sum():.1e}  q_c {c['q_c'].tolist()}")
+    if "dwave" in res:
+        d = res["dwave"]
+        out.append(f"  d-wave   Delta2_SC {pm(d, 'Delta2_SC', 'Delta2_SC_err', '.6f')}  Delta_SC {d['Delta_SC']:.6f}  "
+                   f"[{d['n_far']} displacements |r| >= {d['r_max']:.4f}, {d['nsweeps']} sweeps]")
     if "local" in res:
         loc = res["local"]
-        lines.append(f"max |<S^z_i>| = {loc['max_abs_sz']:.1e}, <n_i> in [{loc['n_min']:.4f}, {loc['n_max']:.4f}]")
-    return "\n".join(lines)
+        out.append(f"  local    max|<S^z_i>| {loc['max_abs_sz']:.1e}  <n_i> in [{loc['n_min']:.4f}, {loc['n_max']:.4f}]")
+    return "\n".join(out)
+
+
+def _to_json(v):
+    """JSON-ready copy: numpy scalars to Python numbers, non-finite floats to None, keys to str."""
+    if isinstance(v, dict):
+        return {str(k): _to_json(x) for k, x in v.items()}
+    v = v.item() if isinstance(v, np.generic) else v
+    return None if isinstance(v, float) and not np.isfinite(v) else v
+
+
+def split_results(res: dict):
+    """(scalars, maps) of assembled results: the maps are the arrays of rank >= 1 one level down
+    (results.npz keys are their inner names), everything else stays in the scalar tree."""
+    scalars, maps = {}, {}
+    for key, val in res.items():
+        if not isinstance(val, dict):
+            scalars[key] = _to_json(val)
+            continue
+        inner = scalars.setdefault(key, {})
+        for k, v in val.items():
+            if isinstance(v, np.ndarray) and v.ndim:
+                maps[k] = v
+            else:
+                inner[k] = _to_json(v)
+    return scalars, maps
 
 
 def save_results(res: dict, out_dir: str) -> None:
-    """results.json (scalars, non-finite floats as null) and results.npz (the maps) in out_dir
-    (``code/measure/assemble.py:383-399``)."""
+    """results.json (scalars; NaN and inf written as null) and results.npz (the maps) in out_dir."""
     import json
     import os
 
-    scalars, arrays = {}, {}
-    for key, val in res.items():
-        if isinstance(val, dict):
-            scalars[key] = {}
-            for k, v in val.items():
-                if isinstance(v, np.ndarray) and v.ndim > 0:
-                    arrays[k] = v
-                else:
-                    scalars[key][k] = _json_value(v)
-        else:
-            scalars[key] = _json_value(val)
+    scalars, maps = split_results(res)
+    text = json.dumps(scalars, indent=1)  # every non-finite float is None by now
     with open(os.path.join(out_dir, "results.json"), "w") as fh:
-        json.dump(scalars, fh, indent=1, allow_nan=False)
-    np.savez(os.path.join(out_dir, "results.npz"), **arrays)
+        fh.write(text)
+    np.savez(os.path.join(out_dir, "results.npz"), **maps)
 
 
 def run_measure(state, H, sampler, nsweeps: int, out_dir: str, name: str = "", tprime: float = 0.0,

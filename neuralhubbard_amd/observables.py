@@ -38,8 +38,10 @@ def _torch():
 
 
 def reference_sites(LX: int, LY: int, nref: int):
-    """nref reference sites spread along the diagonal (``observables.reference_sites``)."""
-    return [(x, x % LY) for x in range(0, LX, max(LX // nref, 1))][:nref]
+    """nref reference sites (x, x mod LY) on the diagonal, x every max(LX // nref, 1) rows from 0
+    (the reference's choice of sites for the transverse terms)."""
+    step = max(LX // nref, 1)
+    return [(int(x), int(x % LY)) for x in np.arange(0, LX, step)[:nref]]
 
 
 def sz_grid(spins, N: int):
@@ -158,38 +160,60 @@ def spin_correlation(state, spins, psi, LX: int, LY: int, refs=None):
     return cs.mean(0), autocorr(sz_grid(spins, LX * LY), LX, LY)
 
 
-FAMILIES = {"pi8": (1, 8), "pi4": (1, 4)}  # stripe wavevector q / pi (``observables.paper_family``)
+FAMILIES = {"pi8": (1, 8), "pi4": (1, 4)}  # stripe wavevector q / pi of each family
+
+
+def _grid_q(LX: int, LY: int, family: str):
+    """q of the family as integer momenta (qx, qy) on the LX x LY grid (q L / 2 pi)."""
+    from fractions import Fraction
+
+    q = Fraction(*FAMILIES[family]) / 2  # q / 2 pi
+    mx, my = q * LX, q * LY
+    if mx.denominator != 1 or my.denominator != 1:
+        raise ValueError(f"q = {FAMILIES[family][0]}/{FAMILIES[family][1]} pi is not on the momentum grid of {LX}x{LY}")
+    return int(mx), int(my)
 
 
 def stripe_k(LX: int, LY: int, family: str):
-    """Integer momenta of k_alpha = (pi -+ q, pi), (pi, pi -+ q) (``observables.stripe_k``)."""
-    num, den = FAMILIES[family]
-    if (num * LX) % (2 * den) or (num * LY) % (2 * den):
-        raise ValueError(f"q = {num}/{den} pi is not on the momentum grid of {LX}x{LY}")
-    nx, ny = num * LX // (2 * den), num * LY // (2 * den)
-    hx, hy = LX // 2, LY // 2
-    return ((hx - nx, hy), (hx + nx, hy), (hx, hy - ny), (hx, hy + ny))
+    """The four stripe momenta k_alpha = (pi - q, pi), (pi + q, pi), (pi, pi - q), (pi, pi + q)
+    as integer grid momenta."""
+    qx, qy = _grid_q(LX, LY, family)
+    px, py = LX // 2, LY // 2
+    return ((px - qx, py), (px + qx, py), (px, py - qy), (px, py + qy))
 
 
 def charge_q(LX: int, LY: int, family: str):
-    """Integer momenta of the charge wavevectors +-(2q, 0), +-(0, 2q) (``observables.charge_q``)."""
-    num, den = FAMILIES[family]
-    if (num * LX) % (2 * den) or (num * LY) % (2 * den):
-        raise ValueError(f"q = {num}/{den} pi is not on the momentum grid of {LX}x{LY}")
-    nx, ny = 2 * (num * LX // (2 * den)), 2 * (num * LY // (2 * den))
-    return ((nx, 0), (LX - nx, 0), (0, ny), (0, LY - ny))
+    """The four charge momenta (2q, 0), (-2q, 0), (0, 2q), (0, -2q) as integer grid momenta in
+    [0, L)."""
+    qx, qy = _grid_q(LX, LY, family)
+    return ((2 * qx, 0), (LX - 2 * qx, 0), (0, 2 * qy), (0, LY - 2 * qy))
+
+
+def _phases(L: int):
+    k = np.arange(L)
+    return np.exp(2j * np.pi * np.outer(k, k) / L)  # [k, x] -> e^{i 2 pi k x / L}
 
 
 def structure_factor(C, LX: int, LY: int):
-    """S(k) = sum_r e^{ik.r} C(r) on the momentum grid, (..., LX, LY) (``assemble.structure_factor``)."""
-    C = np.asarray(C, dtype=np.float64)
-    return (np.fft.ifft2(C.reshape(*C.shape[:-1], LX, LY), axes=(-2, -1)) * (LX * LY)).real
+    """S(k) = Re sum_r e^{i k.r} C(r) for every grid momentum, C (..., N) with r = rx LY + ry ->
+    (..., LX, LY)."""
+    C = np.asarray(C, np.float64).reshape(*np.shape(C)[:-1], LX, LY)
+    return np.einsum("ax,...xy,by->...ab", _phases(LX), C, _phases(LY)).real
+
+
+def stripe_weights(LX: int, LY: int, momenta):
+    """w (N,) with w . C = (1/4N) sum_alpha S(k_alpha): the mean of cos(k_alpha . r) / N over the
+    momenta (S(k) is the real part, so only the cosines of k . r enter)."""
+    rx, ry = np.divmod(np.arange(LX * LY), LY)
+    w = np.zeros(LX * LY)
+    for kx, ky in momenta:
+        w += np.cos(2 * np.pi * (kx * rx / LX + ky * ry / LY))
+    return w / (len(momenta) * LX * LY)
 
 
 def m2_str(C, LX: int, LY: int, family: str):
-    """M^2_str = (1/4N) sum_alpha S(k_alpha) (``assemble.family_mean`` of ``assemble.spin``)."""
-    Sk = structure_factor(C, LX, LY)
-    return np.mean([Sk[..., kx % LX, ky % LY] for kx, ky in stripe_k(LX, LY, family)], axis=0) / (LX * LY)
+    """Stripe order M^2_str = (1/4N) sum_alpha S(k_alpha) of C (..., N)."""
+    return np.asarray(C, np.float64) @ stripe_weights(LX, LY, stripe_k(LX, LY, family))
 
 
 # ------------------------------------------------------------------------ d-wave pair correlation
@@ -329,69 +353,84 @@ def dwave_measure(state, spins, psi, LX: int, LY: int, chunk: int = 512):
     return F / B, G / B
 
 
+def _bonds_from(x, y, d, LX: int, LY: int):
+    """Canonical bonds (lower end x, y; axis) from the sites (x, y) along the unit vector d, and
+    their upper ends; x, y arrays."""
+    if d[0] + d[1] < 0:  # the bond points down or left: its lower end is the neighbour
+        x, y = (x + d[0]) % LX, (y + d[1]) % LY
+    ax = 0 if d[0] else 1
+    return x, y, ax, ((x + 1 - ax) % LX, (y + ax) % LY)
+
+
 class DwaveIndex:
-    """Index tables of C_p(r) over the measured terms of `dwave_local` (the role of
-    ``assemble.DwaveIndex``): terms of left bonds or sites other than the measured ones come from
-    the measured ones by translation (exact for translation-invariant states)."""
+    """C_p(r) from the measured term means: F (the pair terms of the +x and +y bonds at
+    R = (0, 0) with every right bond, dwave_keys' order) and G (<c+_i c_j> for the three ends of
+    those bonds).  Terms of other left bonds and sites come from the measured ones by translation
+    (exact for translation-invariant states).  For each of the 16 pairs of bond directions
+    (d_l, d_r) and every r the columns are computed once, vectorised over r:
+      F column (2 l + o_l) 4N + 4 site(r_bond) + 2 axis + o_r, the right bond shifted with the
+      left one onto the measured bond l of the left bond's axis;
+      G column (3 l_i ... ) of (i, j, s): (index of i among the measured sites, j) or, for any
+      other i, (0, j - i)."""
 
     def __init__(self, LX: int, LY: int):
+        self.LX, self.LY = LX, LY
         N = self.N = LX * LY
-        lefts = _left_bonds()
-        lsites = _left_sites(LY)
+        rx, ry = np.divmod(np.arange(N), LY)
+        measured_sites = {0: 0, 1: 1, LY: 2}  # (0, 0), (0, 1), (1, 0)
 
-        def right_index(bond, o):
-            x, y, ax = bond
-            return 2 * (2 * (x * LY + y) + ax) + o
+        def gcol(ix, iy, jx, jy, spin):
+            isite = ix * LY + iy
+            li = np.vectorize(lambda v: measured_sites.get(int(v), -1))(np.atleast_1d(isite))
+            moved = li < 0
+            jx = np.where(moved, (jx - ix) % LX, jx)
+            jy = np.where(moved, (jy - iy) % LY, jy)
+            li = np.where(moved, 0, li)
+            return (li * N + jx * LY + jy) * 2 + spin
 
-        def fcol(bl, oL, br, oR):
-            l = next(q for q, b in enumerate(lefts) if b[2] == bl[2])
-            tx, ty = lefts[l][0] - bl[0], lefts[l][1] - bl[1]
-            brt = ((br[0] + tx) % LX, (br[1] + ty) % LY, br[2])
-            return (2 * l + oL) * (4 * N) + right_index(brt, oR)
-
-        def gcol(i, j, s):
-            ii = i[0] * LY + i[1]
-            if ii in lsites:
-                return (lsites.index(ii) * N + j[0] * LY + j[1]) * 2 + s
-            jt = ((j[0] - i[0]) % LX, (j[1] - i[1]) % LY)
-            return (0 * N + jt[0] * LY + jt[1]) * 2 + s
-
-        self.fidx = np.empty((N, 16, 4), np.int64)
-        self.gidx = np.empty((N, 16, 4, 2), np.int64)
-        self.ff = np.empty(16)
-        for n in range(N):
-            r2 = divmod(n, LY)
-            m = 0
-            for dl in DIRS:
-                bl = _canon_bond((0, 0), dl, LX, LY)
-                a, b = _bond_sites(bl, LX, LY)
-                for d2 in DIRS:
-                    br = _canon_bond(r2, d2, LX, LY)
-                    c, d = _bond_sites(br, LX, LY)
-                    self.ff[m] = FORM_FACTOR[dl] * FORM_FACTOR[d2]
-                    self.fidx[n, m] = [fcol(bl, oL, br, oR) for oL in (0, 1) for oR in (0, 1)]
-                    self.gidx[n, m] = [[gcol(a, c, 0), gcol(b, d, 1)], [gcol(a, d, 0), gcol(b, c, 1)],
-                                       [gcol(b, c, 0), gcol(a, d, 1)], [gcol(b, d, 0), gcol(a, c, 1)]]
-                    m += 1
+        self.pairs = []  # (form factor, F columns (4, N), G columns (4, 2, N)) per direction pair
+        for dl in DIRS:
+            lx, ly, lax, (ux, uy) = _bonds_from(np.zeros(1, int), np.zeros(1, int), dl, LX, LY)
+            left = ((lx, ly), (ux, uy))  # ends (a, b) of the left bond
+            for dr in DIRS:
+                cx, cy, rax, (dx_, dy_) = _bonds_from(rx, ry, dr, LX, LY)
+                right = ((cx, cy), (dx_, dy_))
+                tx, ty = (cx - lx) % LX, (cy - ly) % LY  # right bond translated with the left one
+                fcols, gcols = [], []
+                for ol in (0, 1):
+                    for orr in (0, 1):
+                        fcols.append((2 * lax + ol) * 4 * N + 4 * (tx * LY + ty) + 2 * rax + orr)
+                        # Wick pairing of A+(bl, ol) A(br, or): the up electron created at the first
+                        # site of the orientation meets the up electron of the right pair, likewise down
+                        (x1, y1), (x2, y2) = left if ol == 0 else left[::-1]
+                        (z1, w1), (z2, w2) = right if orr == 0 else right[::-1]
+                        gcols.append(np.stack([gcol(x1, y1, z1, w1, 0), gcol(x2, y2, z2, w2, 1)]))
+                self.pairs.append((FORM_FACTOR[dl] * FORM_FACTOR[dr], np.stack(fcols), np.stack(gcols)))
 
     def pair_correlation(self, F, G, connected=True):
         """C_p(r) (..., N) from term means F (..., 4, 4N) and G (..., 3, N, 2), or flattened
-        (..., 16N) and (..., 6N)."""
+        (..., 16N) and (..., 6N): (1/16) sum over direction pairs of h_l h_r [1/2 sum over the four
+        orientations of F - 1/2 sum of their Wick products G_up G_dn] (the second part with
+        ``connected`` only)."""
         N = self.N
         F = np.asarray(F, np.float64)
         if F.shape[-1] != 16 * N:
             if F.shape[-2:] != (4, 4 * N):
                 raise ValueError(f"F of shape {F.shape}: expected (..., 4, {4 * N}) or (..., {16 * N})")
-            F = F.reshape(*F.shape[:-2], -1)
-        out = 0.5 * F[..., self.fidx].sum(-1)
+            F = F.reshape(*F.shape[:-2], 16 * N)
         if connected:
             G = np.asarray(G, np.float64)
             if G.shape[-1] != 6 * N:
                 if G.shape[-3:] != (3, N, 2):
                     raise ValueError(f"G of shape {G.shape}: expected (..., 3, {N}, 2) or (..., {6 * N})")
-                G = G.reshape(*G.shape[:-3], -1)
-            out = out - 0.5 * (G[..., self.gidx[..., 0]] * G[..., self.gidx[..., 1]]).sum(-1)
-        return (out * self.ff).sum(-1) / 16
+                G = G.reshape(*G.shape[:-3], 6 * N)
+        total = 0.0
+        for h, fcols, gcols in self.pairs:
+            term = F[..., fcols].sum(axis=-2)
+            if connected:
+                term = term - (G[..., gcols[:, 0]] * G[..., gcols[:, 1]]).sum(axis=-2)
+            total = total + 0.5 * h * term
+        return total / 16
 
 
 def dwave_keys(LX: int, LY: int):
@@ -405,43 +444,44 @@ def dwave_keys(LX: int, LY: int):
 
 
 def far_mask(LX: int, LY: int):
-    """(mask of |r| >= r_max, r_max = d_max / 2) over displacements (``assemble.far_mask``)."""
-    dx, dy = np.divmod(np.arange(LX * LY), LY)
-    dist = np.hypot(np.minimum(dx, LX - dx), np.minimum(dy, LY - dy))
-    r_max = dist.max() / 2
-    return dist >= r_max - 1e-9, float(r_max)
+    """(mask of the displacements with |r| >= r_max, r_max) with |r| the minimum-image length and
+    r_max half the largest of them."""
+    rx, ry = np.divmod(np.arange(LX * LY), LY)
+    length = np.sqrt(np.minimum(rx, LX - rx) ** 2 + np.minimum(ry, LY - ry) ** 2)
+    r_max = 0.5 * length.max()
+    return length >= r_max - 1e-9, float(r_max)
 
 
 def dwave(F, G, LX: int, LY: int) -> dict:
-    """C_p(r) with leave-one-out errors, Delta^2_SC = mean of C_p over the far displacements and
-    Delta_SC with block jackknife errors, from per-sweep term means F (S, 4, 4N) and G (S, 3, N, 2)
-    (``assemble.dwave``)."""
-    from .measure import BLOCK_SIZES, MIN_BLOCKS
+    """C_p(r) with its delete-one-sweep error, Delta^2_SC (the mean of C_p over the far
+    displacements) and Delta_SC = sqrt(Delta^2_SC) with block-jackknife errors, from per-sweep term
+    means F (S, 4, 4N) and G (S, 3, N, 2)."""
+    from .measure import delete_one_means, jackknife
 
     N = LX * LY
     F = np.asarray(F, np.float64).reshape(-1, 16 * N)
     G = np.asarray(G, np.float64).reshape(-1, 6 * N)
+    n = F.shape[0]
     idx = DwaveIndex(LX, LY)
     far, r_max = far_mask(LX, LY)
-    n = F.shape[0]
-    Cp = idx.pair_correlation(F.mean(0), G.mean(0))
+    Cp = idx.pair_correlation(F.mean(axis=0), G.mean(axis=0))
     if n > 1:
-        loo = idx.pair_correlation((F.sum(0) - F) / (n - 1), (G.sum(0) - G) / (n - 1))
-        Cp_err = np.sqrt(n - 1) * loo.std(axis=0)
+        reps = idx.pair_correlation(delete_one_means(F, 1), delete_one_means(G, 1))
+        Cp_err = np.sqrt((n - 1) / n * ((reps - reps.mean(axis=0)) ** 2).sum(axis=0))
     else:
         Cp_err = np.full_like(Cp, np.nan)
-    d2 = float(Cp[far].mean())
-    d2_errs, d_errs = {}, {}
-    for b in BLOCK_SIZES:
-        nb = n // b
-        if nb < MIN_BLOCKS:
-            continue
-        Fb = F[:nb * b].reshape(nb, b, -1).mean(1)
-        Gb = G[:nb * b].reshape(nb, b, -1).mean(1)
-        reps = idx.pair_correlation((Fb.sum(0) - Fb) / (nb - 1), (Gb.sum(0) - Gb) / (nb - 1))[:, far].mean(-1)
-        d2_errs[b] = float(np.sqrt(nb - 1) * reps.std())
-        d_errs[b] = float(np.sqrt(nb - 1) * np.sqrt(reps).std()) if reps.min() > 0 else float("nan")
-    return {"C_p": Cp, "C_p_err": Cp_err, "Delta2_SC": d2, "Delta2_SC_err": max(d2_errs.values(), default=float("nan")),
-            "Delta_SC": float(np.sqrt(d2)) if d2 > 0 else float("nan"),
-            "Delta_SC_err": max(d_errs.values(), default=float("nan")), "Delta2_SC_err_blocks": d2_errs,
-            "r_max": r_max, "n_far": int(far.sum()), "nsweeps": n}
+
+    def far_mean(f, g):
+        return idx.pair_correlation(f, g)[far].mean()
+
+    def root(f, g):
+        v = far_mean(f, g)
+        return np.sqrt(v) if v > 0 else np.nan
+
+    d2, d2_err, d2_blk = jackknife(far_mean, F, G)
+    _, d_err, _ = jackknife(root, F, G)
+    d2 = float(d2)
+    return {"C_p": Cp, "C_p_err": Cp_err, "Delta2_SC": d2, "Delta2_SC_err": float(d2_err),
+            "Delta_SC": float(np.sqrt(d2)) if d2 > 0 else float("nan"), "Delta_SC_err": float(d_err),
+            "Delta2_SC_err_blocks": {int(b): float(e) for b, e in d2_blk.items()}, "r_max": r_max,
+            "n_far": int(far.sum()), "nsweeps": int(n)}

@@ -153,7 +153,7 @@ def test_all_channel_files(tmp_path):
     assert js["charge"]["N2"] == pytest.approx(again["charge"]["N2"], rel=1e-12)
     assert js["local"]["n_min"] <= 56 / 64 <= js["local"]["n_max"]
     assert res["energy"]["nsweeps"] == S
-    assert any(line.startswith("Delta^2_SC") for line in logs[-1].splitlines())
+    assert any(line.strip().startswith("d-wave   Delta2_SC") for line in logs[-1].splitlines())
 
 
 def test_spin_assembly_matches_direct_sums():
