@@ -1,32 +1,41 @@
 """Accepted-sample throughput of the 16x16 Pfaffian backflow path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--sweep P]
-                    [--form both|plain|mixed-da] [--no-cpu] [--workload pf16|tpm02_L16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains C] [--sweep P] [--slice S]
+                    [--measure M] [--form both|plain|mixed-da] [--no-cpu] [--workload pf16|tpm02_L16]
 
 Workload: the 16x16 Pfaffian backflow (t' = -0.2, U = 8, 112 + 112 electrons, 224 x 224
-Pfaffians, 6-layer 32-channel conv, 8-image projection) with seeded random-init weights
-(``--workload tpm02_L16``: the released tpm02_L16_pfaffian state instead, 16-block 64-channel
-residual CNN, chains started from its released chains).  C
-Metropolis chains live in HBM.  One step = one sweep of P exchange proposals per chain, then
-log psi_P and E_loc (every connected configuration evaluated in full) of every chain's state:
-C measured samples per step.  Two forms run from the same thermalised chains:
+Pfaffians, 6-layer 32-channel conv, 8-image projection) with seeded random-init weights, run as
+delayed-acceptance Metropolis with 8192 chains resident in HBM and sweeps of 512 exchange
+proposals (``--workload tpm02_L16``: the released tpm02_L16_pfaffian state instead).  The chains
+start from an equilibrated ensemble of this workload (tests/golden/pf16_chains_8192.npz, written
+by the path's own exact sampler; the random-init state relaxes over ~300 sweeps).
+
+One step = a slice of S = 16 exchange proposals for all C chains, then log psi_P (carried by
+the chain) and E_loc (every connected configuration evaluated in full) of the M = C * S / P = 256
+chains whose turn it is: P / S = 32 steps are one sweep of every chain and one measured sample
+per chain, as in a measurement run of 8192 chains x 512-proposal sweeps.  Two forms run from the
+same ensemble:
 
   plain      FP32 feature network and Pfaffians, plain Metropolis;
-  mixed-da   split-BF16 MFMA feature network (FP32 pairing/Pfaffian) with delayed acceptance
-             screened by the conv surrogate, trained beforehand on the GPU on (configuration,
-             exact log|psi|) pairs recorded while thermalising (training time reported apart).
+  mixed-da   split-FP16 MFMA feature network (FP32 pairing / Pfaffian) with delayed acceptance
+             screened by the conv surrogate, trained beforehand on the GPU by the library's own
+             forward / backward kernels on 2^20 (configuration, exact log|psi|) pairs recorded
+             from the chains (recording and training time reported apart).
 
-value = samples/s of mixed-da (with --form both or mixed-da), inputs resident in HBM.  The JSON
-line also carries the plain FP32 rate and the ratio, acceptance statistics, the surrogate's RMS
-log-ratio error, E/N of both forms with block-jackknife errors, log|psi| and sign agreement
-with the FP64 oracle on a few of the sampled configurations, and
+value = samples/s of mixed-da (or of the one form run), inputs resident in HBM.  The line also
+carries per form: samples/s, accepted moves/s, an autocorrelation-aware effective sample rate,
+acceptance statistics, E/N with block-jackknife errors, the drift of the ensemble over the timed
+steps; the ratio to plain FP32; energy_within_error_bars (flagged on stderr when false); the
+surrogate's RMS log-ratio error; log|psi| / sign / E_loc against the FP64 C restatement on
+sampled configurations, and
 
   roofline      the dominant kernel class of the headline form, its algorithmic flops per
                 launch over the average launch time measured with HIP events on its stream,
                 against the MI355X dense peak of its arithmetic type; traffic from the
-                committed rocprofv3 counter summary when present;
-  cpu_baseline  the FP32 C restatement of the reference CPU path (oracle/nh_cpu.c, OpenMP) on
-                a bounded sample of the same workload on this host's cores.
+                committed rocprofv3 counter summary;
+  cpu_baseline  the FP32 C restatement of the reference CPU path (oracle/nh_cpu.c, OpenMP over
+                all host cores) timed on a bounded sample of the same workload: E_loc plus the
+                proposals' amplitude evaluations of one measured sample.
 """
 
 from __future__ import annotations
@@ -44,7 +53,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "samples/s (log-psi+E_loc, 16x16 Pfaffian): mixed+surrogate vs plain FP32"
-PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}
+PEAK = {"fp32": 157.3e12, "fp16": 2.5e15, "bf16": 2.5e15}
+CHAINS_FILE = os.path.join(ROOT, "tests", "golden", "pf16_chains_8192.npz")
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_image_pf_pf16_hbm_traffic.md")
 TRAFFIC_SUMMARY_RESCONV = os.path.join(ROOT, "profiles", "k_resconv_bf16_tpm02_L16_hbm_traffic.md")
 
@@ -54,13 +64,13 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=8)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--chains", type=int, default=256)
-    p.add_argument("--sweep", type=int, default=512, help="exchange proposals per chain per step")
+    p.add_argument("--chains", type=int, default=8192)
+    p.add_argument("--sweep", type=int, default=512, help="exchange proposals per chain between its samples")
+    p.add_argument("--slice", type=int, default=16, help="exchange proposals per chain per step")
     p.add_argument("--form", default="both", choices=["both", "plain", "mixed-da"])
-    p.add_argument("--thermal", type=int, default=256, help="plain proposals per chain before recording")
-    p.add_argument("--records", type=int, default=256, help="recorded states per chain for the surrogate")
-    p.add_argument("--record-gap", type=int, default=4)
-    p.add_argument("--epochs", type=int, default=8)
+    p.add_argument("--pairs", type=int, default=1 << 20, help="(configuration, exact log|psi|) pairs for the surrogate")
+    p.add_argument("--record-gap", type=int, default=4, help="proposals between recorded states")
+    p.add_argument("--epochs", type=int, default=3)
     p.add_argument("--chunk", type=int, default=16, help="E_loc samples per chunk")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 checks")
     p.add_argument("--cpu-samples", type=int, default=12)
@@ -90,30 +100,63 @@ def read_traffic(images_per_launch, summary=TRAFFIC_SUMMARY):
         return None
 
 
-def cpu_leg(wk, params, spins, gpu_psi, gpu_eloc, per_call=2):
-    """CPU baseline (FP32 C port, timed) and the FP64 oracle as checker of the GPU amplitudes."""
+def decorrelation(series, slice_props, sweep):
+    """Autocorrelation of a per-chain observable sampled every `slice_props` proposals (series
+    [steps, chains]): rho(lag) averaged over chains, its exponential time tau (in proposals, from
+    a least-squares fit of log rho over the lags with rho > 0.05), and the factor by which
+    measuring every `sweep` proposals inflates the variance of a mean, 1 + 2 sum_k rho(k sweep)
+    (geometric series of the fitted decay), whose inverse scales samples/s to effective samples/s."""
+    import numpy as np
+
+    x = np.asarray(series, np.float64)
+    x = x - x.mean(axis=0, keepdims=True)
+    var = (x * x).mean()
+    if x.shape[0] < 3 or var == 0:
+        return {"rho": [], "tau_proposals": None, "variance_inflation": None}
+    rho = [float((x[:-k] * x[k:]).mean() / var) for k in range(1, x.shape[0] // 2 + 1)]
+    lags = np.arange(1, len(rho) + 1) * slice_props
+    ok = np.array(rho) > 0.05
+    if ok.sum() >= 2:
+        slope = np.polyfit(lags[ok], np.log(np.array(rho)[ok]), 1)[0]
+        tau = float(-1.0 / slope) if slope < 0 else float("inf")
+    else:
+        tau = float(slice_props) / 3.0  # decorrelated within one slice
+    r = float(np.exp(-sweep / tau)) if np.isfinite(tau) else 1.0
+    infl = (1 + r) / (1 - r) if r < 1 else float("inf")
+    return {"rho": rho[:6], "tau_proposals": tau, "variance_inflation": infl}
+
+
+def cpu_leg(wk, params, spins, gpu_psi, gpu_eloc, props_exact):
+    """CPU baseline (FP32 C port, timed on all host cores) and the FP64 oracle as checker."""
     import numpy as np
 
     from oracle.cpu_port import CpuPath
 
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores = len(os.sched_getaffinity(0))
     out = {}
     cp = CpuPath(wk, params, bits=32, threads=cores)
     x = spins
     cp.psi(x[:1])  # warm-up (page-in, thread pool)
-    # per_call configurations at a time (2: 16 images keep 16 threads busy) until about 15 s
+    # E_loc of sampled configurations, two at a time, for about 15 s
     t0 = time.perf_counter()
     n_done = 0
     while n_done < len(x) and (time.perf_counter() - t0) < 15.0:
-        xi = x[n_done:n_done + per_call]
+        xi = x[n_done:n_done + 2]
         sg, la = cp.psi(xi)
         cp.local_energy(xi, sg, la)
         n_done += len(xi)
-    dt = time.perf_counter() - t0
-    out["cpu_baseline"] = {"value": n_done / dt, "unit": "samples/s", "cores": cores, "kind": "port",
-                           "sample": f"{n_done} sampled configurations of the same workload, {per_call} at a time: "
-                                     f"projected psi + full E_loc, no sweep ({dt:.1f} s on {cores} threads, "
-                                     f"oracle/nh_cpu.c in FP32)"}
+    t_eloc = (time.perf_counter() - t0) / n_done
+    # projected amplitudes of proposal configurations (the chain's exact evaluations)
+    m = 4 * cores
+    t0 = time.perf_counter()
+    cp.psi(np.resize(x, (m, x.shape[1])))
+    t_psi = (time.perf_counter() - t0) / m
+    per_sample = t_eloc + props_exact * t_psi
+    out["cpu_baseline"] = {"value": 1.0 / per_sample, "unit": "samples/s", "cores": cores, "kind": "port",
+                           "sample": f"E_loc of {n_done} sampled configurations ({t_eloc:.2f} s each) plus "
+                                     f"{props_exact:.0f} projected amplitudes per sample for the sweep's exact "
+                                     f"evaluations ({t_psi * 1e3:.1f} ms each, timed on {m}), oracle/nh_cpu.c "
+                                     f"in FP32 on {cores} threads"}
     ref = CpuPath(wk, params, bits=64, threads=cores)
     sg64, la64 = ref.psi(x)
     acc = {}
@@ -149,77 +192,98 @@ def main():
         return
 
     from neuralhubbard_amd import _lib
-    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow, hops
-    from neuralhubbard_amd.configs import WORKLOADS, make_configurations, make_params
+    from neuralhubbard_amd.backflow import Hubbard, LogArray, Samples, Variational, build_backflow, hops
+    from neuralhubbard_amd.configs import WORKLOADS, make_params
     from neuralhubbard_amd.measure import energy, within_error_bars
+    from neuralhubbard_amd.released import unpack_spins
     from neuralhubbard_amd.sampler import MetropolisSampler, Surrogate
 
     lib = _lib.load()
-    C = args.chains
+    C, S, P = args.chains, args.slice, args.sweep
+    if P % S or C % (P // S):
+        sys.exit("--sweep must be a multiple of --slice, and --chains a multiple of sweep / slice")
+    M = C // (P // S)  # chains measured per step
     if args.workload == "tpm02_L16":
         from neuralhubbard_amd.configs import Params
-        from neuralhubbard_amd.released import STATES, unpack_spins
+        from neuralhubbard_amd.released import STATES
 
         wk = STATES["tpm02_L16_pfaffian"].workload()
         with np.load(args.weights, allow_pickle=False) as f:
             params = Params(list(f["conv_w"]), list(f["conv_b"]), f["W"], float(f["theta"]), None, None, f["emb"])
             released = unpack_spins(f["bits"], f["shape"])
-        start0 = np.resize(released, (C, 2 * wk.N))
+        if C > len(released):
+            sys.exit(f"--chains {C} exceeds the {len(released)} released chains")
+        start0 = released[:C]
         data = "released tpm02_L16_pfaffian parameters; chains started from its released equilibrated chains"
         net = "16-block 64-channel residual CNN (released tpm02_L16_pfaffian)"
     else:
         wk = WORKLOADS["pf16"]
         params = make_params(wk)
-        start0 = make_configurations(wk, C)
-        data = ("synthetic: seeded random configurations thermalised by the path's own Metropolis chains, "
-                "random-init weights of the workload's architecture")
+        with np.load(CHAINS_FILE, allow_pickle=False) as f:
+            ens = unpack_spins(f["bits"], f["shape"])
+            eq_sweeps = len(f["acceptance"])
+        if C > len(ens):
+            sys.exit(f"--chains {C} exceeds the {len(ens)} chains of {CHAINS_FILE}")
+        start0 = ens[:C]
+        data = (f"synthetic: random-init weights of the workload's architecture (seeded); chains from an "
+                f"ensemble equilibrated by the path's own exact sampler over {eq_sweeps} sweeps")
         net = "6-layer 32-channel conv"
     resconv = wk.network == "resconv"
+    N = wk.N
     forms = ["plain", "mixed-da"] if args.form == "both" else [args.form]
     states = {"plain": Variational(build_backflow(wk, params, "fp32")),
               "mixed-da": Variational(build_backflow(wk, params, "mixed"))}
     H = Hubbard(wk.U, 1.0, wk.tprime, chunk=args.chunk)
 
-    # thermalise with exact plain Metropolis on the mixed-precision amplitude and record the
-    # (configuration, exact log|psi|) pairs the surrogate learns from
-    t0 = time.perf_counter()
-    rec = MetropolisSampler(states["mixed-da"], start0, args.record_gap, seed=1,
-                            thermal_steps=args.thermal)
-    X, Y = [], []
-    for _ in range(args.records):
-        smp = rec.sweep()
-        X.append(smp.spins)
-        Y.append(smp.psi.logabs)
-    X = torch.cat(X).cpu().numpy()
-    Y = torch.cat(Y).cpu().numpy()
-    t_record = time.perf_counter() - t0
-    log(f"thermalised + recorded {len(X)} pairs in {t_record:.1f} s")
+    # (configuration, exact log|psi|) pairs for the surrogate: exact Metropolis on the mixed
+    # amplitude, one recorded state per chain every record_gap proposals
     sur = None
     sur_info = None
+    start = torch.as_tensor(start0, device="cuda")
     if "mixed-da" in forms:
-        ntr = int(0.9 * len(X))
+        t0 = time.perf_counter()
+        rec = MetropolisSampler(states["mixed-da"], start, args.record_gap, seed=1)
+        X, Yl = [], []
+        n_rec = max(1, args.pairs // C)
+        for _ in range(n_rec):
+            smp = rec.sweep()
+            X.append(smp.spins)
+            Yl.append(smp.psi.logabs)
+        X = torch.cat(X).cpu().numpy()
+        Yl = torch.cat(Yl).cpu().numpy()
+        t_record = time.perf_counter() - t0
+        log(f"recorded {len(X)} pairs in {t_record:.1f} s")
+        ntr = int(0.95 * len(X))
         sur = Surrogate(wk.LX, wk.LY, 32, seed=2)
         t0 = time.perf_counter()
-        hist = sur.train(X[:ntr], Y[:ntr], epochs=args.epochs, batch=512, lr=2e-3)
+        hist = sur.train(X[:ntr], Yl[:ntr], epochs=args.epochs, batch=1024, lr=2e-3,
+                         log=lambda m: log(m))
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         held = sur(X[ntr:]).cpu().numpy()
-        sur_info = {"pairs": int(ntr), "epochs": args.epochs, "train_s": t_train, "train_mse": hist[-1],
-                    "heldout_mse": float(np.mean((held - Y[ntr:]) ** 2)), "target_var": float(Y.var())}
+        sur_info = {"pairs": int(ntr), "heldout": int(len(X) - ntr), "epochs": args.epochs, "record_s": t_record,
+                    "train_s": t_train, "train_mse": hist[-1], "heldout_mse": float(np.mean((held - Yl[ntr:]) ** 2)),
+                    "target_var": float(Yl.var()), "arithmetic": "fp32 (VALU kernels)"}
         log(f"surrogate trained in {t_train:.1f} s: {sur_info}")
-    start = rec.s.clone()
+        del X, Yl
+        start = rec.s.clone()
 
     results = {}
     prof_names = ((_lib.PROF_FEATURES, "features"), (_lib.PROF_PAIRING_LINALG, "pairing+pfaffian"),
                   (_lib.PROF_OTHER, "other"))
     for f in forms:
         st = states[f]
-        sm = MetropolisSampler(st, start, args.sweep, seed=7, surrogate=sur if f == "mixed-da" else None)
+        sm = MetropolisSampler(st, start, S, seed=7, surrogate=sur if f == "mixed-da" else None)
+        turn = [0]
 
         def step():
-            smp = sm.sweep()
-            e = H.Oloc(st, smp)
-            return smp, e
+            sm.run(S)
+            j = turn[0] % (P // S)
+            turn[0] += 1
+            sl = slice(j * M, (j + 1) * M)
+            spins = sm.s[sl].clone()
+            smp = Samples(spins, LogArray(sm.sign[sl].clone(), sm.logabs[sl].clone()))
+            return smp, H.Oloc(st, smp)
 
         for i in range(args.warmup):
             step()
@@ -228,13 +292,15 @@ def main():
         torch.cuda.synchronize()
         lib.nh_profile_reset()
         lib.nh_profile_enable(1)
-        e_mean, e_sqr, spins = [], [], []
+        e_mean, e_sqr, spins, dser, lser = [], [], [], [], []
         t0 = time.perf_counter()
         for i in range(args.steps):
             smp, e = step()
             e_mean.append(e.mean())
             e_sqr.append((e * e).mean())
             spins.append(smp.spins)
+            dser.append(((sm.s[:, :N] > 0) & (sm.s[:, N:] > 0)).sum(1, dtype=torch.int32))
+            lser.append(sm.logabs.clone())
             log(f"{f}: step {i + 1}/{args.steps} issued")
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
@@ -249,9 +315,19 @@ def main():
         n_conn = sum(int(hops(x, wk.LX, wk.LY, 1.0, wk.tprime, wk.n_up + wk.n_dn)[0].sum().item()) for x in spins)
         e_mean = torch.stack(e_mean).cpu().numpy()
         e_sqr = torch.stack(e_sqr).cpu().numpy()
-        results[f] = {"rate": args.steps * C / dt, "ms_per_step": dt / args.steps * 1e3, "stats": stats,
-                      "kinds": kinds, "n_conn": n_conn, "energy": energy(e_mean, e_sqr, wk.N), "sampler": sm}
-        log(f"{f}: {results[f]['rate']:.2f} samples/s, {stats}, kernel ms {({k: v[1] for k, v in kinds.items()})}")
+        dser = torch.stack(dser).double().cpu().numpy()
+        lser = torch.stack(lser).cpu().numpy()
+        dec = decorrelation(dser, S, P)
+        rate = args.steps * M / dt
+        infl = dec["variance_inflation"]
+        drift = {"D_per_site": [float(dser[0].mean() / N), float(dser[-1].mean() / N)],
+                 "mean_logabs": [float(lser[0].mean()), float(lser[-1].mean())]}
+        results[f] = {"rate": rate, "ms_per_step": dt / args.steps * 1e3, "stats": stats, "kinds": kinds,
+                      "n_conn": n_conn, "energy": energy(e_mean, e_sqr, N), "sampler": sm,
+                      "accepted_per_s": stats["accepted"] / dt, "decorrelation": dec,
+                      "effective_rate": rate / infl if infl and np.isfinite(infl) else None, "drift": drift}
+        log(f"{f}: {rate:.2f} samples/s, {stats}, decorrelation {dec}, kernel ms "
+            f"{({k: v[1] for k, v in kinds.items()})}")
 
     head = "mixed-da" if "mixed-da" in forms else "plain"
     r = results[head]
@@ -260,7 +336,7 @@ def main():
     dom = max(flops, key=lambda k: r["kinds"][k][1])
     launches, ms = r["kinds"][dom]
     achieved = flops[dom] / (ms * 1e-3)
-    arith = "bf16" if (head == "mixed-da" and dom == "features") else "fp32"
+    arith = ("bf16" if resconv else "fp16") if (head == "mixed-da" and dom == "features") else "fp32"
     traffic = None
     if dom == "pairing+pfaffian" and not resconv:
         traffic = read_traffic(images / max(1, launches))
@@ -286,35 +362,48 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16+fp32" if head == "mixed-da" else "fp32",
+        "dtype": "fp16+fp32" if head == "mixed-da" else "fp32",
         "data": data,
         "config": {"workload": f"16x16 Pfaffian backflow, t'=-0.2, U=8, 112+112 electrons (224x224 Pfaffians), "
-                               f"{net}, 8-image projection; {C} chains, {args.sweep} exchange "
-                               f"proposals per chain per step, then log psi_P + full E_loc of every chain",
-                   "form": head, "chains": C, "sweep": args.sweep,
-                   "connected_per_sample": r["n_conn"] / (args.steps * C), "parallelism": "single GPU"},
+                               f"{net}, 8-image projection; {C} chains resident, sweeps of {P} exchange "
+                               f"proposals; one step = {S} proposals for every chain, then log psi_P + full E_loc "
+                               f"of the {M} chains whose turn it is ({P // S} steps = one sweep and one "
+                               f"measured sample per chain)",
+                   "form": head, "chains": C, "sweep": P, "slice": S, "measured_per_step": M,
+                   "connected_per_sample": r["n_conn"] / (args.steps * M), "parallelism": "single GPU"},
         "roofline": roof,
     }
     for f in forms:
         q = results[f]
+        st_ = q["stats"]
         out[f] = {"samples_per_s": q["rate"], "ms_per_step": q["ms_per_step"],
-                  "acceptance": q["stats"]["acceptance"], "exact_evaluations": q["stats"]["exact_evaluations"],
-                  "stage1_acceptance": q["stats"]["stage1_acceptance"],
-                  "stage2_acceptance": q["stats"]["stage2_acceptance"],
-                  "stage2_rejection": (1 - q["stats"]["stage2_acceptance"]) if q["stats"]["stage2_acceptance"]
-                  is not None else None,
-                  "E_per_site": q["energy"]["E_per_site"], "E_err": q["energy"]["E_err"]}
+                  "accepted_moves_per_s": q["accepted_per_s"],
+                  "effective_samples_per_s": q["effective_rate"],
+                  "decorrelation_D": q["decorrelation"],
+                  "acceptance": st_["acceptance"], "exact_evaluations": st_["exact_evaluations"],
+                  "stage1_acceptance": st_["stage1_acceptance"], "stage2_acceptance": st_["stage2_acceptance"],
+                  "stage2_rejection": (1 - st_["stage2_acceptance"]) if st_["stage2_acceptance"] is not None else None,
+                  "E_per_site": q["energy"]["E_per_site"], "E_err": q["energy"]["E_err"],
+                  "drift_over_timed_steps": q["drift"]}
     if len(forms) == 2:
         out["ratio_vs_plain_fp32"] = results["mixed-da"]["rate"] / results["plain"]["rate"]
-        out["energy_within_error_bars"] = within_error_bars(results["mixed-da"]["energy"], results["plain"]["energy"])
+        ep, em = results["plain"]["effective_rate"], results["mixed-da"]["effective_rate"]
+        out["effective_ratio_vs_plain_fp32"] = em / ep if ep and em else None
+        out["accepted_moves_ratio_vs_plain_fp32"] = (results["mixed-da"]["accepted_per_s"] /
+                                                     results["plain"]["accepted_per_s"])
+        ok = within_error_bars(results["mixed-da"]["energy"], results["plain"]["energy"])
+        out["energy_within_error_bars"] = bool(ok)
+        if not ok:
+            log("WARNING: E/N of mixed-da and plain differ by more than 2 standard errors: "
+                f"{results['mixed-da']['energy']['E_per_site']:.5f} +- {results['mixed-da']['energy']['E_err']:.5f} vs "
+                f"{results['plain']['energy']['E_per_site']:.5f} +- {results['plain']['energy']['E_err']:.5f}")
     if sur is not None:
-        # RMS error of the surrogate's log-ratio on one random exchange hop per chain
+        # RMS error of the surrogate's log-ratio on one random changing exchange hop per chain
         sm = results["mixed-da"]["sampler"]
-        s = sm.s.cpu().numpy()
+        s = sm.s[:2048].cpu().numpy()
         rng = np.random.default_rng(5)
         sp = s.copy()
-        N = wk.N
-        for c in range(C):
+        for c in range(len(s)):
             while True:
                 i, dirn, spin = rng.integers(N), rng.integers(4), rng.integers(2)
                 x, y = divmod(int(i), wk.LY)
@@ -332,16 +421,17 @@ def main():
         sur_info["rms_exact_log_ratio"] = float(np.sqrt(np.mean(d_exact[ok] ** 2)))
         out["surrogate"] = sur_info
     if not args.no_cpu:
-        # the residual CNN costs ~30 s of host time per sample's E_loc: one at a time, and its FP64
-        # E_loc parity is left to tests/test_gpu_released.py
-        n = max(1, min(args.cpu_samples, C))
+        n = max(1, min(args.cpu_samples, M))
         x = results[head]["sampler"].s[:n]
         gpu_psi, gpu_eloc = {}, {}
         for f in forms:
             p = states[f](x)
             gpu_psi[f] = (p.sign.cpu().numpy(), p.logabs.cpu().numpy())
             gpu_eloc[f] = np.zeros(0) if resconv else H.Oloc(states[f], Samples(x[:2])).cpu().numpy()
-        out.update(cpu_leg(wk, params, x.cpu().numpy(), gpu_psi, gpu_eloc, per_call=1 if resconv else 2))
+        plain_exact = results["plain"]["stats"]["exact_evaluations"] if "plain" in results else \
+            results[head]["stats"]["changing"]
+        props_exact = plain_exact / (args.steps * M)
+        out.update(cpu_leg(wk, params, x.cpu().numpy(), gpu_psi, gpu_eloc, props_exact))
     print(json.dumps(out))
 
 

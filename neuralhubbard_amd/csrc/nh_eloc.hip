@@ -177,6 +177,10 @@ __global__ void k_eloc_reduce(int N, double U, int batch, int b0, const int8_t* 
   const double l0 = logabs_s[b];
   const float s0 = sign_s[b];
   const int c = count[lb];
+  if (c > max_hops) {  // more electrons than the model holds: only max_hops slots were written
+    if (lane == 0) eloc[b] = NAN;
+    return;
+  }
   for (int h = lane; h < c; h += kWave) {
     const size_t i = (size_t)lb * max_hops + h;
     const float sg = hop_sign[i];

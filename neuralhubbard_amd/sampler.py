@@ -21,7 +21,6 @@ import numpy as np
 
 from . import _lib
 from .backflow import Samples, Variational, _Workspace
-from .lattice import apply_group
 
 
 def _torch():
@@ -94,30 +93,37 @@ class Surrogate:
                    "nh_adam")
 
     def train(self, spins, logabs, epochs=10, batch=1024, lr=2e-3, seed=0, augment=True, log=None):
-        """Fit f + offset to log|psi| by mean squared error with Adam.  With ``augment`` each
-        configuration enters in a random image of the projection group (|psi_P| is invariant
-        under it).  Returns the per-epoch mean squared error."""
+        """Fit f + offset to log|psi| by mean squared error with Adam.  The pairs stay on the GPU;
+        with ``augment`` each configuration enters every batch in a random image of the
+        projection group (|psi_P| is invariant under it; every group element is an involution, so
+        the image is a gather of the modes).  Returns the per-epoch mean squared error."""
         torch = _torch()
-        spins = np.asarray(spins, np.int8)
-        y = np.asarray(logabs, np.float64)
-        self.offset = float(y.mean())
-        target = torch.as_tensor((y - self.offset).astype(np.float32), device="cuda")
-        rng = np.random.Generator(np.random.PCG64(seed))
-        n = spins.shape[0]
+        from .lattice import group_mode_perms
+
+        x_all = torch.as_tensor(spins, dtype=torch.int8, device="cuda")
+        y = torch.as_tensor(logabs, dtype=torch.float64, device="cuda")
+        self.offset = float(y.mean().item())
+        target = (y - self.offset).float()
+        perms = torch.as_tensor(np.asarray(group_mode_perms(self.LX, self.LY)), dtype=torch.int64, device="cuda")
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        n = x_all.shape[0]
         hist = []
         for ep in range(epochs):
-            order = rng.permutation(n)
-            tot = 0.0
+            order = torch.randperm(n, generator=gen, device="cuda")
+            tot = torch.zeros((), dtype=torch.float64, device="cuda")
+            nb = 0
             for b0 in range(0, n - batch + 1, batch):
                 idx = order[b0:b0 + batch]
-                x = spins[idx]
+                x = x_all[idx]
                 if augment:
-                    gs = rng.integers(0, 8, size=len(idx))
-                    x = np.stack([apply_group(x[k], int(gs[k]), self.LX, self.LY) for k in range(len(idx))])
-                g, loss = self.grad(x, target[torch.as_tensor(idx, device="cuda")], 1.0 / len(idx))
+                    gs = torch.randint(0, 8, (batch,), generator=gen, device="cuda")
+                    x = torch.gather(x, 1, perms[gs])
+                g, loss = self.grad(x, target[idx], 1.0 / batch)
                 self.adam(g, lr=lr)
-                tot += float(loss.item()) * 2.0
-            hist.append(tot / max(1, n // batch))
+                tot += loss[0].double()
+                nb += 1
+            hist.append(2.0 * float(tot.item()) / max(1, nb))
             if log:
                 log(f"surrogate epoch {ep}: mse {hist[-1]:.4f}")
         return hist

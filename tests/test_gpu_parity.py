@@ -229,18 +229,52 @@ def test_log_psi_pf16_matches_oracle():
 
 def test_log_psi_pf32_matches_oracle():
     """32x32 Pfaffian backflow (896 x 896 Pfaffians, triangles in the global workspace, VALU
-    feature network): same sign as FP64 and |d log psi| < 1e-2 (the measured error is
-    printed; FP32 elimination over 448 pivot steps)."""
+    feature network) on 8 configurations against the FP64 C restatement: same sign and
+    |d log psi| < 1e-2 (the measured error is printed; FP32 elimination over 448 pivot steps)."""
     from neuralhubbard_amd.backflow import Variational
+    from oracle.cpu_port import CpuPath
 
     wk = WORKLOADS["pf32"]
-    dev, ref = _model(wk)
-    s = make_configurations(wk, 2)
+    p = make_params(wk)
+    dev, _ = _model(wk)
+    s = make_configurations(wk, 8)
     sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
-    for b in range(2):
-        rs, rl = ref.psi(s[b])
-        print(f"pf32 b={b} gpu={la[b]:.6f} fp64={rl:.6f} |d|={abs(la[b] - rl):.2e}", flush=True)
-        assert sg[b] == rs and abs(la[b] - rl) < 1e-2, (b, la[b], rl)
+    rs, rl = CpuPath(wk, p, bits=64).psi(s)
+    for b in range(len(s)):
+        print(f"pf32 b={b} gpu={la[b]:.6f} fp64={rl[b]:.6f} |d|={abs(la[b] - rl[b]):.2e}", flush=True)
+    assert np.array_equal(sg, rs)
+    assert np.abs(la - rl).max() < 1e-2
+
+
+def test_local_energy_pf32_matches_materialized_hops():
+    """E_loc at 32x32 Pfaffian (n = 896) through nh_local_energy (hop items, chunked images,
+    triangles in the workspace) equals U D(s) + sum over the oracle's hop list (Jordan-Wigner signs
+    included) of h psi(s') / psi(s) with every psi(s') from nh_log_psi on the materialised
+    configurations; 16 of those psi(s') against the FP64 C restatement (its full E_loc, 4036
+    amplitudes of 8 896 x 896 Pfaffians each, takes minutes on the host)."""
+    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational
+    from oracle.cpu_port import CpuPath
+
+    wk = WORKLOADS["pf32"]
+    dev, _ = _model(wk)
+    st = Variational(dev)
+    s = make_configurations(wk, 1)
+    ts = torch.tensor(s, device="cuda")
+    psi = st(ts)
+    e = Hubbard(wk.U, 1.0, wk.tprime, chunk=1).Oloc(st, Samples(ts, psi)).cpu().numpy()[0]
+    hl = list(O.hops(s[0], wk.LX, wk.LY, 1.0, wk.tprime))
+    sp = np.repeat(s[:1], len(hl), axis=0)
+    for k, (a, b2, _) in enumerate(hl):
+        sp[k, a], sp[k, b2] = -1, 1
+    sg2, la2 = st(torch.tensor(sp, device="cuda")).numpy()
+    sg0, la0 = float(psi.sign[0]), float(psi.logabs[0])
+    want = O.diag_energy(s[0], wk.U) + sum(h * sg2[k] * sg0 * math.exp(la2[k] - la0) for k, (_, _, h) in enumerate(hl))
+    print(f"pf32 eloc gpu={e / wk.N:.8f} materialised={want / wk.N:.8f} per site ({len(hl)} hops)", flush=True)
+    assert abs(e - want) < 1e-5 * abs(want) + 1e-4
+    pick = np.linspace(0, len(hl) - 1, 16).astype(int)
+    r_s, r_l = CpuPath(wk, make_params(wk), bits=64).psi(sp[pick])
+    assert np.array_equal(sg2[pick], r_s)
+    assert np.abs(la2[pick] - r_l).max() < 1e-2
 
 
 def test_log_psi_bcs16_matches_oracle():
@@ -347,6 +381,30 @@ def test_local_energy_small_matches_oracle(kind):
             continue
         er = O.eloc(ref, s[b], 8.0, 1.0, -0.3)
         assert abs(e[b] - er) < log_tol(1e-6, c) * abs(er) + 1e-5, (b, e[b], er)
+
+
+def test_local_energy_of_malformed_configuration_is_nan():
+    """A configuration with more electrons than the model (its hop count exceeds the item slots of
+    nh_local_energy) gives NaN, as its amplitude does, and leaves the other samples' E_loc intact."""
+    from neuralhubbard_amd.backflow import Hubbard, LogArray, Samples, Variational
+
+    wk = small_workload(LX=4, LY=4, n_up=3, n_dn=3, channels=8, n_layers=2, tprime=-0.3)
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 3)
+    bad = s[1].copy()
+    bad[np.flatnonzero(bad[:16] < 0)[:4]] = 1  # 7 + 3 electrons
+    s[1] = bad
+    st = Variational(dev)
+    ts = torch.tensor(s, device="cuda")
+    psi = st(ts)
+    sg, la = psi.numpy()
+    assert np.isnan(la[1]) and sg[1] == 0
+    ok = LogArray(torch.where(torch.isnan(psi.logabs), 0, psi.sign), torch.nan_to_num(psi.logabs, nan=0.0))
+    e = Hubbard(8.0, 1.0, -0.3, chunk=3).Oloc(st, Samples(ts, ok)).cpu().numpy()
+    assert np.isnan(e[1])
+    for b in (0, 2):
+        er = O.eloc(ref, s[b], 8.0, 1.0, -0.3)
+        assert abs(e[b] - er) < log_tol(1e-6, cancellation(ref, s[b])) * abs(er) + 1e-5, (b, e[b], er)
 
 
 def test_local_energy_pf8_matches_oracle():
