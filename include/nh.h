@@ -147,6 +147,18 @@ size_t nh_log_psi_workspace_size(const nh_model* model, int32_t batch);
 int nh_log_psi(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* sign,
                double* logabs, void* workspace, size_t workspace_bytes);
 
+/* ---- allocation-free projected amplitude (no counterpart in the reference: for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same result as nh_log_psi, with the workspace taken from a static
+ * device array of NH_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends; no hipMalloc, hipFree or allocating library call.  Any batch is processed in
+ * passes that fit the array; a model of which one configuration needs more workspace than that
+ * returns NH_ERR_WORKSPACE.  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other host threads) race and must be serialised by the caller. */
+#define NH_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int nh_log_psi_noalloc(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* sign,
+                       double* logabs);
+
 /* ---- a13: Hubbard local energy  (replaces quantax Hubbard.Oloc, code/quick_start.py:33,
  * built at code/states.py:244-247).  E_loc(s) = U sum_i n_iu n_id
  *   - sum_{<ij>,spin} t_ij JW ψ(s^{i->j}) / ψ(s), t_ij = t (nearest) or tprime (diagonal,

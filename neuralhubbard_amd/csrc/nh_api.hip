@@ -175,12 +175,64 @@ extern "C" int nh_log_psi(void* stream, const nh_model* model, int32_t batch, co
   return run_items((hipStream_t)stream, model, s, nullptr, nullptr, batch, b, sign, logabs);
 }
 
+// ---- allocation-free projected amplitude: the workspace is a static device array of
+// NH_NOALLOC_SCRATCH_BYTES (each GPU the library runs on holds its own copy until the process
+// ends).  Host code never uses the array's name as an address: hipGetSymbolAddress gives each
+// device's copy once, with that device current, and the kernels receive that pointer.
+static __device__ __align__(256) char scratch[NH_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_of_current_device(void** out) {
+  int dev = -1;
+  NH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return NH_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return NH_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NH_OK;
+}
+
 int nh_log_psi_items(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
                      const int32_t* item_hop, int n_items, float* sign, double* logabs, void* ws, size_t wsb) {
   if (wsb < nh_log_psi_workspace_size(m, n_items)) return NH_ERR_WORKSPACE;
   Carver c((void*)align_up((size_t)ws));
   ItemBuffers b = carve_items(c, m, n_items);
   return run_items(st, m, s, item_b, item_hop, n_items, b, sign, logabs);
+}
+
+extern "C" int nh_log_psi_noalloc(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* sign,
+                                  double* logabs) {
+  int rc = check_model(model);
+  if (rc != NH_OK) return rc;
+  if (batch < 0) return NH_ERR_ARG;
+  if (batch == 0) return NH_OK;
+  if (!s || !sign || !logabs) return NH_ERR_ARG;
+  // the largest pass (configurations per nh_log_psi call) whose workspace fits the scratch;
+  // workspace sizes grow with the pass, so a binary search finds it
+  const size_t cap = NH_NOALLOC_SCRATCH_BYTES;
+  if (nh_log_psi_workspace_size(model, 1) > cap) return NH_ERR_WORKSPACE;  // one configuration does not fit
+  int32_t lo = 1, hi = batch;
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo + 1) / 2;
+    if (nh_log_psi_workspace_size(model, mid) <= cap) lo = mid;
+    else hi = mid - 1;
+  }
+  void* ws = nullptr;
+  rc = scratch_of_current_device(&ws);
+  if (rc != NH_OK) return rc;
+  const size_t two_n = 2 * (size_t)model->LX * model->LY;
+  for (int32_t b0 = 0; b0 < batch; b0 += lo) {
+    const int32_t nb = std::min(lo, batch - b0);
+    rc = nh_log_psi(stream, model, nb, s + (size_t)b0 * two_n, sign + b0, logabs + b0, ws, cap);
+    if (rc != NH_OK) return rc;
+  }
+  return NH_OK;
 }
 
 static size_t eloc_carve(Carver& c, const nh_model* m, double tprime, int32_t chunk, int32_t** count,
