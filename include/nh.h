@@ -199,20 +199,39 @@ int nh_sur_grad(void* stream, const nh_surrogate* sur, int32_t batch, const int8
 int nh_adam(void* stream, int32_t n, float* w, const float* g, float* m, float* v, float lr, float beta1,
             float beta2, float eps, int32_t step);
 
-/* ---- a14 (+ a16): Metropolis exchange sweep  (replaces quantax sampler.sweep of the
- * MixSampler built at code/states.py:355-408; the delayed-acceptance stage is new).
+/* ---- a14 (+ a16): Metropolis sweep  (replaces quantax sampler.sweep of the MixSampler of
+ * ParticleHop and SiteExchange built at code/states.py:355-408 and
+ * code/train_pfaffian_16x16_tpm02.py:93-99, with n_neighbor 1 or (1, 2) per state,
+ * code/states.py:95-96,105-119; the delayed-acceptance stage is new).
  * n_steps proposals per chain; step t of chain c uses Philox4x32-10 at counter
- * (c, step0 + t, 0) and key seed.  Proposal: site i = r0 mod N, direction r1 & 3 of
- * (+x, -x, +y, -y), spin r1 >> 2 & 1: the occupations of modes (spin, i) and (spin, j) are
- * exchanged (a spin-preserving nearest-neighbour hop).  sur == NULL: accept with
- * min(1, |psi'/psi|^2).  sur != NULL: delayed acceptance (stage 1 with the surrogate, stage 2
- * exact), f_cur [n_chains] the surrogate value of the current configurations.
+ * (c, step0 + t, 0) and key seed: r0, r1 draw the move, r2, r3 two 24-bit uniforms.
+ * Proposal kinds (nh_proposal, NULL = NH_MOVE_EXCHANGE):
+ *   NH_MOVE_EXCHANGE       site i = r0 mod N, direction r1 & 3 of (+x, -x, +y, -y), spin
+ *                          r1 >> 2 & 1: the occupations of modes (spin, i) and (spin, j) trade
+ *                          places (a spin-preserving nearest-neighbour hop);
+ *   NH_MOVE_PARTICLE_HOP   particle r0 mod (n_up + n_dn) (occupied modes in ascending order)
+ *                          to neighbour r1 & (z - 1) of (+x, -x, +y, -y, +x+y, +x-y, -x+y,
+ *                          -x-y), z = ph_neighbors (4: nearest, 8: and next-nearest); no move
+ *                          when that mode is occupied;
+ *   NH_MOVE_SITE_EXCHANGE  site i = r0 mod N and neighbour r1 & (z - 1), z = se_neighbors: the
+ *                          two sites' occupations trade places in both spin blocks;
+ *   NH_MOVE_MIX            particle hop or site exchange by r1 >> 4 & 1 (probability 1/2
+ *                          each, per chain and step).
+ * Every kind is a symmetric proposal.  sur == NULL: accept with min(1, |psi'/psi|^2).
+ * sur != NULL: delayed acceptance (stage 1 with the surrogate, stage 2 exact), f_cur
+ * [n_chains] the surrogate value of the current configurations.
  * s / sign / logabs (/ f_cur) are updated in place; counters int32 [4][n_chains] are
  * incremented: changing proposals, stage-1 survivors, accepted, exact evaluations. */
+enum { NH_MOVE_EXCHANGE = 0, NH_MOVE_PARTICLE_HOP = 1, NH_MOVE_SITE_EXCHANGE = 2, NH_MOVE_MIX = 3 };
+typedef struct nh_proposal {
+  int32_t kind;         /* NH_MOVE_* */
+  int32_t ph_neighbors; /* partners of a particle hop: 4 or 8 (unused unless hops are drawn) */
+  int32_t se_neighbors; /* partners of a site exchange: 4 or 8 (unused unless exchanges are drawn) */
+} nh_proposal;
 size_t nh_metropolis_workspace_size(const nh_model* model, int32_t n_chains);
-int nh_metropolis_sweep(void* stream, const nh_model* model, const nh_surrogate* sur, int32_t n_chains, int8_t* s,
-                        float* sign, double* logabs, float* f_cur, uint64_t seed, int64_t step0, int32_t n_steps,
-                        int32_t* counters, void* workspace, size_t workspace_bytes);
+int nh_metropolis_sweep(void* stream, const nh_model* model, const nh_surrogate* sur, const nh_proposal* proposal,
+                        int32_t n_chains, int8_t* s, float* sign, double* logabs, float* f_cur, uint64_t seed,
+                        int64_t step0, int32_t n_steps, int32_t* counters, void* workspace, size_t workspace_bytes);
 
 /* ---- measurement hook (no reference counterpart; used by bench.py): while enabled, every
  * launch of a kernel class is bracketed by HIP events recorded on the launch stream.

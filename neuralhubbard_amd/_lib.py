@@ -28,6 +28,13 @@ class NhSurrogate(ctypes.Structure):
                 ("reserved", ctypes.c_int32), ("params", ctypes.c_void_p)]
 
 
+class NhProposal(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("ph_neighbors", ctypes.c_int32), ("se_neighbors", ctypes.c_int32)]
+
+
+MOVE_EXCHANGE, MOVE_PARTICLE_HOP, MOVE_SITE_EXCHANGE, MOVE_MIX = 0, 1, 2, 3
+
+
 class NhModel(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
@@ -78,8 +85,8 @@ SIGNATURES = {
     "nh_sur_grad": (_I, [_P, ctypes.POINTER(NhSurrogate), _I, _P, _P, ctypes.c_float, _P, _P, _P, _S]),
     "nh_adam": (_I, [_P, _I, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I]),
     "nh_metropolis_workspace_size": (_S, [_MP, _I]),
-    "nh_metropolis_sweep": (_I, [_P, _MP, ctypes.POINTER(NhSurrogate), _I, _P, _P, _P, _P, ctypes.c_uint64,
-                                 ctypes.c_int64, _I, _P, _P, _S]),
+    "nh_metropolis_sweep": (_I, [_P, _MP, ctypes.POINTER(NhSurrogate), ctypes.POINTER(NhProposal), _I, _P, _P,
+                                 _P, _P, ctypes.c_uint64, ctypes.c_int64, _I, _P, _P, _S]),
     "nh_profile_enable": (_I, [_I]),
     "nh_profile_query": (_I, [_I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
     "nh_profile_reset": (_I, []),

@@ -62,8 +62,8 @@ __global__ void __launch_bounds__(MAXT) k_cnn_fp32(int N, int LX, int LY, int n_
   float xin0 = 0.f, xin1 = 0.f;
   if (active) {
     const int8_t* sb = s + (size_t)b * two_n;
-    xin0 = (float)item_occ(sb, hop, group_map(site, grp, N, LX, LY));
-    xin1 = (float)item_occ(sb, hop, group_map(N + site, grp, N, LX, LY));
+    xin0 = (float)item_occ(sb, hop, group_map(site, grp, N, LX, LY), N);
+    xin1 = (float)item_occ(sb, hop, group_map(N + site, grp, N, LX, LY), N);
   }
   g_act[t] = xin0;
   g_act[S + t] = xin1;

@@ -71,8 +71,8 @@ __global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const vo
   {
     const int8_t* sb = s + (size_t)b * two_n;
     for (int site = t; site < N; site += blockDim.x) {
-      const float o0 = (float)item_occ(sb, hop, group_map(site, grp, N, LX, LY));
-      const float o1 = (float)item_occ(sb, hop, group_map(N + site, grp, N, LX, LY));
+      const float o0 = (float)item_occ(sb, hop, group_map(site, grp, N, LX, LY), N);
+      const float o1 = (float)item_occ(sb, hop, group_map(N + site, grp, N, LX, LY), N);
       if (LP) {  // +-1 is exact in fp16: lo part zero
         _Float16* g = reinterpret_cast<_Float16*>(lds_act) + site * kMfC;
         _Float16* gl = g + N * kMfC;

@@ -314,7 +314,7 @@ __device__ inline int occupied_modes(const int8_t* sb, int32_t hop, int two_n, i
   int cnt = 0;
   for (int m0 = 0; m0 < two_n; m0 += kWave) {
     const int m = m0 + lane;
-    const bool occ = (m < two_n) && item_occ(sb, hop, m) > 0;
+    const bool occ = (m < two_n) && item_occ(sb, hop, m, N) > 0;
     const uint64_t mask = __ballot(occ);
     const int pos = cnt + lanes_below(mask);
     if (occ && pos < cap) modes[pos] = group_map(m, g, N, LX, LY);

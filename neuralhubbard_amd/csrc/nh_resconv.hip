@@ -175,8 +175,8 @@ __global__ void __launch_bounds__(256) k_resconv(int LX, int n_blocks, const flo
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int site = (x0 + r) * kRcLY + 4 * g16 + q;
-        const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY)) > 0;
-        const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
+        const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY), N) > 0;
+        const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY), N) > 0;
         const float* e = emb + (up + 2 * dn) * kRcC + j;
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) h[r][qq][q] = e[16 * qq];
@@ -440,8 +440,8 @@ __global__ void __launch_bounds__(NT) k_resconv_bf16(int LX, int n_blocks, const
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int site = (x0 + r) * kRcLY + j;
-      const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY)) > 0;
-      const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
+      const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY), N) > 0;
+      const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY), N) > 0;
       const float* e = emb + (up + 2 * dn) * kRcC + 4 * g16;
 #pragma unroll
       for (int qq = 0; qq < QN; ++qq) {
@@ -634,8 +634,8 @@ __global__ void __launch_bounds__(NT) k_resconv_fs(int LX, int n_blocks, const f
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int site = (x0 + r) * kRcLY + 4 * g16 + q;
-        const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY)) > 0;
-        const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY)) > 0;
+        const int up = item_occ(sb, hop, group_map(site, grp, N, LX, kRcLY), N) > 0;
+        const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, kRcLY), N) > 0;
         const float* e = emb + (up + 2 * dn) * kRcC + j;
 #pragma unroll
         for (int qq = 0; qq < QN; ++qq) h[r][qq][q] = e[16 * (q0 + qq)];
@@ -721,8 +721,8 @@ __global__ void __launch_bounds__(256) k_rg_embed(int LX, int LY, const int8_t* 
   if (hop != kHopInvalid) {
     const int b = item_b ? item_b[item] : item;
     const int8_t* sb = s + (size_t)b * 2 * N;
-    const int up = item_occ(sb, hop, group_map(site, grp, N, LX, LY)) > 0;
-    const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, LY)) > 0;
+    const int up = item_occ(sb, hop, group_map(site, grp, N, LX, LY), N) > 0;
+    const int dn = item_occ(sb, hop, group_map(N + site, grp, N, LX, LY), N) > 0;
     v = emb[(up + 2 * dn) * kRcC + c];
   }
   H[(size_t)i * N * kRcC + rg_at(N, site, c)] = v;

@@ -86,8 +86,8 @@ __global__ void __launch_bounds__(256) k_sur_forward(int N, int LX, int LY, cons
   int nb[9];
   neighbour_slots(li, site, N, LX, LY, nb);
   const int8_t* sb = s + (size_t)b * two_n;
-  x0[t] = active ? (float)item_occ(sb, hop, site) : 0.f;
-  x0[S + t] = active ? (float)item_occ(sb, hop, N + site) : 0.f;
+  x0[t] = active ? (float)item_occ(sb, hop, site, N) : 0.f;
+  x0[S + t] = active ? (float)item_occ(sb, hop, N + site, N) : 0.f;
   __syncthreads();
   float z[CS];
 #pragma unroll

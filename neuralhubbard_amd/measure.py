@@ -373,16 +373,18 @@ def main(argv=None) -> dict:
             --weights weights/tpm02_L16_pfaffian.eqx --spins weights/tpm02_L16_pfaffian_spins.npz \\
             --out runs/tpm02_L16_pfaffian
 
-    The chains start at the released chains (repeated or truncated to --nsamples), are
-    thermalised for --thermal x N exchange proposals and measured every --sweep-steps x N
-    proposals (defaults 50 and 4, the paper's values for the 16x16 states, ``code/states.py:99-126``).
-    Writes series.npz, spins_last.npz, results.json and results.npz into --out."""
+    The chains start at the released chains (repeated or truncated to --nsamples) and move by
+    the paper's sampler (`released.make_sampler`: particle hops and site exchanges with the
+    state's neighbour ranges; --moves exchange keeps the plain same-spin exchange), thermalised
+    for --thermal x N steps and measured every --sweep-steps x N steps (defaults: the state's
+    values, ``code/states.py:105-119``).  Writes series.npz, spins_last.npz, results.json and
+    results.npz into --out."""
     import argparse
 
     import torch
 
     from .backflow import Hubbard, Variational, build_backflow
-    from .released import STATES
+    from .released import STATES, make_sampler
     from .sampler import MetropolisSampler
 
     p = argparse.ArgumentParser(description="Measure energy, spin, charge and d-wave correlations of a released state.")
@@ -392,8 +394,10 @@ def main(argv=None) -> dict:
     p.add_argument("--out", required=True, help="output folder")
     p.add_argument("--nsamples", type=int, default=8192, help="chains, one sample each per sweep")
     p.add_argument("--nsweeps", type=int, default=64)
-    p.add_argument("--thermal", type=float, default=50.0, help="thermalisation proposals per chain / N")
-    p.add_argument("--sweep-steps", type=float, default=4.0, help="proposals per chain between sweeps / N")
+    p.add_argument("--thermal", type=float, help="thermalisation steps per chain / N (default: the state's)")
+    p.add_argument("--sweep-steps", type=float, help="steps per chain between sweeps / N (default: the state's)")
+    p.add_argument("--moves", default="mix", choices=["mix", "exchange"],
+                   help="mix: the paper's particle hops + site exchanges; exchange: same-spin NN exchange")
     p.add_argument("--channels", default=",".join(CHANNELS), help="comma-separated subset of " + ",".join(CHANNELS))
     p.add_argument("--precision", default="mixed", choices=["fp32", "mixed"])
     p.add_argument("--seed", type=int, default=42)
@@ -405,9 +409,14 @@ def main(argv=None) -> dict:
     wk, params, chains = _load_state_files(args.state, args.weights, args.spins)
     state = Variational(build_backflow(wk, params, args.precision))
     start = np.resize(chains, (args.nsamples, 2 * wk.N))
-    print(f"{info.name}: {wk.LX}x{wk.LY}, {args.nsamples} chains, {args.precision}; thermalising", flush=True)
-    sampler = MetropolisSampler(state, torch.as_tensor(start), int(round(args.sweep_steps * wk.N)), seed=args.seed,
-                                thermal_steps=int(round(args.thermal * wk.N)))
+    sweep = int(round((info.sweeps[0] if args.sweep_steps is None else args.sweep_steps) * wk.N))
+    thermal = int(round((info.sweeps[1] if args.thermal is None else args.thermal) * wk.N))
+    print(f"{info.name}: {wk.LX}x{wk.LY}, {args.nsamples} chains, {args.precision}, {args.moves} moves; "
+          f"thermalising", flush=True)
+    if args.moves == "mix":
+        sampler = make_sampler(state, info, args.nsamples, start, sweep, thermal, seed=args.seed)
+    else:
+        sampler = MetropolisSampler(state, torch.as_tensor(start), sweep, seed=args.seed, thermal_steps=thermal)
     H = Hubbard(U=info.U, t=1.0, tprime=info.tprime)
     return run_measure(state, H, sampler, args.nsweeps, args.out, name=info.name, tprime=info.tprime,
                        channels=channels)

@@ -43,6 +43,10 @@ class StateInfo:
     var_per_site: float
     var_err: float
     U: float = 8.0
+    # n_neighbor of (ParticleHop, SiteExchange), and (measurement sweep, thermalisation,
+    # optimisation sweep) in Metropolis steps per chain / N (code/states.py:95-103,105-119)
+    neighbors: tuple = ((1,), (1,))
+    sweeps: tuple = (4, 50, 2)
 
     @property
     def N(self) -> int:
@@ -59,19 +63,21 @@ class StateInfo:
                         notes=f"released state {self.name}")
 
 
+_NN, _NNN = (1,), (1, 2)
+
 STATES = {
     s.name: s
     for s in (
         StateInfo("tp0_L16_bcs", "bcs", 16, 0.0, "cdfdb8e6065b904232bf10dec6ca2227", 1_208_320,
-                  -0.75878, 2e-5, 0.0255, 0.0020),
+                  -0.75878, 2e-5, 0.0255, 0.0020, neighbors=(_NN, _NN), sweeps=(4, 50, 2)),
         StateInfo("tp0_L16_pfaffian", "pfaffian", 16, 0.0, "17b770fe57367b09b27ad37538842ba0", 1_208_320,
-                  -0.75813, 1e-5, 0.0259, 0.0002),
+                  -0.75813, 1e-5, 0.0259, 0.0002, neighbors=(_NN, _NN), sweeps=(4, 50, 2)),
         StateInfo("tpm02_L16_pfaffian", "pfaffian", 16, -0.2, "aef48e8b7c10cfa183f8c0e8dcf84d06", 1_208_320,
-                  -0.74533, 1e-5, 0.0272, 0.0005),
+                  -0.74533, 1e-5, 0.0272, 0.0005, neighbors=(_NNN, _NNN), sweeps=(4, 50, 3)),
         StateInfo("tpm02_L24_pfaffian", "pfaffian", 24, -0.2, "17653481d380416e6df7033ba7902aee", 1_241_088,
-                  -0.74585, 1e-5, 0.0257, 0.0003),
+                  -0.74585, 1e-5, 0.0257, 0.0003, neighbors=(_NNN, _NNN), sweeps=(4, 50, 3)),
         StateInfo("tp0_L32_bcs", "bcs", 32, 0.0, "e45a997c3202fae9df7c4f38bd7e6acc", 1_257_472,
-                  -0.75617, 1e-5, 0.046, 0.010),
+                  -0.75617, 1e-5, 0.046, 0.010, neighbors=(_NN, _NN), sweeps=(5, 40, 4)),
     )
 }
 
@@ -188,3 +194,27 @@ def load_spins(path: str, info: StateInfo | None = None) -> np.ndarray:
         if np.any((spins[:, :N] > 0).sum(1) != info.n_up) or np.any((spins[:, N:] > 0).sum(1) != info.n_up):
             raise ValueError(f"{path}: wrong particle numbers.")
     return spins
+
+
+def make_sampler(state, info: StateInfo, nsamples: int, initial_spins=None, sweep_steps: int | None = None,
+                 thermal_steps: int | None = None, seed: int = 42, spins_path: str | None = None, surrogate=None):
+    """The sampler of the paper (``code/states.py:355-408``): `sampler.MixSampler` of
+    ``ParticleHop(nsamples // 2)`` and ``SiteExchange(nsamples // 2)`` with the state's neighbour
+    ranges, chains from `initial_spins` (or the released chains at `spins_path`) repeated or
+    truncated to nsamples, thermalised for ``thermal x N`` steps, ``sweep x N`` steps per sweep."""
+    from .sampler import MixSampler, ParticleHop, SiteExchange
+
+    if nsamples % 2:
+        raise ValueError("nsamples must be even.")
+    if initial_spins is None:
+        if spins_path is None:
+            raise ValueError("make_sampler needs initial_spins or spins_path")
+        initial_spins = load_spins(spins_path, info)
+    N = info.N
+    sweep_steps = int(round(info.sweeps[0] * N)) if sweep_steps is None else int(sweep_steps)
+    thermal_steps = int(round(info.sweeps[1] * N)) if thermal_steps is None else int(thermal_steps)
+    ph = ParticleHop(state, nsamples // 2, n_neighbor=info.neighbors[0])
+    se = SiteExchange(state, nsamples // 2, n_neighbor=info.neighbors[1])
+    return MixSampler([ph, se], thermal_steps=thermal_steps, sweep_steps=sweep_steps, initial_spins=initial_spins,
+                      seed=seed, surrogate=surrogate)
+

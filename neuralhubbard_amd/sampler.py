@@ -1,11 +1,13 @@
-"""Metropolis exchange sampler and the learned surrogate of delayed acceptance (host side).
+"""Metropolis samplers and the learned surrogate of delayed acceptance (host side).
 
-`MetropolisSampler` plays the role of the reference's ``make_sampler`` / quantax
-``MixSampler.sweep()`` (``code/states.py:355-408``, ``code/measure_orderparams.py:128``):
-chains of configurations resident in HBM, ``sweep()`` runs ``sweep_steps`` proposals per chain
-and returns `Samples` with the projected amplitudes.  With a `Surrogate` it runs delayed
-acceptance: the surrogate screens every proposal and only survivors reach the exact amplitude,
-which keeps |psi|^2 exactly stationary (Christen & Fox 2005).
+`MetropolisSampler` plays the role of quantax's samplers behind the reference's
+``make_sampler`` (``code/states.py:355-408``, ``code/measure_orderparams.py:128``): chains of
+configurations resident in HBM, ``sweep()`` runs ``sweep_steps`` proposals per chain and returns
+`Samples` with the projected amplitudes.  The move set is a `Proposal`: the default exchange of
+two nearest-neighbour modes of one spin, or the reference's `ParticleHop`, `SiteExchange` and
+their `MixSampler` with n_neighbor 1 (nearest) or (1, 2) (and next-nearest).  With a `Surrogate`
+it runs delayed acceptance: the surrogate screens every proposal and only survivors reach the
+exact amplitude, which keeps |psi|^2 exactly stationary (Christen & Fox 2005).
 
 `Surrogate` is a small translation-equivariant conv network predicting log|psi| (layout in
 ``include/nh.h``), trained on the GPU by the library's own forward/backward kernels with Adam
@@ -19,8 +21,34 @@ import math
 
 import numpy as np
 
+from dataclasses import dataclass
+
 from . import _lib
 from .backflow import Samples, Variational, _Workspace
+
+
+@dataclass(frozen=True)
+class Proposal:
+    """Move set of a sweep (``nh_proposal`` of include/nh.h): kind `_lib.MOVE_*` and the number of
+    partner sites of a particle hop / site exchange (4 nearest, or 8 with the next-nearest)."""
+
+    kind: int = _lib.MOVE_EXCHANGE
+    ph_neighbors: int = 4
+    se_neighbors: int = 4
+
+    def struct(self) -> _lib.NhProposal:
+        return _lib.NhProposal(self.kind, self.ph_neighbors, self.se_neighbors)
+
+
+def neighbor_count(n_neighbor) -> int:
+    """quantax's ``n_neighbor``: 1 -> the 4 nearest neighbours, (1, 2) -> 8 with the next-nearest
+    (``code/states.py:95-96``)."""
+    t = tuple(int(v) for v in np.atleast_1d(n_neighbor))
+    if t == (1,):
+        return 4
+    if t == (1, 2):
+        return 8
+    raise ValueError(f"n_neighbor {n_neighbor!r}: expected 1 or (1, 2)")
 
 
 def _torch():
@@ -130,16 +158,18 @@ class Surrogate:
 
 
 class MetropolisSampler:
-    """Chains resident on the GPU; ``sweep()`` = ``sweep_steps`` exchange proposals per chain
-    (quantax ``Sampler.sweep``), optionally screened by a surrogate (delayed acceptance)."""
+    """Chains resident on the GPU; ``sweep()`` = ``sweep_steps`` proposals per chain (quantax
+    ``Sampler.sweep``) drawn from `proposal`, optionally screened by a surrogate (delayed
+    acceptance)."""
 
     def __init__(self, state: Variational, initial_spins, sweep_steps: int, seed: int = 42,
-                 surrogate: Surrogate | None = None, thermal_steps: int = 0):
+                 surrogate: Surrogate | None = None, thermal_steps: int = 0, proposal: Proposal | None = None):
         torch = _torch()
         self.state = state
         self.sweep_steps = int(sweep_steps)
         self.seed = int(seed)
         self.surrogate = surrogate
+        self.proposal = proposal if proposal is not None else Proposal()
         if not isinstance(initial_spins, torch.Tensor):
             initial_spins = np.asarray(initial_spins, np.int8)
         self.s = torch.as_tensor(initial_spins, dtype=torch.int8, device="cuda").contiguous().clone()
@@ -167,8 +197,10 @@ class MetropolisSampler:
         if self.surrogate is not None:
             # the kernels compare raw network outputs; the offset cancels in differences
             f_raw = self.f_cur - self.surrogate.offset
+        prop = self.proposal.struct()
         _lib.check(lib.nh_metropolis_sweep(
-            _lib.stream_handle(), ctypes.byref(m), ctypes.byref(sur) if sur is not None else None, self.n_chains,
+            _lib.stream_handle(), ctypes.byref(m), ctypes.byref(sur) if sur is not None else None,
+            ctypes.byref(prop), self.n_chains,
             _lib.ptr(self.s), _lib.ptr(self.sign), _lib.ptr(self.logabs),
             _lib.ptr(f_raw) if sur is not None else None, self.seed, self.step, int(n_steps),
             _lib.ptr(self.counters), _lib.ptr(ws), ws.numel()), "nh_metropolis_sweep")
@@ -201,3 +233,95 @@ class MetropolisSampler:
                 "acceptance": float(c[2] / max(1, c[0])),
                 "stage1_acceptance": float(c[1] / max(1, c[0])) if self.surrogate is not None else None,
                 "stage2_acceptance": float(c[2] / max(1, c[1])) if self.surrogate is not None else None}
+
+
+def _start_spins(state: Variational, nsamples: int, initial_spins, seed: int):
+    """initial_spins repeated or truncated to nsamples chains, or random configurations with the
+    state's particle numbers."""
+    if initial_spins is None:
+        from .configs import make_configurations
+
+        return make_configurations(state.model.workload, nsamples, seed=seed)
+    torch = _torch()
+    if isinstance(initial_spins, torch.Tensor):
+        initial_spins = initial_spins.cpu().numpy()
+    spins = np.asarray(initial_spins, np.int8)
+    return np.resize(spins, (nsamples, spins.shape[-1])) if spins.shape[0] != nsamples else spins
+
+
+class _MoveSampler(MetropolisSampler):
+    """A sampler with one move kind, in the form of quantax's constructors.  Inside a
+    `MixSampler` it only names its move, neighbour range and chain count; its own chains are
+    made on first use."""
+
+    kind = _lib.MOVE_EXCHANGE
+
+    def __init__(self, state: Variational, nsamples: int, thermal_steps: int = 0, sweep_steps: int | None = None,
+                 initial_spins=None, n_neighbor=1, seed: int = 42, surrogate: Surrogate | None = None):
+        self.state = state
+        self.nsamples = int(nsamples)
+        self.z = neighbor_count(n_neighbor)
+        self._pending = (initial_spins, sweep_steps, seed, surrogate, thermal_steps)
+
+    def _start(self):
+        if getattr(self, "_pending", None) is None:
+            return
+        init, sweep_steps, seed, surrogate, thermal = self._pending
+        self._pending = None
+        N = self.state.model.workload.N
+        MetropolisSampler.__init__(self, self.state, _start_spins(self.state, self.nsamples, init, seed),
+                                   N if sweep_steps is None else sweep_steps, seed, surrogate, thermal,
+                                   Proposal(self.kind, self.z, self.z))
+
+    def run(self, n_steps: int):
+        self._start()
+        super().run(n_steps)
+
+    def __getattr__(self, name):
+        if name in ("s", "sign", "logabs", "counters", "f_cur", "step", "_stats_from", "_ws", "proposal",
+                    "sweep_steps", "seed", "surrogate"):
+            self._start()
+            return self.__dict__[name]
+        raise AttributeError(name)
+
+
+class ParticleHop(_MoveSampler):
+    """quantax ``sampler.ParticleHop``: a uniformly drawn particle hops to one of its 4 (or 8)
+    neighbouring sites of the same spin; no move when that mode is occupied."""
+
+    kind = _lib.MOVE_PARTICLE_HOP
+
+
+class SiteExchange(_MoveSampler):
+    """quantax ``sampler.SiteExchange``: a uniformly drawn site trades its (up, down) occupation
+    with one of its 4 (or 8) neighbouring sites."""
+
+    kind = _lib.MOVE_SITE_EXCHANGE
+
+
+class MixSampler(MetropolisSampler):
+    """quantax ``sampler.MixSampler`` of `ParticleHop` and/or `SiteExchange` as the reference
+    builds it (``code/states.py:397-408``): sum(nsamples) chains; with both moves each chain draws
+    particle hop or site exchange with probability 1/2 at every step, a symmetric mixture (the
+    reference gives each move its share of the chains, nsamples // 2 each)."""
+
+    def __init__(self, samplers, thermal_steps: int = 0, sweep_steps: int | None = None, initial_spins=None,
+                 seed: int = 42, surrogate: Surrogate | None = None):
+        samplers = list(samplers)
+        if not samplers or not all(isinstance(x, (ParticleHop, SiteExchange)) for x in samplers):
+            raise TypeError("MixSampler takes ParticleHop and SiteExchange samplers")
+        state = samplers[0].state
+        ph = [x for x in samplers if isinstance(x, ParticleHop)]
+        se = [x for x in samplers if isinstance(x, SiteExchange)]
+        if len({x.z for x in ph}) > 1 or len({x.z for x in se}) > 1:
+            raise ValueError("one neighbour range per move kind")
+        if ph and se:
+            prop = Proposal(_lib.MOVE_MIX, ph[0].z, se[0].z)
+        elif ph:
+            prop = Proposal(_lib.MOVE_PARTICLE_HOP, ph[0].z, 4)
+        else:
+            prop = Proposal(_lib.MOVE_SITE_EXCHANGE, 4, se[0].z)
+        nsamples = sum(x.nsamples for x in samplers)
+        N = state.model.workload.N
+        super().__init__(state, _start_spins(state, nsamples, initial_spins, seed),
+                         N if sweep_steps is None else sweep_steps, seed, surrogate, thermal_steps, prop)

@@ -55,6 +55,12 @@ Pinning status per function (see the tests named in brackets):
   philox_draw /             chi-square stationarity of |psi|^2 over an
   metropolis_chain          enumerated 3x3 space, plain and delayed;
                             a |psi|-targeting chain fails it          [test_oracle_metropolis_*]
+  propose                   the exact proposal matrix over enumerated
+                            3x3 / 4x3 spaces is symmetric; the moves
+                            are the neighbour hops / site swaps drawn
+                            by brute force; site exchange alone keeps
+                            the doublon count, the mix connects the
+                            space; chi-square of the mixed chain     [test_oracle_proposals_*]
 """
 
 from __future__ import annotations
@@ -704,34 +710,81 @@ def philox_draw(chain, step, seed):
     return philox4x32([chain, step & 0xFFFFFFFF, (step >> 32) & 0xFFFFFFFF, 0], [seed & 0xFFFFFFFF, seed >> 32])
 
 
-def metropolis_chain(psi, s0, chain, seed, step0, n_steps, LX, LY, sur=None, margin=1e-6):
-    """Replays chain `chain` of the GPU sweep: proposals from Philox4x32-10 at counter
-    (chain, step), exchange of two nearest-neighbour modes of one spin, acceptance with
-    min(1, |psi'/psi|^2) or, with `sur` (a function s -> f(s)), delayed acceptance.  psi is a
-    function s -> (sign, logabs).  Returns (final configuration, accepted count, near_tie),
-    near_tie True when some acceptance test fell within `margin` of its threshold (then a
-    rounding-level difference can legitimately change the trajectory)."""
+MOVE_EXCHANGE, MOVE_PARTICLE_HOP, MOVE_SITE_EXCHANGE, MOVE_MIX = 0, 1, 2, 3
+# neighbour directions: the four nearest, then the four next-nearest (the E_loc hop order)
+NEIGHBOUR_DIRS = [(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (1, -1), (-1, 1), (-1, -1)]
+
+
+def _neighbour(i, d, LX, LY):
+    dx, dy = NEIGHBOUR_DIRS[d]
+    x, y = divmod(i, LY)
+    return ((x + dx) % LX) * LY + (y + dy) % LY
+
+
+def propose(s, r0, r1, LX, LY, kind=MOVE_EXCHANGE, ph_z=4, se_z=4):
+    """The configuration a proposal makes from s, or None for no change.  r0, r1 are the first
+    two Philox words.  Moves of quantax's samplers behind the reference's MixSampler
+    (code/states.py:397-408), as this build draws them:
+      exchange:      site r0 mod N, direction r1 & 3, spin r1 >> 2 & 1: modes (spin, i), (spin, j)
+                     trade occupations;
+      particle hop:  the (r0 mod n_occ)-th occupied mode (ascending) moves to neighbour
+                     r1 & (ph_z - 1) of its site in its spin block, unless that mode is occupied;
+      site exchange: site r0 mod N and neighbour r1 & (se_z - 1) trade (up, down) occupations;
+      mix:           r1 >> 4 & 1 picks particle hop (0) or site exchange (1)."""
     N = LX * LY
+    if kind == MOVE_MIX:
+        kind = MOVE_SITE_EXCHANGE if (r1 >> 4) & 1 else MOVE_PARTICLE_HOP
+    sp = np.array(s, dtype=np.int8, copy=True)
+    if kind == MOVE_EXCHANGE:
+        i = r0 % N
+        spin = (r1 >> 2) & 1
+        mi, mj = spin * N + i, spin * N + _neighbour(i, r1 & 3, LX, LY)
+        if s[mi] == s[mj]:
+            return None
+        sp[mi], sp[mj] = s[mj], s[mi]
+        return sp
+    if kind == MOVE_SITE_EXCHANGE:
+        i = r0 % N
+        j = _neighbour(i, r1 & (se_z - 1), LX, LY)
+        if s[i] == s[j] and s[N + i] == s[N + j]:
+            return None
+        for off in (0, N):
+            sp[off + i], sp[off + j] = s[off + j], s[off + i]
+        return sp
+    occ = np.flatnonzero(np.asarray(s) > 0)
+    n_occ = len(occ)
+    if n_occ == 0:
+        return None
+    m = int(occ[r0 % n_occ])
+    off = N if m >= N else 0
+    t = off + _neighbour(m - off, r1 & (ph_z - 1), LX, LY)
+    if s[t] > 0:
+        return None
+    sp[m], sp[t] = -1, 1
+    return sp
+
+
+def metropolis_chain(psi, s0, chain, seed, step0, n_steps, LX, LY, sur=None, margin=1e-6, kind=MOVE_EXCHANGE,
+                     ph_z=4, se_z=4):
+    """Replays chain `chain` of the GPU sweep: proposals from Philox4x32-10 at counter
+    (chain, step) by `propose`, acceptance with min(1, |psi'/psi|^2) or, with `sur` (a function
+    s -> f(s)), delayed acceptance.  psi is a function s -> (sign, logabs).  Returns (final
+    configuration, accepted count, near_tie), near_tie True when some acceptance test fell
+    within `margin` of its threshold (then a rounding-level difference can legitimately change
+    the trajectory).  The particle hop counts the particles of the configuration itself; the
+    kernel uses the model's n_up + n_dn, the same number for a well-formed configuration."""
     s = np.array(s0, dtype=np.int8, copy=True)
     cur = psi(s)
     f_cur = sur(s) if sur is not None else 0.0
-    dirs = [(1, 0), (-1, 0), (0, 1), (0, -1)]
     acc = 0
     tie = False
     for t in range(step0, step0 + n_steps):
         r = philox_draw(chain, t, seed)
-        i = r[0] % N
-        dx, dy = dirs[r[1] & 3]
-        spin = (r[1] >> 2) & 1
-        x, y = divmod(i, LY)
-        j = ((x + dx) % LX) * LY + (y + dy) % LY
-        mi, mj = spin * N + i, spin * N + j
-        if s[mi] == s[mj]:
+        sp = propose(s, r[0], r[1], LX, LY, kind, ph_z, se_z)
+        if sp is None:
             continue
         u1 = (r[2] >> 8) / 16777216.0
         u2 = (r[3] >> 8) / 16777216.0
-        sp = s.copy()
-        sp[mi], sp[mj] = s[mj], s[mi]
         f_new = 0.0
         if sur is not None:
             f_new = sur(sp)

@@ -207,6 +207,44 @@ def test_released_state_energy_per_site(name, prec):
         assert info.var_per_site / 1.5 < var_per_site < 1.5 * info.var_per_site
 
 
+def test_released_chains_under_the_papers_sampler():
+    """The paper's sampler (`released.make_sampler`: particle hops and site exchanges with the
+    state's nearest + next-nearest ranges, code/states.py:355-408) moves 1024 released
+    tpm02_L16_pfaffian chains by N steps each: both move kinds are accepted, the particle numbers
+    hold, and E/N of the moved chains stays at the paper's -0.74533(1) within four standard
+    errors (the released chains are equilibrated under the reference's sampler, so a sampler
+    that left |psi_P|^2 would drift them)."""
+    from neuralhubbard_amd.backflow import Hubbard, Samples, Variational, build_backflow
+    from neuralhubbard_amd.released import make_sampler
+
+    info, wk, p, S, _ = _released("tpm02_L16_pfaffian")
+    state = Variational(build_backflow(wk, p, "mixed"), symm=True)
+    K, N = 1024, wk.N
+    sm = make_sampler(state, info, K, S[:K], sweep_steps=N, thermal_steps=0, seed=5)
+    assert sm.proposal.kind == 3 and sm.proposal.ph_neighbors == 8 and sm.proposal.se_neighbors == 8
+    smp = sm.sweep()
+    st = sm.stats()
+    x = smp.spins.cpu().numpy()
+    moved = (x != S[:K]).any(1).mean()
+    print(f"mix sampler: {st}; chains moved {moved:.3f}", flush=True)
+    assert ((x[:, :N] > 0).sum(1) == info.n_up).all() and ((x[:, N:] > 0).sum(1) == info.n_up).all()
+    assert st["accepted"] > 0 and moved > 0.9
+    # moves that change the doublon count (particle hops onto or off doubly occupied sites) and
+    # moves that keep it (site exchanges of an up and a down electron among them) both occur
+    d0 = ((S[:K, :N] > 0) & (S[:K, N:] > 0)).sum(1)
+    d1 = ((x[:, :N] > 0) & (x[:, N:] > 0)).sum(1)
+    assert (d0 != d1).any()
+    H = Hubbard(U=wk.U, tprime=wk.tprime, chunk=64)
+    el = []
+    for b0 in range(0, K, 256):
+        s = smp.spins[b0:b0 + 256]
+        el.append(H.Oloc(state, Samples(s, state(s))).cpu().numpy())
+    e = np.concatenate(el) / N
+    mean, err = e.mean(), e.std(ddof=1) / np.sqrt(len(e))
+    print(f"after N mixed steps: E/N = {mean:.5f} +- {err:.5f} (paper {info.E_per_site})", flush=True)
+    assert abs(mean - info.E_per_site) < 4 * err + info.E_err
+
+
 def test_released_24x24_energy_per_site():
     """The released tpm02_L24_pfaffian state (24x24, 252 + 252 electrons, 504 x 504 Pfaffians, the
     residual CNN layer by layer): E/N over the first 32 fixture chains against the paper's

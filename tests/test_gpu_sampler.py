@@ -2,9 +2,13 @@
 
 * surrogate forward vs the FP64 oracle; gradients vs FP64 central differences of the oracle;
 * GPU chains replayed step by step by the oracle (same Philox draws), plain and delayed
-  acceptance: configurations bit-exact;
+  acceptance, for every move kind (same-spin exchange, particle hop, site exchange and their
+  mix, 4 or 8 neighbours): configurations bit-exact;
 * stationarity: the chain histogram on a 3x3 lattice (1296 states) against the exact
-  |psi_P|^2 / Z (chi-square), for plain Metropolis and for delayed acceptance;
+  |psi_P|^2 / Z (chi-square), for plain Metropolis and for delayed acceptance, with the
+  exchange and with the mix of particle hops and site exchanges; the chain energy of a 4x4
+  lattice against exact enumeration with both move sets;
+* the quantax-style constructors (ParticleHop, SiteExchange, MixSampler) run the mix;
 * delayed acceptance with a trained surrogate needs fewer exact evaluations.
 """
 
@@ -100,10 +104,27 @@ def _small_state(LX=4, LY=4, n_up=3, n_dn=3, channels=8, project=True):
     return wk, p, Variational(build_backflow(wk, p), symm=project), O.Model(wk, p)
 
 
-@pytest.mark.parametrize("delayed", [False, True])
-def test_metropolis_replays_oracle(delayed):
+MOVES = {  # name: (kind, ph_z, se_z)
+    "exchange": (O.MOVE_EXCHANGE, 4, 4),
+    "hop4": (O.MOVE_PARTICLE_HOP, 4, 4),
+    "swap8": (O.MOVE_SITE_EXCHANGE, 4, 8),
+    "mix8": (O.MOVE_MIX, 8, 8),
+    "mix48": (O.MOVE_MIX, 4, 8),
+}
+
+
+def _proposal(name):
+    from neuralhubbard_amd.sampler import Proposal
+
+    return Proposal(*MOVES[name])
+
+
+@pytest.mark.parametrize("delayed,moves", [(False, "exchange"), (True, "exchange"), (False, "hop4"),
+                                           (False, "swap8"), (False, "mix8"), (True, "mix48")])
+def test_metropolis_replays_oracle(delayed, moves):
     from neuralhubbard_amd.sampler import MetropolisSampler
 
+    kind, ph_z, se_z = MOVES[moves]
     wk, p, state, ref = _small_state()
     s0 = L.random_configurations(wk.geom, 6, 21)
     sur = None
@@ -112,7 +133,7 @@ def test_metropolis_replays_oracle(delayed):
         pp = sur.params.cpu().numpy()
         pp[-(9 * 16 + 1):] *= 20.0
         sur.params.copy_(torch.tensor(pp))
-    sm = MetropolisSampler(state, s0, sweep_steps=25, seed=1234, surrogate=sur)
+    sm = MetropolisSampler(state, s0, sweep_steps=25, seed=1234, surrogate=sur, proposal=_proposal(moves))
     sm.sweep()
     got = sm.s.cpu().numpy()
     sur_fn = None
@@ -121,7 +142,8 @@ def test_metropolis_replays_oracle(delayed):
         sur_fn = lambda x: O.sur_forward(pp, x, 4, 4, 16)
     checked = 0
     for c in range(6):
-        sf, acc, tie = O.metropolis_chain(ref.psi, s0[c], c, 1234, 0, 25, 4, 4, sur=sur_fn, margin=1e-4)
+        sf, acc, tie = O.metropolis_chain(ref.psi, s0[c], c, 1234, 0, 25, 4, 4, sur=sur_fn, margin=1e-4,
+                                          kind=kind, ph_z=ph_z, se_z=se_z)
         if tie:
             continue
         checked += 1
@@ -148,8 +170,8 @@ def _exact_distribution(ref, wk):
     return states, w / w.sum()
 
 
-@pytest.mark.parametrize("delayed", [False, True])
-def test_metropolis_stationary_distribution(delayed):
+@pytest.mark.parametrize("delayed,moves", [(False, "exchange"), (True, "exchange"), (False, "mix8"), (True, "mix8")])
+def test_metropolis_stationary_distribution(delayed, moves):
     """Chain histogram on a 3x3 lattice with 2 + 2 electrons vs |psi_P|^2 / Z: 32768
     independent chains, 1000 proposals of burn-in, then the state of every chain after each of
     two sweeps of 200 proposals; each sweep's 32768 states are independent draws (the two
@@ -162,7 +184,8 @@ def test_metropolis_stationary_distribution(delayed):
     n_chains = 32768
     s0 = L.random_configurations(wk.geom, n_chains, 5)
     sur = _sur(3, 3, 16, seed=4) if delayed else None
-    sm = MetropolisSampler(state, s0, sweep_steps=200, seed=99, surrogate=sur, thermal_steps=1000)
+    sm = MetropolisSampler(state, s0, sweep_steps=200, seed=99, surrogate=sur, thermal_steps=1000,
+                           proposal=_proposal(moves))
     for sweep in range(2):
         x = sm.sweep().spins.cpu().numpy()
         counts = np.bincount(np.array([key[s.tobytes()] for s in x]), minlength=len(states)).astype(float)
@@ -175,7 +198,7 @@ def test_metropolis_stationary_distribution(delayed):
             chi2 += (counts[~mask].sum() - rest) ** 2 / rest
             dof += 1
         pval = sps.chi2.sf(chi2, dof)
-        print(f"delayed={delayed} sweep {sweep}: chi2 {chi2:.1f} dof {dof} p {pval:.3g}")
+        print(f"delayed={delayed} {moves} sweep {sweep}: chi2 {chi2:.1f} dof {dof} p {pval:.3g}")
         assert pval > 1e-4
     print(sm.stats())
 
@@ -213,8 +236,8 @@ def test_trained_surrogate_reduces_exact_evaluations():
     assert gain > 1.2
 
 
-@pytest.mark.parametrize("delayed", [False, True])
-def test_chain_energy_matches_enumeration(delayed):
+@pytest.mark.parametrize("delayed,moves", [(False, "exchange"), (True, "exchange"), (False, "mix8")])
+def test_chain_energy_matches_enumeration(delayed, moves):
     """E/N from 2048 chains x 32 sweeps (block jackknife over sweeps) against the exact
     variational energy sum_s |psi_P(s)|^2 E_loc(s) / sum_s |psi_P(s)|^2 over all 14400
     configurations of a 4x4 lattice with 2 + 2 electrons."""
@@ -231,11 +254,41 @@ def test_chain_energy_matches_enumeration(delayed):
     E_exact = float((prob[live] * el[live]).sum()) / wk.N
     sur = _sur(4, 4, 16, seed=3) if delayed else None
     sm = MetropolisSampler(state, L.random_configurations(wk.geom, 2048, 9), sweep_steps=32, seed=11, surrogate=sur,
-                           thermal_steps=512)
+                           thermal_steps=512, proposal=_proposal(moves))
     per_sweep = []
     for _ in range(32):
         smp = sm.sweep()
         per_sweep.append(H.Oloc(state, smp).mean().item() / wk.N)
     E, err, _ = O.jackknife(lambda m: m, np.array(per_sweep))
-    print(f"delayed={delayed}: E/N {E:.6f} +- {err:.6f}, exact {E_exact:.6f}, stats {sm.stats()}")
+    print(f"delayed={delayed} {moves}: E/N {E:.6f} +- {err:.6f}, exact {E_exact:.6f}, stats {sm.stats()}")
     assert abs(E - E_exact) < 4 * err + 1e-6
+
+
+def test_mix_sampler_constructors_run_the_mix():
+    """MixSampler([ParticleHop(n/2, n_neighbor=(1, 2)), SiteExchange(n/2, (1, 2))]) as the reference
+    builds it: the chains run the mixed proposal (replayed by the oracle), keep their particle
+    numbers, and site exchanges change the doublon count only together with particle hops."""
+    from neuralhubbard_amd.sampler import MixSampler, ParticleHop, SiteExchange
+
+    wk, p, state, ref = _small_state()
+    s0 = L.random_configurations(wk.geom, 8, 33)
+    sm = MixSampler([ParticleHop(state, 4, n_neighbor=(1, 2)), SiteExchange(state, 4, n_neighbor=(1, 2))],
+                    sweep_steps=20, initial_spins=s0, seed=77)
+    assert sm.n_chains == 8 and sm.proposal.kind == O.MOVE_MIX and sm.proposal.ph_neighbors == 8
+    smp = sm.sweep()
+    got = smp.spins.cpu().numpy()
+    N = wk.N
+    assert ((got[:, :N] > 0).sum(1) == wk.n_up).all() and ((got[:, N:] > 0).sum(1) == wk.n_dn).all()
+    for c in range(8):
+        sf, _, tie = O.metropolis_chain(ref.psi, s0[c], c, 77, 0, 20, 4, 4, margin=1e-4, kind=O.MOVE_MIX, ph_z=8,
+                                        se_z=8)
+        if not tie:
+            assert np.array_equal(got[c], sf), c
+    # amplitudes carried by the chains equal a fresh evaluation
+    psi = state(smp.spins)
+    np.testing.assert_allclose(smp.psi.logabs.cpu().numpy(), psi.logabs.cpu().numpy(), rtol=0, atol=1e-9)
+    # a single-move sampler started lazily
+    hop = ParticleHop(state, 4, n_neighbor=1, initial_spins=s0[:4], sweep_steps=5, seed=3)
+    hop.sweep()
+    assert hop.stats()["proposals"] == 20
+
