@@ -106,6 +106,15 @@ int nh_pairing_ud(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch
  * Outputs sign (+1/-1, 0 if singular) and logabs (-inf if singular) per matrix.
  * Parlett-Reid elimination with partial pivoting, FP32, pivot logs summed in FP64. */
 int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs);
+/* The same for n <= 1024 (fermix.slogpf at the 32x32 sizes, n = 896) with a workspace: each
+ * matrix's strict lower triangle is packed into the workspace and eliminated there by the
+ * blocked Parlett-Reid routine of the image kernels (panel vectors in LDS).  A workspace of
+ * nh_slogpf_workspace_size(batch, n) bytes takes the batch in one launch; a smaller one (at
+ * least one matrix: nh_slogpf_workspace_size(1, n)) runs the batch in chunks that fit.
+ * n <= 64 needs no workspace (the result of nh_slogpf). */
+size_t nh_slogpf_workspace_size(int32_t batch, int32_t n);
+int nh_slogpf_ws(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs,
+                 void* workspace, size_t workspace_bytes);
 
 /* ---- a9: batched determinant  (replaces fermix.slogdet, called at code/backflow/models.py:297)
  * A [batch][n][n], n <= 192.  LU with partial pivoting, FP32, logs summed in FP64. */

@@ -67,6 +67,8 @@ SIGNATURES = {
     "nh_pairing_occ": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "nh_pairing_ud": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "nh_slogpf": (_I, [_P, _I, _I, _P, _P, _P]),
+    "nh_slogpf_workspace_size": (_S, [_I, _I]),
+    "nh_slogpf_ws": (_I, [_P, _I, _I, _P, _P, _P, _P, _S]),
     "nh_slogdet": (_I, [_P, _I, _I, _P, _P, _P]),
     "nh_pairing_occ_vjp": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "nh_slogpf_vjp": (_I, [_P, _I, _I, _P, _P, _P]),

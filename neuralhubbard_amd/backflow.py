@@ -147,16 +147,35 @@ def pairing_ud(Yu, Yd, iu, idn, Wt, LX: int, LY: int):
     return F
 
 
-def slogpf(F):
-    """Batched (sign, log|Pf|) of antisymmetric (B, n, n) float32 (``fermix.slogpf``)."""
+_SLOGPF_WS = None
+_SLOGPF_WS_CAP = 1 << 30  # bytes of packed triangles per launch; larger batches run in chunks
+
+
+def slogpf(F, workspace: bool | None = None):
+    """Batched (sign, log|Pf|) of antisymmetric (B, n, n) float32 (``fermix.slogpf``), n <= 1024.
+    workspace: use `nh_slogpf_ws` (triangles in a workspace); default for n > 256, where the
+    LDS form does not reach."""
+    global _SLOGPF_WS
     torch = _torch()
     _lib.require_gpu()
     F = F.contiguous().float()
     B, n, _ = F.shape
     sign = torch.empty(B, dtype=torch.float32, device=F.device)
     logabs = torch.empty(B, dtype=torch.float64, device=F.device)
-    _lib.check(_lib.load().nh_slogpf(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs)),
-               "nh_slogpf")
+    lib = _lib.load()
+    if workspace is None:
+        workspace = n > 256
+    if not workspace:
+        _lib.check(lib.nh_slogpf(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs)),
+                   "nh_slogpf")
+        return sign, logabs
+    per = max(1, lib.nh_slogpf_workspace_size(1, n) - 256)
+    nb = max(1, min(B, _SLOGPF_WS_CAP // per))
+    if _SLOGPF_WS is None:
+        _SLOGPF_WS = _Workspace()
+    ws = _SLOGPF_WS.get(lib.nh_slogpf_workspace_size(nb, n))
+    _lib.check(lib.nh_slogpf_ws(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs),
+                                _lib.ptr(ws), ws.numel()), "nh_slogpf_ws")
     return sign, logabs
 
 

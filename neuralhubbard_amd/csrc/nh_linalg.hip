@@ -763,6 +763,29 @@ __global__ void __launch_bounds__(NT) k_slogpf(int n, const float* __restrict__ 
   }
 }
 
+// Workspace form (64 < n <= NT): matrix b0 + blockIdx.x's strict lower triangle is packed into
+// its slice of the workspace (gstride floats, the triangle then kWave dummy floats) and
+// eliminated there; LDS holds only the panel scratch, so up to n = 256 four 256-thread
+// workgroups share a CU.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_slogpf_ws(int n, int b0, const float* __restrict__ A, float* sign,
+                                                  double* logabs, float* __restrict__ gtri, size_t gstride) {
+  extern __shared__ float lds[];
+  const int b = b0 + blockIdx.x;
+  float* tri = gtri + (size_t)blockIdx.x * gstride;
+  const float* Ab = A + (size_t)b * n * n;
+  for (int i = 1 + threadIdx.x / kWave; i < n; i += NT / kWave)
+    for (int j = threadIdx.x & (kWave - 1); j < i; j += kWave) tri[toff(i, j)] = Ab[(size_t)i * n + j];
+  __syncthreads();
+  float sg;
+  double la;
+  pf_blocked2<NT, kPanel, 10>(tri, n, lds, sg, la);
+  if (threadIdx.x == 0) {
+    sign[b] = sg;
+    logabs[b] = la;
+  }
+}
+
 template <int NT>
 __global__ void __launch_bounds__(NT) k_slogdet(int n, const float* __restrict__ A, float* sign,
                                                 double* logabs) {
@@ -917,6 +940,40 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   else
     hipLaunchKernelGGL(k_slogpf<1024>, dim3(batch), dim3(1024), lds, st, n, A, sign, logabs);
   NH_LAUNCH_CHECK();
+  return NH_OK;
+}
+
+static size_t slogpf_ws_stride(int n) { return (((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63; }
+
+extern "C" size_t nh_slogpf_workspace_size(int32_t batch, int32_t n) {
+  if (batch <= 0 || n <= 0 || n % 2 || n > 1024) return 0;
+  if (n <= 64) return 256;
+  return sizeof(float) * slogpf_ws_stride(n) * (size_t)batch + 256;
+}
+
+extern "C" int nh_slogpf_ws(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs,
+                            void* workspace, size_t workspace_bytes) {
+  if (batch < 0 || n < 0 || (batch > 0 && (!A || !sign || !logabs))) return NH_ERR_ARG;
+  if (n == 0 || n % 2 || n > 1024) return NH_ERR_SHAPE;
+  if (batch == 0) return NH_OK;
+  if (n <= 64) return nh_slogpf(stream, batch, n, A, sign, logabs);
+  if (!workspace) return NH_ERR_ARG;
+  const size_t gs = slogpf_ws_stride(n);
+  float* gtri = (float*)(((size_t)workspace + 255) & ~(size_t)255);
+  const size_t usable = workspace_bytes > 256 ? workspace_bytes - 256 : 0;
+  const int64_t fit = (int64_t)(usable / (sizeof(float) * gs));
+  if (fit < 1) return NH_ERR_WORKSPACE;
+  const int chunk = fit < batch ? (int)fit : batch;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = sizeof(float) * (size_t)pf2_scratch_floats<kPanel>(n);
+  for (int b0 = 0; b0 < batch; b0 += chunk) {
+    const int nb = batch - b0 < chunk ? batch - b0 : chunk;
+    if (n <= 256)
+      hipLaunchKernelGGL(k_slogpf_ws<256>, dim3(nb), dim3(256), lds, st, n, b0, A, sign, logabs, gtri, gs);
+    else
+      hipLaunchKernelGGL(k_slogpf_ws<1024>, dim3(nb), dim3(1024), lds, st, n, b0, A, sign, logabs, gtri, gs);
+    NH_LAUNCH_CHECK();
+  }
   return NH_OK;
 }
 

@@ -77,6 +77,55 @@ def test_slogpf_blocked_sizes(n):
         assert abs(la[b] - l_ref) < _cond_tol(A64, 2e-4 * n / 56), (n, b, la[b], l_ref)
 
 
+@pytest.mark.parametrize("n", [66, 130, 224, 256, 258, 384, 512, 896, 1024])
+def test_slogpf_workspace_form(n):
+    """nh_slogpf_ws (the triangles in a workspace, n <= 1024; n = 896 is the 32x32 Pfaffian):
+    sign against the FP64 oracle, log|Pf| against LAPACK's log|det| / 2 within the
+    condition-scaled tolerance of the LDS form, and a workspace of two matrices (the batch in
+    chunks) gives bit-identical results to one of the whole batch."""
+    import ctypes
+
+    from neuralhubbard_amd import _lib
+    from neuralhubbard_amd.backflow import slogpf
+
+    rng = np.random.default_rng(1000 + n)
+    B = 5 if n <= 512 else 3
+    A = np.stack([rand_skew(n, rng, 0.3) for _ in range(B)]).astype(np.float32)
+    At = torch.tensor(A, device="cuda")
+    sg, la = slogpf(At, workspace=True)
+    sg, la = sg.cpu().numpy(), la.cpu().numpy()
+    print(f"slogpf_ws n={n}", flush=True)
+    tol = []
+    for b in range(B):
+        A64 = A[b].astype(np.float64)
+        _, ld = np.linalg.slogdet(A64)
+        s_ref, _ = O.slogpf(A64) if n <= 512 else (None, None)
+        if s_ref is not None:
+            assert sg[b] == s_ref, (n, b)
+        tol.append(_cond_tol(A64, 2e-4 * n / 56))
+        print(f"  b={b}: |d log| vs LAPACK {abs(la[b] - ld / 2):.2e} (tolerance {tol[-1]:.2e})", flush=True)
+        assert abs(la[b] - ld / 2) < tol[-1], (n, b, la[b], ld / 2)
+    if n > 512:  # sign (the FP64 oracle is slow here): swapping two rows and columns flips it
+        P = np.arange(n)
+        P[[0, 1]] = P[[1, 0]]
+        sg2, la2 = slogpf(torch.tensor(A[:, P][:, :, P].copy(), device="cuda"), workspace=True)
+        assert (sg2.cpu().numpy() == -sg).all()
+        assert (np.abs(la2.cpu().numpy() - la) < 2 * np.array(tol)).all()
+    lib = _lib.load()
+    two = lib.nh_slogpf_workspace_size(2, n)
+    assert two < lib.nh_slogpf_workspace_size(B, n)
+    ws = torch.empty(two, dtype=torch.uint8, device="cuda")
+    sgc = torch.empty(B, dtype=torch.float32, device="cuda")
+    lac = torch.empty(B, dtype=torch.float64, device="cuda")
+    _lib.check(lib.nh_slogpf_ws(_lib.stream_handle(), B, n, _lib.ptr(At), _lib.ptr(sgc), _lib.ptr(lac),
+                                _lib.ptr(ws), ctypes.c_size_t(two)), "nh_slogpf_ws")
+    np.testing.assert_array_equal(sgc.cpu().numpy(), sg)
+    np.testing.assert_array_equal(lac.cpu().numpy(), la)
+    # too small a workspace is refused before any launch
+    assert lib.nh_slogpf_ws(_lib.stream_handle(), B, n, _lib.ptr(At), _lib.ptr(sgc), _lib.ptr(lac), _lib.ptr(ws),
+                            ctypes.c_size_t(lib.nh_slogpf_workspace_size(1, n) // 2)) == _lib.NH_ERR_WORKSPACE
+
+
 @pytest.mark.parametrize("n", [65, 100, 112, 127, 128, 129, 160, 192])
 def test_slogdet_blocked_sizes(n):
     """Blocked LU at sizes around its 16-column panels and thread-count switch (n = 128)."""

@@ -151,6 +151,11 @@ def test_argument_errors_without_gpu():
     assert lib.nh_slogpf(None, 4, 7, None, None, None) == _lib.NH_ERR_ARG
     assert lib.nh_slogpf(None, 1, 300, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16)) == _lib.NH_ERR_SHAPE
     assert lib.nh_slogdet(None, 1, 500, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_void_p(16)) == _lib.NH_ERR_SHAPE
+    v = ctypes.c_void_p(16)
+    assert lib.nh_slogpf_ws(None, 1, 1026, v, v, v, v, 1 << 30) == _lib.NH_ERR_SHAPE
+    assert lib.nh_slogpf_ws(None, 1, 895, v, v, v, v, 1 << 30) == _lib.NH_ERR_SHAPE
+    assert lib.nh_slogpf_ws(None, 2, 896, v, v, v, v, lib.nh_slogpf_workspace_size(1, 896) - 1024) == _lib.NH_ERR_WORKSPACE
+    assert lib.nh_slogpf_workspace_size(1, 896) == 4 * ((896 * 895 // 2 + 64 + 63) // 64 * 64) + 256
     # the layer-by-layer network needs its workspace: too small a one is refused before any launch
     R = _lib.NH_NET_RESCONV
     big = _lib.NhModel(0, 24, 24, 252, 252, 64, 16, 0, 1, R, 16, 16, 16, 0, 0)
