@@ -139,6 +139,26 @@ int nh_pairing_occ_vjp(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t 
  * Gauss-Jordan with partial pivoting in FP32 on G itself; a singular F gives NaN. */
 int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G);
 
+/* ---- f2: reverse mode of the BCS up x down pairing block
+ *      (replaces the backward of bcs_kernel.pairing_ud's custom_vjp, code/backflow/bcs_kernel.py:274-288,
+ *      kernels _dy_kernel :123-141 (both sides) and _dw_kernel :169-206)
+ * For the cotangent G [batch][n_up][n_dn] of F = nh_pairing_ud(Yu, Yd, iu, idn, Wt), per sample:
+ *   dYu[b,p,a] = sum_q G[p,q] Wt[r(iu_p - idn_q)][a] Yd[b,q,a]
+ *   dYd[b,q,a] = sum_p G[p,q] Wt[r(iu_p - idn_q)][a] Yu[b,p,a]
+ *   dWt[b][r][a] = sum over (p, q) with r(iu_p - idn_q) = r of G[p,q] Yu[p,a] Yd[q,a].
+ * dYu [batch][n_up][d], dYd [batch][n_dn][d], dWt [batch][N][d] (device; any may be NULL to skip
+ * it).  The down sites of a sample are distinct; d divides 256 and d <= 64, N <= 1024,
+ * batch <= 65535; a site outside [0, N) gives NaN rows. */
+int nh_pairing_ud_vjp(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch, int32_t n_up,
+                      int32_t n_dn, const float* Yu, const float* Yd, const int32_t* iu, const int32_t* idn,
+                      const float* Wt, const float* G, float* dYu, float* dYd, float* dWt);
+
+/* ---- f2: determinant VJP (the reverse mode of fermix.slogdet, code/backflow/models.py:297)
+ * G[b] = cot[b] * d log|det F[b]| / dF[b] = cot[b] * F[b]^{-T}; cot NULL means 1.  F, G
+ * [batch][n][n] (device, distinct buffers), n <= 1024; the Gauss-Jordan kernel of nh_slogpf_vjp;
+ * a singular F gives NaN. */
+int nh_slogdet_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G);
+
 /* ---- a4: backflow features  (replaces PfaffianBackflow.features, code/backflow/models.py:163-167)
  * s int8 [batch][2N] -> y float [batch][2][N][d]  (no projection; the identity image). */
 int nh_features(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* y);

@@ -72,6 +72,8 @@ SIGNATURES = {
     "nh_slogdet": (_I, [_P, _I, _I, _P, _P, _P]),
     "nh_pairing_occ_vjp": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "nh_slogpf_vjp": (_I, [_P, _I, _I, _P, _P, _P]),
+    "nh_slogdet_vjp": (_I, [_P, _I, _I, _P, _P, _P]),
+    "nh_pairing_ud_vjp": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "nh_features": (_I, [_P, _MP, _I, _P, _P]),
     "nh_features_workspace_size": (_S, [_MP, _I]),
     "nh_features_ws": (_I, [_P, _MP, _I, _P, _P, _P, _S]),

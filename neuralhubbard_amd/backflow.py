@@ -224,6 +224,39 @@ def slogpf_vjp(F, cot=None):
     return G
 
 
+def pairing_ud_vjp(Yu, Yd, iu, idn, Wt, G, LX: int, LY: int):
+    """(dYu (B, n_up, d), dYd (B, n_dn, d), dWt (B, N, d)): the VJP of `pairing_ud` for the
+    cotangent G (B, n_up, n_dn), per sample (the backward of ``pairing_ud``'s custom_vjp,
+    ``bcs_kernel.py:274-288``)."""
+    torch = _torch()
+    _lib.require_gpu()
+    Yu, Yd = Yu.contiguous().float(), Yd.contiguous().float()
+    iu, idn = iu.contiguous().to(torch.int32), idn.contiguous().to(torch.int32)
+    Wt, G = Wt.contiguous().float(), G.contiguous().float()
+    B, nu, d = Yu.shape
+    nd = Yd.shape[1]
+    dYu = torch.empty((B, nu, d), dtype=torch.float32, device=Yu.device)
+    dYd = torch.empty((B, nd, d), dtype=torch.float32, device=Yu.device)
+    dWt = torch.empty((B, LX * LY, d), dtype=torch.float32, device=Yu.device)
+    _lib.check(_lib.load().nh_pairing_ud_vjp(_lib.stream_handle(), LX, LY, d, B, nu, nd, _lib.ptr(Yu), _lib.ptr(Yd),
+                                             _lib.ptr(iu), _lib.ptr(idn), _lib.ptr(Wt), _lib.ptr(G), _lib.ptr(dYu),
+                                             _lib.ptr(dYd), _lib.ptr(dWt)), "nh_pairing_ud_vjp")
+    return dYu, dYd, dWt
+
+
+def slogdet_vjp(F, cot=None):
+    """cot * d log|det F| / dF = cot * F^{-T} for (B, n, n) float32."""
+    torch = _torch()
+    _lib.require_gpu()
+    F = F.contiguous().float()
+    B, n, _ = F.shape
+    G = torch.empty_like(F)
+    c = None if cot is None else torch.as_tensor(cot, dtype=torch.float32, device=F.device).contiguous()
+    _lib.check(_lib.load().nh_slogdet_vjp(_lib.stream_handle(), B, n, _lib.ptr(F), 0 if c is None else _lib.ptr(c),
+                                          _lib.ptr(G)), "nh_slogdet_vjp")
+    return G
+
+
 # ------------------------------------------------------------------------ models
 
 

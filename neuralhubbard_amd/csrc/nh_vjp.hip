@@ -1,5 +1,5 @@
-// Reverse-mode pieces of the amplitude (gfx950): the VJP of the occupied pairing block and the
-// Pfaffian VJP.
+// Reverse-mode pieces of the amplitude (gfx950): the VJPs of the occupied pairing block (Pfaffian
+// states) and of the up x down pairing block (BCS states), and the Pfaffian and determinant VJPs.
 //
 // nh_pairing_occ_vjp replaces the backward of the reference's pairing_occ custom_vjp
 // (code/backflow/pf_kernel.py:354-365): for the cotangent G of F = pairing_occ(Y, idx, Wt),
@@ -10,6 +10,14 @@
 // the table identity Wt[2 s_q + s_p][-r] = -Wt[2 s_p + s_q][r] of build_wtilde.  dWt loops over the
 // occupied p and finds the partner q of (p, r, s_q) through the inverse mode map, so every
 // output element is one thread's plain sum (no atomics, fixed order).
+//
+// nh_pairing_ud_vjp replaces the backward of the reference's BCS pairing_ud custom_vjp
+// (code/backflow/bcs_kernel.py:274-288): for the cotangent G (n_up x n_dn) of
+// F[p, q] = sum_a Yu[p, a] Wt[r(iu_p - idn_q)][a] Yd[q, a],
+//   dYu[p, a] = sum_q G[p, q] Wt[r(iu_p - idn_q)][a] Yd[q, a]      (k_vjp_ud_dy, side 0)
+//   dYd[q, a] = sum_p G[p, q] Wt[r(iu_p - idn_q)][a] Yu[p, a]      (k_vjp_ud_dy, side 1)
+//   dWt[r][a] = sum_p G[p, q(p, r)] Yu[p, a] Yd[q(p, r), a]          (k_vjp_ud_dw)
+// with q(p, r) the occupied down site iu_p - r (at most one), found through the inverse site map.
 //
 // nh_slogpf_vjp is the Pfaffian VJP: d log|Pf F| / dF = (1/2) F^{-T} (d Pf = (1/2) Pf tr(F^{-1} dF)),
 // scaled by the cotangent of log|Pf|.  It inverts the transposed copy of F in place in the output
@@ -112,9 +120,104 @@ __global__ void __launch_bounds__(kVjpThreads) k_vjp_dw(int LX, int LY, int d, i
   for (int blk = 0; blk < 4; ++blk) out[((size_t)blk * N + r) * d + a] = acc[blk];
 }
 
-// In-place inverse of the transposed copy of F (so X = F^{-T}), scaled by 0.5 * cot.
+// One thread per (own row, a): side 0 rows are the up sites (partners down), side 1 the down
+// sites (partners up); the displacement is always r(up site - down site) and G is [n_up][n_dn].
+__global__ void __launch_bounds__(kVjpThreads)
+    k_vjp_ud_dy(int LX, int LY, int d, int n_up, int n_dn, int side, const float* __restrict__ Yoth,
+                const int32_t* __restrict__ iown, const int32_t* __restrict__ ioth, const float* __restrict__ Wt,
+                const float* __restrict__ G, float* __restrict__ dYown) {
+  extern __shared__ int vj_lds[];
+  int* coord = vj_lds;  // (x << 15) | y of each partner site, -1 outside the lattice
+  const int b = blockIdx.y;
+  const int N = LX * LY;
+  const int n_own = side ? n_dn : n_up, n_oth = side ? n_up : n_dn;
+  const int32_t* ob = ioth + (size_t)b * n_oth;
+  for (int q = threadIdx.x; q < n_oth; q += kVjpThreads) {
+    const int site = ob[q];
+    coord[q] = (site < 0 || site >= N) ? -1 : (((site / LY) << 15) | (site % LY));
+  }
+  __syncthreads();
+  const int rows = kVjpThreads / d;
+  const int pl = threadIdx.x / d, a = threadIdx.x - pl * d;
+  const int p = blockIdx.x * rows + pl;
+  if (p >= n_own) return;
+  float* out = dYown + ((size_t)b * n_own + p) * d + a;
+  const int site_p = iown[(size_t)b * n_own + p];
+  if (site_p < 0 || site_p >= N) {
+    *out = NAN;
+    return;
+  }
+  const int xp = site_p / LY, yp = site_p % LY;
+  const float* Gb = G + (size_t)b * n_up * n_dn;
+  const float* Yb = Yoth + (size_t)b * n_oth * d;
+  float acc = 0.f;
+  for (int q = 0; q < n_oth; ++q) {
+    const int cq = coord[q];
+    if (cq < 0) {
+      acc = NAN;
+      continue;
+    }
+    int dx = xp - (cq >> 15), dy = yp - (cq & 0x7fff);
+    if (side) {  // r(up - down) with the up site the partner
+      dx = -dx;
+      dy = -dy;
+    }
+    dx += dx < 0 ? LX : 0;
+    dy += dy < 0 ? LY : 0;
+    const float g = side ? Gb[(size_t)q * n_dn + p] : Gb[(size_t)p * n_dn + q];
+    acc = fmaf(g * Wt[(size_t)(dx * LY + dy) * d + a], Yb[(size_t)q * d + a], acc);
+  }
+  *out = acc;
+}
+
+// One thread per (displacement r, a): the sum over the occupied up sites p of
+// G[p, q] Yu[p, a] Yd[q, a] with q the occupied down site at site_p - r.
+__global__ void __launch_bounds__(kVjpThreads)
+    k_vjp_ud_dw(int LX, int LY, int d, int n_up, int n_dn, const float* __restrict__ Yu,
+                const float* __restrict__ Yd, const int32_t* __restrict__ iu, const int32_t* __restrict__ idn,
+                const float* __restrict__ G, float* __restrict__ dWt) {
+  extern __shared__ int vj_lds[];
+  const int N = LX * LY;
+  int* inv = vj_lds;     // [N]: position of the down electron on each site, -1 if empty
+  int* cu = inv + N;     // [n_up]: up site, -1 outside the lattice
+  const int b = blockIdx.y;
+  for (int m = threadIdx.x; m < N; m += kVjpThreads) inv[m] = -1;
+  __syncthreads();
+  for (int q = threadIdx.x; q < n_dn; q += kVjpThreads) {
+    const int site = idn[(size_t)b * n_dn + q];
+    if (site >= 0 && site < N) inv[site] = q;
+  }
+  for (int p = threadIdx.x; p < n_up; p += kVjpThreads) {
+    const int site = iu[(size_t)b * n_up + p];
+    cu[p] = (site < 0 || site >= N) ? -1 : site;
+  }
+  __syncthreads();
+  const int rs = kVjpThreads / d;
+  const int rl = threadIdx.x / d, a = threadIdx.x - rl * d;
+  const int r = blockIdx.x * rs + rl;
+  if (r >= N) return;
+  const int dxr = r / LY, dyr = r - dxr * LY;
+  const float* Gb = G + (size_t)b * n_up * n_dn;
+  const float* Yub = Yu + (size_t)b * n_up * d;
+  const float* Ydb = Yd + (size_t)b * n_dn * d;
+  float acc = 0.f;
+  for (int p = 0; p < n_up; ++p) {
+    const int site = cu[p];
+    if (site < 0) continue;
+    int xq = site / LY - dxr, yq = site % LY - dyr;
+    xq += xq < 0 ? LX : 0;
+    yq += yq < 0 ? LY : 0;
+    const int q = inv[xq * LY + yq];
+    if (q >= 0) acc = fmaf(Gb[(size_t)p * n_dn + q] * Yub[(size_t)p * d + a], Ydb[(size_t)q * d + a], acc);
+  }
+  dWt[((size_t)b * N + r) * d + a] = acc;
+}
+
+// In-place inverse of the transposed copy of F (so X = F^{-T}), scaled by factor * cot
+// (factor 1/2 for the Pfaffian, 1 for the determinant).
 __global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __restrict__ F,
-                                                        const float* __restrict__ cot, float* __restrict__ G) {
+                                                        const float* __restrict__ cot, float factor,
+                                                        float* __restrict__ G) {
   extern __shared__ float vf_lds[];
   float* rowk = vf_lds;             // [n] pivot row before scaling
   float* oldk = rowk + n;           // [n] row k before the interchange
@@ -191,7 +294,7 @@ __global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __re
       }
     }
   }
-  const float scale = 0.5f * (cot ? cot[b] : 1.0f);
+  const float scale = factor * (cot ? cot[b] : 1.0f);
   for (int i = 0; i < n; ++i) {
     __syncthreads();  // perm ready; the previous row's reads of rowk are done
     for (int j = t; j < n; j += kVjpThreads) rowk[j] = A[(size_t)i * n + j];
@@ -232,13 +335,53 @@ extern "C" int nh_pairing_occ_vjp(void* stream, int32_t LX, int32_t LY, int32_t 
   return NH_OK;
 }
 
-extern "C" int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G) {
+static int inverse_transpose(void* stream, int32_t batch, int32_t n, const float* F, const float* cot,
+                             float factor, float* G) {
   if (batch < 0 || n < 0) return NH_ERR_ARG;
   if (batch == 0 || n == 0) return NH_OK;
   if (!F || !G || F == G) return NH_ERR_ARG;
   if (n > 1024) return NH_ERR_SHAPE;
   const size_t lds = sizeof(float) * 3 * (size_t)n + sizeof(int) * (2 * (size_t)n + 1);
-  hipLaunchKernelGGL(k_pf_vjp, dim3(batch), dim3(kVjpThreads), lds, (hipStream_t)stream, n, F, cot, G);
+  hipLaunchKernelGGL(k_pf_vjp, dim3(batch), dim3(kVjpThreads), lds, (hipStream_t)stream, n, F, cot, factor, G);
   NH_LAUNCH_CHECK();
+  return NH_OK;
+}
+
+extern "C" int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G) {
+  return inverse_transpose(stream, batch, n, F, cot, 0.5f, G);
+}
+
+extern "C" int nh_slogdet_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G) {
+  return inverse_transpose(stream, batch, n, F, cot, 1.0f, G);
+}
+
+extern "C" int nh_pairing_ud_vjp(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch, int32_t n_up,
+                                 int32_t n_dn, const float* Yu, const float* Yd, const int32_t* iu,
+                                 const int32_t* idn, const float* Wt, const float* G, float* dYu, float* dYd,
+                                 float* dWt) {
+  if (batch < 0 || n_up < 0 || n_dn < 0 || LX < 1 || LY < 1) return NH_ERR_ARG;
+  if (batch == 0 || n_up == 0 || n_dn == 0) return NH_OK;
+  if (!Yu || !Yd || !iu || !idn || !Wt || !G || (!dYu && !dYd && !dWt)) return NH_ERR_ARG;
+  const int N = LX * LY;
+  if (N > 1024 || n_up > N || n_dn > N || d < 1 || d > 64 || (kVjpThreads % d) != 0 || LX >= (1 << 15) ||
+      LY >= (1 << 15) || batch > 65535)
+    return NH_ERR_SHAPE;
+  hipStream_t st = (hipStream_t)stream;
+  const int rows = kVjpThreads / d;
+  if (dYu) {
+    hipLaunchKernelGGL(k_vjp_ud_dy, dim3((n_up + rows - 1) / rows, batch), dim3(kVjpThreads), sizeof(int) * n_dn, st,
+                       LX, LY, d, n_up, n_dn, 0, Yd, iu, idn, Wt, G, dYu);
+    NH_LAUNCH_CHECK();
+  }
+  if (dYd) {
+    hipLaunchKernelGGL(k_vjp_ud_dy, dim3((n_dn + rows - 1) / rows, batch), dim3(kVjpThreads), sizeof(int) * n_up, st,
+                       LX, LY, d, n_up, n_dn, 1, Yu, idn, iu, Wt, G, dYd);
+    NH_LAUNCH_CHECK();
+  }
+  if (dWt) {
+    hipLaunchKernelGGL(k_vjp_ud_dw, dim3((N + rows - 1) / rows, batch), dim3(kVjpThreads), sizeof(int) * (N + n_up),
+                       st, LX, LY, d, n_up, n_dn, Yu, Yd, iu, idn, G, dWt);
+    NH_LAUNCH_CHECK();
+  }
   return NH_OK;
 }

@@ -30,7 +30,9 @@ Pinning status per function (see the tests named in brackets):
                             <Delta+_0 Delta_r> built directly on the
                             Fock space over all bonds                 [test_oracle_dwave_*]
   pairing_occ_vjp /         finite differences of <G, F(Y, Wt)> and of
-  slogpf_vjp                log|Pf| along random skew directions      [test_oracle_vjp_*]
+  slogpf_vjp /              log|Pf| / log|det| along random directions
+  pairing_ud_vjp /          (skew ones for the Pfaffian); the BCS dW by
+  slogdet_vjp               a direct loop over displacements          [test_oracle_vjp_*]
   slogpf                    Pf^2 = det, exact rational expansion,
                             Pf(B A B^T) = det(B) Pf(A), block form    [test_oracle_pfaffian]
   slogdet                   numpy (LAPACK) slogdet                   [test_oracle_det]
@@ -245,6 +247,32 @@ def slogpf_vjp(A):
     """d log|Pf A| / dA with every entry independent: (1/2) A^{-T} (from d Pf = (1/2) Pf
     tr(A^{-1} dA)), the Pfaffian VJP behind ``fermix.slogpf`` at ``code/backflow/models.py:210``."""
     return 0.5 * np.linalg.inv(np.asarray(A, np.float64)).T
+
+
+def pairing_ud_vjp(Yu, Yd, sites_u, sites_d, Wt, G, LX, LY):
+    """Cotangents (dYu (n_up, d), dYd (n_dn, d), dWt (N, d)) of F = `pairing_ud` for the cotangent
+    G (n_up, n_dn): the VJP that ``code/backflow/bcs_kernel.py:274-288`` computes with its
+    ``_dy_kernel`` (both sides) and ``_dw_kernel``.
+    dYu[p] = sum_q G[p, q] Wt[r(p, q)] Yd[q];  dYd[q] = sum_p G[p, q] Wt[r(p, q)] Yu[p];
+    dWt[r] = sum over pairs (p, q) with r(p, q) = r of G[p, q] Yu[p] Yd[q]."""
+    disp, _, _ = lattice_tables(LX, LY)
+    N = LX * LY
+    r = disp[np.asarray(sites_u)[:, None], np.asarray(sites_d)[None, :]]
+    Yu = np.asarray(Yu, np.float64)
+    Yd = np.asarray(Yd, np.float64)
+    G = np.asarray(G, np.float64)
+    W = np.asarray(Wt, np.float64)[r]  # (n_up, n_dn, d)
+    dYu = np.einsum("pq,pqa,qa->pa", G, W, Yd)
+    dYd = np.einsum("pq,pqa,pa->qa", G, W, Yu)
+    dWt = np.zeros((N, Yu.shape[1]))
+    np.add.at(dWt, r.ravel(), (G[:, :, None] * Yu[:, None, :] * Yd[None, :, :]).reshape(-1, Yu.shape[1]))
+    return dYu, dYd, dWt
+
+
+def slogdet_vjp(A):
+    """d log|det A| / dA = A^{-T} (Jacobi's formula), the VJP behind ``fermix.slogdet`` at
+    ``code/backflow/models.py:297``."""
+    return np.linalg.inv(np.asarray(A, np.float64)).T
 
 
 def pairing_full(y, W, theta, LX, LY):

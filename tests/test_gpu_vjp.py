@@ -1,7 +1,9 @@
-"""Reverse mode of the pairing block and the Pfaffian on the GPU (marked gpu), against the FP64
-oracle's VJPs (pinned by finite differences in tests/test_oracle.py): nh_pairing_occ_vjp
-(dY, dWt per sample) and nh_slogpf_vjp ((1/2) F^{-T}), and their chain, the gradient of
-log|Pf F(Y, Wt)| with respect to the features and the table, against an FP64 finite difference.
+"""Reverse mode of the pairing blocks, the Pfaffian and the determinant on the GPU (marked gpu),
+against the FP64 oracle's VJPs (pinned by finite differences in tests/test_oracle.py):
+nh_pairing_occ_vjp (dY, dWt per sample), nh_pairing_ud_vjp (dYu, dYd, dWt per sample),
+nh_slogpf_vjp ((1/2) F^{-T}) and nh_slogdet_vjp (F^{-T}), and their chains, the gradients of
+log|Pf F(Y, Wt)| and of log|det F_ud(Yu, Yd, Wt)| with respect to the features and the table,
+against FP64 finite differences.
 Tolerances: 1e-5 of the largest magnitude for the pairing VJP (FP32 sums of n terms), 5e-4 for
 the FP32 Gauss-Jordan inverse of random antisymmetric matrices (condition numbers up to ~1e3
 here; an FP32 emulation of the same elimination reaches 9e-5 at n = 300)."""
@@ -98,3 +100,72 @@ def test_log_pfaffian_gradient_chain():
     got = float(np.sum(dY * vY) + np.sum(dWt * V))
     print(f"directional derivative: GPU {got:.6f}, FP64 central difference {fd:.6f}", flush=True)
     assert got == pytest.approx(fd, rel=1e-3, abs=1e-3)
+
+
+def _ud_case(LX, LY, nu, nd, d, B, seed):
+    rng = np.random.default_rng(seed)
+    N = LX * LY
+    Wt = O.build_wt_ud(rng.standard_normal((N, d)) * 30.0, 0.4, LX, LY).astype(np.float32)
+    iu = np.stack([np.sort(rng.choice(N, nu, replace=False)) for _ in range(B)]).astype(np.int32)
+    idn = np.stack([np.sort(rng.choice(N, nd, replace=False)) for _ in range(B)]).astype(np.int32)
+    Yu = rng.standard_normal((B, nu, d)).astype(np.float32)
+    Yd = rng.standard_normal((B, nd, d)).astype(np.float32)
+    G = rng.standard_normal((B, nu, nd)).astype(np.float32)
+    return Wt, iu, idn, Yu, Yd, G
+
+
+@pytest.mark.parametrize("LX,LY,nu,nd,d,B", [(4, 3, 5, 4, 4, 3), (8, 8, 28, 28, 16, 3), (16, 16, 112, 112, 64, 2),
+                                             (32, 32, 448, 448, 64, 1)])
+def test_bcs_pairing_vjp_matches_oracle(LX, LY, nu, nd, d, B):
+    from neuralhubbard_amd.backflow import pairing_ud_vjp
+
+    print(f"bcs pairing vjp {LX}x{LY} {nu}+{nd} d={d} batch={B}", flush=True)
+    Wt, iu, idn, Yu, Yd, G = _ud_case(LX, LY, nu, nd, d, B, seed=LX + nu)
+    t = lambda x: torch.tensor(x, device="cuda")  # noqa: E731
+    dYu, dYd, dWt = (x.cpu().numpy() for x in pairing_ud_vjp(t(Yu), t(Yd), t(iu), t(idn), t(Wt), t(G), LX, LY))
+    for b in range(B):
+        rYu, rYd, rW = O.pairing_ud_vjp(Yu[b], Yd[b], iu[b], idn[b], Wt, G[b], LX, LY)
+        for got, ref in ((dYu[b], rYu), (dYd[b], rYd), (dWt[b], rW)):
+            np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n,B", [(5, 4), (56, 3), (112, 3), (448, 1)])
+def test_determinant_vjp_matches_oracle(n, B):
+    from neuralhubbard_amd.backflow import slogdet_vjp
+
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((B, n, n)).astype(np.float32)
+    cot = rng.standard_normal(B).astype(np.float32)
+    G = slogdet_vjp(torch.tensor(A, device="cuda"), torch.tensor(cot, device="cuda")).cpu().numpy()
+    for b in range(B):
+        ref = cot[b] * O.slogdet_vjp(A[b])
+        print(f"det vjp n={n} b={b}: max |d| / max |ref| {np.abs(G[b] - ref).max() / np.abs(ref).max():.2e}", flush=True)
+        np.testing.assert_allclose(G[b], ref, rtol=0, atol=5e-4 * np.abs(ref).max())
+
+
+def test_log_determinant_gradient_chain():
+    """GPU F = pairing_ud(Yu, Yd, iu, idn, Wt), G = slogdet_vjp(F), (dYu, dYd, dWt) =
+    pairing_ud_vjp(G): the directional derivative of log|det F| along random (vu, vd, vW) against
+    an FP64 central difference."""
+    from neuralhubbard_amd.backflow import pairing_ud, pairing_ud_vjp, slogdet_vjp
+
+    LX, LY, n, d = 8, 8, 28, 16
+    rng = np.random.default_rng(12)
+    Wt, iu, idn, Yu, Yd, _ = _ud_case(LX, LY, n, n, d, 1, seed=31)
+    t = lambda x: torch.tensor(x, device="cuda")  # noqa: E731
+    F = pairing_ud(t(Yu), t(Yd), t(iu), t(idn), t(Wt), LX, LY)
+    G = slogdet_vjp(F)
+    dYu, dYd, dWt = (x.cpu().numpy()[0] for x in pairing_ud_vjp(t(Yu), t(Yd), t(iu), t(idn), t(Wt), G, LX, LY))
+    vu, vd, vW = rng.standard_normal((n, d)), rng.standard_normal((n, d)), rng.standard_normal(Wt.shape)
+    h = 1e-5
+
+    def logdet(s):
+        F64 = O.pairing_ud(Yu[0].astype(np.float64) + s * vu, Yd[0].astype(np.float64) + s * vd, iu[0], idn[0],
+                           Wt.astype(np.float64) + s * vW, LX, LY)
+        return O.slogdet(F64)[1]
+
+    fd = (logdet(h) - logdet(-h)) / (2 * h)
+    got = float(np.sum(dYu * vu) + np.sum(dYd * vd) + np.sum(dWt * vW))
+    print(f"directional derivative of log|det|: GPU {got:.6f}, FP64 central difference {fd:.6f}", flush=True)
+    assert got == pytest.approx(fd, rel=1e-3, abs=1e-3)
+

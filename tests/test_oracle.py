@@ -441,6 +441,47 @@ def test_oracle_vjp_pairing_finite_differences():
     np.testing.assert_allclose(np.einsum("pq,pqa,qa->pa", G - G.T, Wpq, Y), dY, rtol=1e-12, atol=1e-12)
 
 
+def test_oracle_vjp_bcs_pairing_finite_differences():
+    """<G, F_ud(Yu + h vu, Yd + h vd, Wt + h vW)> by central differences against
+    <vu, dYu> + <vd, dYd> + <vW, dWt>; dWt also against a direct loop over the pairs."""
+    rng = np.random.default_rng(8)
+    LX, LY, d = 4, 3, 3
+    N = LX * LY
+    iu = np.sort(rng.choice(N, 5, replace=False))
+    idn = np.sort(rng.choice(N, 4, replace=False))
+    Yu, Yd = rng.standard_normal((5, d)), rng.standard_normal((4, d))
+    Wt = rng.standard_normal((N, d))
+    G = rng.standard_normal((5, 4))
+    dYu, dYd, dWt = O.pairing_ud_vjp(Yu, Yd, iu, idn, Wt, G, LX, LY)
+    for _ in range(3):
+        vu, vd, vW = rng.standard_normal(Yu.shape), rng.standard_normal(Yd.shape), rng.standard_normal(Wt.shape)
+        h = 1e-6
+
+        def f(t):
+            return np.sum(G * O.pairing_ud(Yu + t * vu, Yd + t * vd, iu, idn, Wt + t * vW, LX, LY))
+
+        fd = (f(h) - f(-h)) / (2 * h)
+        assert fd == pytest.approx(np.sum(dYu * vu) + np.sum(dYd * vd) + np.sum(dWt * vW), rel=1e-7, abs=1e-9)
+    ref = np.zeros((N, d))
+    for p in range(5):
+        for q in range(4):
+            (xp, yp), (xq, yq) = divmod(int(iu[p]), LY), divmod(int(idn[q]), LY)
+            ref[((xp - xq) % LX) * LY + (yp - yq) % LY] += G[p, q] * Yu[p] * Yd[q]
+    np.testing.assert_allclose(dWt, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_vjp_determinant_finite_differences():
+    rng = np.random.default_rng(9)
+    n = 9
+    A = rng.standard_normal((n, n))
+    G = O.slogdet_vjp(A)
+    for _ in range(3):
+        V = rng.standard_normal((n, n))
+        h = 1e-6
+        fd = (O.slogdet(A + h * V)[1] - O.slogdet(A - h * V)[1]) / (2 * h)
+        assert fd == pytest.approx(np.sum(G * V), rel=1e-6, abs=1e-8)
+
+
 def test_oracle_vjp_pfaffian_finite_differences():
     rng = np.random.default_rng(5)
     n = 12
