@@ -147,7 +147,7 @@ int launch_cnn(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t
     return launch_resconv(st, m, s, item_b, item_hop, img0, img_end, ipi, Y, scratch);
   if (m->precision == NH_PREC_MIXED)
     return launch_cnn_mfma(st, m, true, s, item_b, item_hop, img0, img_end, ipi, Y);
-  if (m->conv_w_mfma && cnn_mfma_supported(m))
+  if (m->conv_w_mfma && m->LY == 16 && cnn_mfma_supported(m))
     return launch_cnn_mfma(st, m, false, s, item_b, item_hop, img0, img_end, ipi, Y);
   switch (m->channels) {
     case 8: return launch_c<8>(st, m, s, item_b, item_hop, img0, img_end, ipi, Y);

@@ -1,4 +1,4 @@
-// Backflow feature network on MFMA (gfx950) for lattices with LY == 16 sites per row and
+// Backflow feature network on MFMA (gfx950) for lattices with LY == 16 or 32 sites per row and
 // C == 32 channels: FP32 (v_mfma_f32_16x16x4_f32) and split-FP16 (v_mfma_f32_16x16x32_f16,
 // FP32 accumulation) variants.  Split FP16 carries every operand as hi + lo FP16 parts and
 // accumulates lo*hi + hi*lo + hi*hi: 22 significant bits per operand, features within ~2e-7 of
@@ -7,16 +7,19 @@
 // kLpScale = 2^6, so that their lo parts stay FP16 normals; the accumulator starts at the scaled
 // bias and the layer output is multiplied by 2^-6, both exact.
 //
-// A lattice row of 16 sites is the M = 16 dimension of one MFMA tile: lane l of an operand
-// fragment holds site y = l % 16, so the y-shift of a 3x3 tap is a DPP rotation inside each
-// row of 16 lanes (periodic for free) and the x-shift is the fragment of the neighbouring
-// lattice row.  One block (LX / 4 waves) evaluates one image: wave w produces lattice rows
-// 4w .. 4w+3 for all 32 output channels (2 N-halves of 16), holding the residual stream h in
-// the MFMA result layout (lane l, register r -> site y = 4 (l / 16) + r, channel 16 half + l % 16).
-// Each layer: gelu(h) -> LDS as [x][y][c] -> operand fragments of rows 4w-1 .. 4w+4 -> 9 taps
-// x 4 rows x 2 halves MFMA chains -> h += result.  The embedding (2 input channels) runs as an
-// ordinary layer with zero-padded input channels.  Weights arrive pre-packed in fragment order
-// (neuralhubbard_amd/mixed.py), so every lane loads 16 contiguous bytes per fragment.
+// A segment of 16 sites of a lattice row (SEG = LY / 16 segments per row) is the M = 16
+// dimension of one MFMA tile: lane l of an operand fragment holds site y = 16 seg + l % 16, so
+// the y-shift of a 3x3 tap is a DPP rotation inside each row of 16 lanes, periodic for free when
+// SEG = 1; with SEG = 2 the lane at the segment edge takes the rotated value of the row's other
+// segment instead, which is its periodic neighbour.  The x-shift is the fragment of the
+// neighbouring lattice row.  One block (LX SEG / 4 waves) evaluates one image: wave w produces
+// 4 segment-rows (lattice rows RW w .. RW w + RW - 1, RW = 4 / SEG, all their segments) for all 32
+// output channels (2 N-halves of 16), holding the residual stream h in the MFMA result layout
+// (lane l, register r -> site y = 16 seg + 4 (l / 16) + r, channel 16 half + l % 16).  Each layer:
+// gelu(h) -> LDS as [x][y][c] -> operand fragments of rows RW w - 1 .. RW w + RW -> 9 taps x 4
+// segment-rows x 2 halves MFMA chains -> h += result.  The embedding (2 input channels) runs as
+// an ordinary layer with zero-padded input channels.  Weights arrive pre-packed in fragment
+// order (neuralhubbard_amd/mixed.py), so every lane loads 16 contiguous bytes per fragment.
 
 #include "nh_common.h"
 #include "nh_internal.h"
@@ -29,7 +32,6 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 constexpr float kLpScale = 64.f;  // weight pre-scale of the split-FP16 packing (a power of two)
 
 constexpr int kMfC = 32;   // channels
-constexpr int kMfLY = 16;  // sites per lattice row
 
 
 // DPP rotation within rows of 16 lanes: row_ror:1 (0x121) gives lane y the value of lane y - 1,
@@ -39,6 +41,19 @@ __device__ inline int rot(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL
 template <int DYI>
 __device__ inline int rot_dy(int v) { return DYI == 0 ? rot<0x121>(v) : rot<0x12F>(v); }
 
+// y-shift of a 16-lane segment of a row with SEG segments: the rotation inside the segment, and
+// at the segment edge (lane 0 for y - 1, lane 15 for y + 1) the rotated value of the other segment.
+template <int SEG, int DYI>
+__device__ inline int shift_y(int mine, int other, int lane) {
+  const int r = rot_dy<DYI>(mine);
+  if constexpr (SEG == 1) {
+    return r;
+  } else {
+    const int o = rot_dy<DYI>(other);
+    return (lane & 15) == (DYI == 0 ? 0 : 15) ? o : r;
+  }
+}
+
 // gelu in the sigmoid form x * sigmoid(2 u), u = sqrt(2/pi) (x + 0.044715 x^3): identical to
 // 0.5 x (1 + tanh u); the fast form uses the hardware exp2 / reciprocal (split-FP16 path).
 __device__ inline float gelu_fast(float x) {
@@ -46,16 +61,18 @@ __device__ inline float gelu_fast(float x) {
   return x * __builtin_amdgcn_rcpf(1.0f + exp2f(u2));
 }
 
-template <bool LP>
-__global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const void* __restrict__ Wp,
-                                                  const float* __restrict__ Bias, const int8_t* __restrict__ s,
-                                                  const int32_t* __restrict__ item_b,
-                                                  const int32_t* __restrict__ item_hop, int img0, int img_end,
-                                                  int ipi, float* __restrict__ Y) {
-  // activation tile [LX][16][32]: split fp16 (hi tile, then lo tile; channel order natural) or
+template <bool LP, int SEG>
+__global__ void __launch_bounds__(SEG == 1 ? 256 : 1024) k_cnn_mfma(int LX, int n_layers, const void* __restrict__ Wp,
+                                                        const float* __restrict__ Bias, const int8_t* __restrict__ s,
+                                                        const int32_t* __restrict__ item_b,
+                                                        const int32_t* __restrict__ item_hop, int img0, int img_end,
+                                                        int ipi, float* __restrict__ Y) {
+  // activation tile [LX][LY][32]: split fp16 (hi tile, then lo tile; channel order natural) or
   // fp32 (channel c stored at (c % 4) * 8 + c / 4 so that a lane's 8 K-values of the 16x16x4
   // fragments are contiguous)
   extern __shared__ float lds_act[];
+  constexpr int RW = 4 / SEG;       // lattice rows per wave
+  constexpr int NF = (RW + 2) * SEG;  // operand fragments: rows x0 - 1 .. x0 + RW, every segment
   const int img = img0 + blockIdx.x;
   if (img >= img_end) return;
   const int item = img / ipi;
@@ -63,9 +80,9 @@ __global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const vo
   const int32_t hop = item_hop ? item_hop[item] : kHopNone;
   if (hop == kHopInvalid) return;
   const int b = item_b ? item_b[item] : item;
-  const int LY = kMfLY, N = LX * LY, two_n = 2 * N;
+  const int LY = 16 * SEG, N = LX * LY, two_n = 2 * N;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int x0 = 4 * wave;
+  const int x0 = RW * wave;
 
   // layer-0 input: the 2-channel +-1 occupancy of the image, channels 2..31 zero
   {
@@ -95,52 +112,122 @@ __global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const vo
   }
   __syncthreads();
 
-  f32x4 h[4][2];  // residual stream: [row][half]
+  // segment-row v = r * SEG + seg of this wave: lattice row x0 + r, sites 16 seg .. 16 seg + 15
+  f32x4 h[4][2];  // residual stream: [segment-row][half]
   for (int l = 0; l <= n_layers; ++l) {
     f32x4 acc[4][2];
     const float* bl = Bias + l * kMfC;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int v = 0; v < 4; ++v)
 #pragma unroll
       for (int hf = 0; hf < 2; ++hf) {
         const float bv = bl[hf * 16 + (lane & 15)] * (LP ? kLpScale : 1.f);
-        acc[r][hf] = f32x4{bv, bv, bv, bv};
+        acc[v][hf] = f32x4{bv, bv, bv, bv};
       }
-    if (LP) {
-      // operand fragments (hi, lo) of rows x0-1 .. x0+4, 8 fp16 per lane each
-      f16x8 fh[6], fl[6];
+    if constexpr (LP && SEG == 2) {
+      // 32-site rows under 1024 threads (128 VGPRs): per lattice row of the wave (unrolled, so
+      // the accumulators stay in registers) the taps run as loops, one row's two segments
+      // loaded per x-tap and each segment shifted just before its MFMAs; h, acc, one fragment
+      // pair and the tap's weights are all that is live
+      const f16x8* wh = reinterpret_cast<const f16x8*>(Wp) + (size_t)l * 9 * 2 * 64;
+      const f16x8* wlo = wh + (size_t)(n_layers + 1) * 9 * 2 * 64;
 #pragma unroll
-      for (int u = 0; u < 6; ++u) {
+      for (int r = 0; r < RW; ++r) {
+#pragma unroll 1
+        for (int dxi = 0; dxi < 3; ++dxi) {
+          int xr = x0 - 1 + r + dxi;
+          xr += xr < 0 ? LX : 0;
+          xr -= xr >= LX ? LX : 0;
+          f16x8 fh[2], fl[2];
+#pragma unroll
+          for (int sg = 0; sg < 2; ++sg) {
+            const _Float16* g = reinterpret_cast<const _Float16*>(lds_act) +
+                                ((xr * LY + 16 * sg + (lane & 15)) * kMfC + 8 * (lane >> 4));
+            fh[sg] = *reinterpret_cast<const f16x8*>(g);
+            fl[sg] = *reinterpret_cast<const f16x8*>(g + N * kMfC);
+          }
+#pragma unroll 1
+          for (int dyi = 0; dyi < 3; ++dyi) {
+            const int tap = dxi * 3 + dyi;
+            const f16x8 w0 = wh[(tap * 2 + 0) * 64 + lane], w1 = wh[(tap * 2 + 1) * 64 + lane];
+            const f16x8 v0 = wlo[(tap * 2 + 0) * 64 + lane], v1 = wlo[(tap * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int sg = 0; sg < 2; ++sg) {
+              f16x8 sh = fh[sg], sl = fl[sg];
+              if (dyi != 1) {  // wave-uniform
+                const int4 v = *reinterpret_cast<int4*>(&fh[sg]), vo = *reinterpret_cast<int4*>(&fh[1 - sg]);
+                const int4 w = *reinterpret_cast<int4*>(&fl[sg]), wo = *reinterpret_cast<int4*>(&fl[1 - sg]);
+                int4 a, c;
+                if (dyi == 0) {
+                  a = make_int4(shift_y<2, 0>(v.x, vo.x, lane), shift_y<2, 0>(v.y, vo.y, lane),
+                                shift_y<2, 0>(v.z, vo.z, lane), shift_y<2, 0>(v.w, vo.w, lane));
+                  c = make_int4(shift_y<2, 0>(w.x, wo.x, lane), shift_y<2, 0>(w.y, wo.y, lane),
+                                shift_y<2, 0>(w.z, wo.z, lane), shift_y<2, 0>(w.w, wo.w, lane));
+                } else {
+                  a = make_int4(shift_y<2, 2>(v.x, vo.x, lane), shift_y<2, 2>(v.y, vo.y, lane),
+                                shift_y<2, 2>(v.z, vo.z, lane), shift_y<2, 2>(v.w, vo.w, lane));
+                  c = make_int4(shift_y<2, 2>(w.x, wo.x, lane), shift_y<2, 2>(w.y, wo.y, lane),
+                                shift_y<2, 2>(w.z, wo.z, lane), shift_y<2, 2>(w.w, wo.w, lane));
+                }
+                sh = *reinterpret_cast<f16x8*>(&a);
+                sl = *reinterpret_cast<f16x8*>(&c);
+              }
+              const int v = r * 2 + sg;
+              acc[v][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl, w0, acc[v][0], 0, 0, 0);
+              acc[v][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl, w1, acc[v][1], 0, 0, 0);
+              acc[v][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh, v0, acc[v][0], 0, 0, 0);
+              acc[v][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh, v1, acc[v][1], 0, 0, 0);
+              acc[v][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh, w0, acc[v][0], 0, 0, 0);
+              acc[v][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh, w1, acc[v][1], 0, 0, 0);
+            }
+          }
+        }
+      }
+    } else if constexpr (LP) {
+      // operand fragments (hi, lo) of rows x0-1 .. x0+RW, every segment, 8 fp16 per lane each
+      f16x8 fh[NF], fl[NF];
+#pragma unroll
+      for (int u = 0; u < RW + 2; ++u) {
         int xr = x0 - 1 + u;
         xr += xr < 0 ? LX : 0;
         xr -= xr >= LX ? LX : 0;
-        const _Float16* g = reinterpret_cast<const _Float16*>(lds_act) + ((xr * LY + (lane & 15)) * kMfC + 8 * (lane >> 4));
-        fh[u] = *reinterpret_cast<const f16x8*>(g);
-        fl[u] = *reinterpret_cast<const f16x8*>(g + N * kMfC);
+#pragma unroll
+        for (int sg = 0; sg < SEG; ++sg) {
+          const _Float16* g = reinterpret_cast<const _Float16*>(lds_act) +
+                              ((xr * LY + 16 * sg + (lane & 15)) * kMfC + 8 * (lane >> 4));
+          fh[u * SEG + sg] = *reinterpret_cast<const f16x8*>(g);
+          fl[u * SEG + sg] = *reinterpret_cast<const f16x8*>(g + N * kMfC);
+        }
       }
       // weights: hi fragments of all layers, then lo fragments
       const f16x8* wh = reinterpret_cast<const f16x8*>(Wp) + (size_t)l * 9 * 2 * 64;
       const f16x8* wlo = wh + (size_t)(n_layers + 1) * 9 * 2 * 64;
 #pragma unroll
       for (int dyi = 0; dyi < 3; ++dyi) {
-        f16x8 sh[6], sl[6];
+        f16x8 sh[NF], sl[NF];
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
+        for (int u = 0; u < NF; ++u) {
           if (dyi == 1) {
             sh[u] = fh[u];
             sl[u] = fl[u];
           } else {  // dy = -1: lane y <- y-1; dy = +1: y <- y+1
-            int4 v = *reinterpret_cast<int4*>(&fh[u]);
-            int4 w = *reinterpret_cast<int4*>(&fl[u]);
+            const int ou = (SEG == 1) ? u : (u ^ 1);  // the other segment of the same row
+            const int4 v = *reinterpret_cast<int4*>(&fh[u]), vo = *reinterpret_cast<int4*>(&fh[ou]);
+            const int4 w = *reinterpret_cast<int4*>(&fl[u]), wo = *reinterpret_cast<int4*>(&fl[ou]);
+            int4 a, c;
             if (dyi == 0) {
-              v.x = rot_dy<0>(v.x); v.y = rot_dy<0>(v.y); v.z = rot_dy<0>(v.z); v.w = rot_dy<0>(v.w);
-              w.x = rot_dy<0>(w.x); w.y = rot_dy<0>(w.y); w.z = rot_dy<0>(w.z); w.w = rot_dy<0>(w.w);
+              a = make_int4(shift_y<SEG, 0>(v.x, vo.x, lane), shift_y<SEG, 0>(v.y, vo.y, lane),
+                            shift_y<SEG, 0>(v.z, vo.z, lane), shift_y<SEG, 0>(v.w, vo.w, lane));
+              c = make_int4(shift_y<SEG, 0>(w.x, wo.x, lane), shift_y<SEG, 0>(w.y, wo.y, lane),
+                            shift_y<SEG, 0>(w.z, wo.z, lane), shift_y<SEG, 0>(w.w, wo.w, lane));
             } else {
-              v.x = rot_dy<2>(v.x); v.y = rot_dy<2>(v.y); v.z = rot_dy<2>(v.z); v.w = rot_dy<2>(v.w);
-              w.x = rot_dy<2>(w.x); w.y = rot_dy<2>(w.y); w.z = rot_dy<2>(w.z); w.w = rot_dy<2>(w.w);
+              a = make_int4(shift_y<SEG, 2>(v.x, vo.x, lane), shift_y<SEG, 2>(v.y, vo.y, lane),
+                            shift_y<SEG, 2>(v.z, vo.z, lane), shift_y<SEG, 2>(v.w, vo.w, lane));
+              c = make_int4(shift_y<SEG, 2>(w.x, wo.x, lane), shift_y<SEG, 2>(w.y, wo.y, lane),
+                            shift_y<SEG, 2>(w.z, wo.z, lane), shift_y<SEG, 2>(w.w, wo.w, lane));
             }
-            sh[u] = *reinterpret_cast<f16x8*>(&v);
-            sl[u] = *reinterpret_cast<f16x8*>(&w);
+            sh[u] = *reinterpret_cast<f16x8*>(&a);
+            sl[u] = *reinterpret_cast<f16x8*>(&c);
           }
         }
 #pragma unroll
@@ -149,35 +236,47 @@ __global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const vo
           const f16x8 w0 = wh[(tap * 2 + 0) * 64 + lane], w1 = wh[(tap * 2 + 1) * 64 + lane];
           const f16x8 v0 = wlo[(tap * 2 + 0) * 64 + lane], v1 = wlo[(tap * 2 + 1) * 64 + lane];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl[r + dxi], w0, acc[r][0], 0, 0, 0);
-            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl[r + dxi], w1, acc[r][1], 0, 0, 0);
-            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[r + dxi], v0, acc[r][0], 0, 0, 0);
-            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[r + dxi], v1, acc[r][1], 0, 0, 0);
-            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[r + dxi], w0, acc[r][0], 0, 0, 0);
-            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[r + dxi], w1, acc[r][1], 0, 0, 0);
+          for (int v = 0; v < 4; ++v) {
+            const int f = (v / SEG + dxi) * SEG + v % SEG;  // row r + dxi, same segment
+            acc[v][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl[f], w0, acc[v][0], 0, 0, 0);
+            acc[v][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sl[f], w1, acc[v][1], 0, 0, 0);
+            acc[v][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[f], v0, acc[v][0], 0, 0, 0);
+            acc[v][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[f], v1, acc[v][1], 0, 0, 0);
+            acc[v][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[f], w0, acc[v][0], 0, 0, 0);
+            acc[v][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(sh[f], w1, acc[v][1], 0, 0, 0);
           }
         }
       }
-    } else {
+    } else if constexpr (SEG == 1) {  // the FP32 form runs on 16-site rows only
       const float* wl = reinterpret_cast<const float*>(Wp) + (size_t)l * 9 * 2 * 8 * 64;
 #pragma unroll 1
       for (int dyi = 0; dyi < 3; ++dyi) {
-        float fs[6][8];
+        float fs[NF][8];
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
+        for (int u = 0; u < RW + 2; ++u) {
           int xr = x0 - 1 + u;
           xr += xr < 0 ? LX : 0;
           xr -= xr >= LX ? LX : 0;
-          const float4* g = reinterpret_cast<const float4*>(lds_act + (xr * LY + (lane & 15)) * kMfC + 8 * (lane >> 4));
-          const float4 p0 = g[0], p1 = g[1];
-          float v[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+          float vr[SEG][8];
 #pragma unroll
-          for (int kk = 0; kk < 8; ++kk) {
-            if (dyi == 1) fs[u][kk] = v[kk];
-            else if (dyi == 0) fs[u][kk] = __builtin_bit_cast(float, rot_dy<0>(__builtin_bit_cast(int, v[kk])));
-            else fs[u][kk] = __builtin_bit_cast(float, rot_dy<2>(__builtin_bit_cast(int, v[kk])));
+          for (int sg = 0; sg < SEG; ++sg) {
+            const float4* g =
+                reinterpret_cast<const float4*>(lds_act + (xr * LY + 16 * sg + (lane & 15)) * kMfC + 8 * (lane >> 4));
+            const float4 p0 = g[0], p1 = g[1];
+            vr[sg][0] = p0.x; vr[sg][1] = p0.y; vr[sg][2] = p0.z; vr[sg][3] = p0.w;
+            vr[sg][4] = p1.x; vr[sg][5] = p1.y; vr[sg][6] = p1.z; vr[sg][7] = p1.w;
           }
+#pragma unroll
+          for (int sg = 0; sg < SEG; ++sg)
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+              const int mine = __builtin_bit_cast(int, vr[sg][kk]);
+              const int oth = __builtin_bit_cast(int, vr[SEG - 1 - sg][kk]);
+              float& o = fs[u * SEG + sg][kk];
+              if (dyi == 1) o = vr[sg][kk];
+              else if (dyi == 0) o = __builtin_bit_cast(float, shift_y<SEG, 0>(mine, oth, lane));
+              else o = __builtin_bit_cast(float, shift_y<SEG, 2>(mine, oth, lane));
+            }
         }
 #pragma unroll
         for (int dxi = 0; dxi < 3; ++dxi) {
@@ -188,41 +287,43 @@ __global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const vo
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) wv[kk] = wl[((tap * 2 + hf) * 8 + kk) * 64 + lane];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int v = 0; v < 4; ++v) {
+              const int f = (v / SEG + dxi) * SEG + v % SEG;
 #pragma unroll
               for (int kk = 0; kk < 8; ++kk)
-                acc[r][hf] = __builtin_amdgcn_mfma_f32_16x16x4f32(fs[r + dxi][kk], wv[kk], acc[r][hf], 0, 0, 0);
+                acc[v][hf] = __builtin_amdgcn_mfma_f32_16x16x4f32(fs[f][kk], wv[kk], acc[v][hf], 0, 0, 0);
+            }
           }
         }
       }
     }
     // residual stream
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int v = 0; v < 4; ++v)
 #pragma unroll
       for (int hf = 0; hf < 2; ++hf) {
-        if (LP) acc[r][hf] *= 1.f / kLpScale;
-        h[r][hf] = (l == 0) ? acc[r][hf] : h[r][hf] + acc[r][hf];
+        if (LP) acc[v][hf] *= 1.f / kLpScale;
+        h[v][hf] = (l == 0) ? acc[v][hf] : h[v][hf] + acc[v][hf];
       }
     if (l == n_layers) break;
     __syncthreads();  // every wave is done reading the previous activations
     // next layer input gelu(h) -> LDS
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int v = 0; v < 4; ++v)
 #pragma unroll
       for (int hf = 0; hf < 2; ++hf) {
         const int co = hf * 16 + (lane & 15);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int y = 4 * (lane >> 4) + q;
-          const int site = (x0 + r) * LY + y;
+          const int y = 16 * (v % SEG) + 4 * (lane >> 4) + q;
+          const int site = (x0 + v / SEG) * LY + y;
           if (LP) {
-            const float gv = gelu_fast(h[r][hf][q]);
+            const float gv = gelu_fast(h[v][hf][q]);
             const _Float16 hi = (_Float16)gv;
             reinterpret_cast<_Float16*>(lds_act)[site * kMfC + co] = hi;
             reinterpret_cast<_Float16*>(lds_act)[N * kMfC + site * kMfC + co] = (_Float16)(gv - (float)hi);
           } else {
-            lds_act[site * kMfC + (co & 3) * 8 + (co >> 2)] = gelu_tanh(h[r][hf][q]);
+            lds_act[site * kMfC + (co & 3) * 8 + (co >> 2)] = gelu_tanh(h[v][hf][q]);
           }
         }
       }
@@ -231,31 +332,44 @@ __global__ void __launch_bounds__(256) k_cnn_mfma(int LX, int n_layers, const vo
   // features: y[spin = half][site][a = lane % 16] = h
   float* yo = Y + (size_t)(img - img0) * (two_n * 16);
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int v = 0; v < 4; ++v)
 #pragma unroll
     for (int hf = 0; hf < 2; ++hf)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int site = (x0 + r) * LY + 4 * (lane >> 4) + q;
-        yo[((size_t)hf * N + site) * 16 + (lane & 15)] = h[r][hf][q];
+        const int site = (x0 + v / SEG) * LY + 16 * (v % SEG) + 4 * (lane >> 4) + q;
+        yo[((size_t)hf * N + site) * 16 + (lane & 15)] = h[v][hf][q];
       }
 }
 
-bool cnn_mfma_supported(const nh_model* m) { return m->LY == kMfLY && m->channels == kMfC && m->LX % 4 == 0 && m->LX <= 64; }
+// LY = 16: LX % 4 == 0, up to 16 rows (256 threads), FP32 and split FP16; LY = 32 (split FP16
+// only): LX % 2 == 0, up to 32 rows (1024 threads, 128 KB of activations in LDS).
+bool cnn_mfma_supported(const nh_model* m) {
+  if (m->channels != kMfC) return false;
+  if (m->LY == 16) return m->LX % 4 == 0 && m->LX >= 4 && m->LX <= 16;
+  if (m->LY == 32) return m->LX % 2 == 0 && m->LX >= 4 && m->LX <= 32;
+  return false;
+}
 
 static int launch_mf(hipStream_t st, const nh_model* m, bool lp, const void* wp, const int8_t* s,
                      const int32_t* item_b, const int32_t* item_hop, int img0, int img_end, int ipi, float* Y) {
   const int nimg = img_end - img0;
   if (nimg <= 0) return NH_OK;
-  const int threads = 64 * (m->LX / 4);
-  const size_t lds = (size_t)m->LX * kMfLY * kMfC * 4;  // fp32, or fp16 hi + lo
-  if (threads > 256) return NH_ERR_SHAPE;
-  if (lp)
-    hipLaunchKernelGGL(k_cnn_mfma<true>, dim3(nimg), dim3(threads), lds, st, m->LX, m->n_layers, wp, m->conv_b, s,
-                       item_b, item_hop, img0, img_end, ipi, Y);
-  else
-    hipLaunchKernelGGL(k_cnn_mfma<false>, dim3(nimg), dim3(threads), lds, st, m->LX, m->n_layers, wp, m->conv_b, s,
-                       item_b, item_hop, img0, img_end, ipi, Y);
+  const int seg = m->LY / 16, rw = 4 / seg;
+  const int threads = 64 * (m->LX / rw);
+  const size_t lds = (size_t)m->LX * m->LY * kMfC * 4;  // fp32, or fp16 hi + lo
+  if (threads > (seg == 1 ? 256 : 1024) || lds > 160 * 1024) return NH_ERR_SHAPE;
+#define NH_MF_LAUNCH(LPV, SEGV)                                                                              \
+  hipLaunchKernelGGL((k_cnn_mfma<LPV, SEGV>), dim3(nimg), dim3(threads), lds, st, m->LX, m->n_layers, wp, m->conv_b, \
+                     s, item_b, item_hop, img0, img_end, ipi, Y)
+  if (seg == 1) {
+    if (lp) NH_MF_LAUNCH(true, 1);
+    else NH_MF_LAUNCH(false, 1);
+  } else {
+    if (!lp) return NH_ERR_UNSUPPORTED;  // FP32 features at LY = 32 use the VALU network
+    NH_MF_LAUNCH(true, 2);
+  }
+#undef NH_MF_LAUNCH
   NH_LAUNCH_CHECK();
   return NH_OK;
 }

@@ -1,5 +1,6 @@
-"""Fragment-ordered conv weights for the MFMA feature networks (LY == 16; C == 32 for the conv
-stack, C == 64 for the residual CNN, `pack_resconv_bf16`).
+"""Fragment-ordered conv weights for the MFMA feature networks (C == 32 for the conv stack at
+LY == 16 or 32, C == 64 for the residual CNN, `pack_resconv_bf16`).  The layout does not depend
+on the lattice.
 
 FP32 (v_mfma_f32_16x16x4_f32): for layer l, tap, output half hf, K-step kk and lane:
     W[tap][ci = 4 kk + lane // 16][co = 16 hf + lane % 16]

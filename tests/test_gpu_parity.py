@@ -276,26 +276,29 @@ def test_log_psi_pf16_matches_oracle():
         assert sg[b] == rs and abs(la[b] - rl) < log_tol(1e-3, c), (b, la[b], rl)
 
 
-def test_log_psi_pf32_matches_oracle():
-    """32x32 Pfaffian backflow (896 x 896 Pfaffians, triangles in the global workspace, VALU
-    feature network) on 8 configurations against the FP64 C restatement: same sign and
-    |d log psi| < 1e-2 (the measured error is printed; FP32 elimination over 448 pivot steps)."""
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_log_psi_pf32_matches_oracle(precision):
+    """32x32 Pfaffian backflow (896 x 896 Pfaffians, triangles in the global workspace; the
+    feature network on VALU in FP32, on split-FP16 MFMA with 32-site rows in the mixed form) on 8
+    configurations against the FP64 C restatement: same sign and |d log psi| < 1e-2 (the
+    measured error is printed; FP32 elimination over 448 pivot steps)."""
     from neuralhubbard_amd.backflow import Variational
     from oracle.cpu_port import CpuPath
 
     wk = WORKLOADS["pf32"]
     p = make_params(wk)
-    dev, _ = _model(wk)
+    dev, _ = _model(wk, precision)
     s = make_configurations(wk, 8)
     sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
     rs, rl = CpuPath(wk, p, bits=64).psi(s)
     for b in range(len(s)):
-        print(f"pf32 b={b} gpu={la[b]:.6f} fp64={rl[b]:.6f} |d|={abs(la[b] - rl[b]):.2e}", flush=True)
+        print(f"pf32 {precision} b={b} gpu={la[b]:.6f} fp64={rl[b]:.6f} |d|={abs(la[b] - rl[b]):.2e}", flush=True)
     assert np.array_equal(sg, rs)
     assert np.abs(la - rl).max() < 1e-2
 
 
-def test_local_energy_pf32_matches_materialized_hops():
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_local_energy_pf32_matches_materialized_hops(precision):
     """E_loc at 32x32 Pfaffian (n = 896) through nh_local_energy (hop items, chunked images,
     triangles in the workspace) equals U D(s) + sum over the oracle's hop list (Jordan-Wigner signs
     included) of h psi(s') / psi(s) with every psi(s') from nh_log_psi on the materialised
@@ -305,7 +308,7 @@ def test_local_energy_pf32_matches_materialized_hops():
     from oracle.cpu_port import CpuPath
 
     wk = WORKLOADS["pf32"]
-    dev, _ = _model(wk)
+    dev, _ = _model(wk, precision)
     st = Variational(dev)
     s = make_configurations(wk, 1)
     ts = torch.tensor(s, device="cuda")
@@ -318,7 +321,8 @@ def test_local_energy_pf32_matches_materialized_hops():
     sg2, la2 = st(torch.tensor(sp, device="cuda")).numpy()
     sg0, la0 = float(psi.sign[0]), float(psi.logabs[0])
     want = O.diag_energy(s[0], wk.U) + sum(h * sg2[k] * sg0 * math.exp(la2[k] - la0) for k, (_, _, h) in enumerate(hl))
-    print(f"pf32 eloc gpu={e / wk.N:.8f} materialised={want / wk.N:.8f} per site ({len(hl)} hops)", flush=True)
+    print(f"pf32 {precision} eloc gpu={e / wk.N:.8f} materialised={want / wk.N:.8f} per site ({len(hl)} hops)",
+          flush=True)
     assert abs(e - want) < 1e-5 * abs(want) + 1e-4
     pick = np.linspace(0, len(hl) - 1, 16).astype(int)
     r_s, r_l = CpuPath(wk, make_params(wk), bits=64).psi(sp[pick])
@@ -482,6 +486,22 @@ def test_features_mfma_fp32_pf16():
     for b in range(3):
         r = ref.features(s[b])
         np.testing.assert_allclose(y[b], r, rtol=1e-4, atol=1e-4 * np.abs(r).max())
+
+
+@pytest.mark.parametrize("LX", [4, 10, 32])
+def test_features_mfma_f16_32_site_rows(LX):
+    """Split-FP16 MFMA features on lattices with 32-site rows (two 16-site segments per row, the
+    y-taps across the segment edge from the other segment; 1024 threads at 32x32) against the
+    FP64 oracle: within 1e-6 of their largest magnitude, as on 16-site rows."""
+    wk = small_workload(LX=LX, LY=32, n_up=LX * 14 // 2, n_dn=LX * 14 // 2, channels=32, n_layers=6)
+    dev, ref = _model(wk, "mixed")
+    s = make_configurations(wk, 2)
+    y = dev.features(torch.tensor(s, device="cuda")).cpu().numpy()
+    for b in range(2):
+        r = ref.features(s[b])
+        err = np.abs(y[b] - r).max() / np.abs(r).max()
+        print(f"split-fp16 features {LX}x32: max |dy| / max |y| = {err:.2e}", flush=True)
+        assert err < 1e-6
 
 
 def test_features_mfma_f16_pf16():
