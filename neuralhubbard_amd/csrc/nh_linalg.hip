@@ -1,8 +1,11 @@
 // Pairing blocks, skew Pfaffian and determinant on gfx950 (FP32 arithmetic, FP64 log-sum).
 //
-// One workgroup owns one matrix.  A Pfaffian matrix lives as its packed strict lower triangle
-// (row i at i(i-1)/2): in LDS up to n = 256 (n = 224 takes 98 KiB), in a global workspace above
-// (pf_blocked2 works on either).  A determinant matrix lives in LDS as a dense row-major square.
+// One workgroup owns one matrix.  The Pfaffian image kernel for 64 < n <= 256 (k_image_pf_cm)
+// builds each image as a full skew square, column-major, in a global workspace and eliminates it
+// with pf_lazy (lazy pivoting, compacted Schur complements; nh_pf_lazy.h).  The standalone
+// nh_slogpf and the n > 256 image kernel keep the packed strict lower triangle (row i at
+// i(i-1)/2), in LDS up to n = 256 and in a global workspace above (pf_blocked2 works on
+// either).  A determinant matrix lives in LDS as a dense row-major square.
 //   n <= 64:  pf_eliminate / lu_eliminate, one wave, rank-2 (rank-1) updates per step.
 //   n > 64:   pf_blocked2 (blocked Parlett-Reid: left-looking panels, packed pivot keys, two
 //             barriers per step, FP32 MFMA trailing updates) and lu_blocked (blocked LU with
@@ -306,24 +309,9 @@ __device__ void lu_eliminate(float* M, int n, int ld, float* lcol, float* urow, 
 
 #include "nh_lu_blocked.h"
 
-// Ascending occupied modes of the item configuration, mapped by group element g, in the
-// order of the item's own occupied list.  Wave 0 only; returns the count in lane 0's view.
-__device__ inline int occupied_modes(const int8_t* sb, int32_t hop, int two_n, int N, int LX, int LY,
-                                     int g, int* modes, int cap) {
-  const int lane = threadIdx.x & (kWave - 1);
-  int cnt = 0;
-  for (int m0 = 0; m0 < two_n; m0 += kWave) {
-    const int m = m0 + lane;
-    const bool occ = (m < two_n) && item_occ(sb, hop, m, N) > 0;
-    const uint64_t mask = __ballot(occ);
-    const int pos = cnt + lanes_below(mask);
-    if (occ && pos < cap) modes[pos] = group_map(m, g, N, LX, LY);
-    cnt += __popcll(mask);
-  }
-  return cnt;
-}
+#include "nh_pf_lazy.h"
 
-__device__ inline float parity_sign(long long k) { return (k & 1) ? -1.f : 1.f; }
+#include "nh_image_pf.h"
 
 // ------------------------------------------------------------------ Pfaffian image kernel
 
@@ -497,6 +485,28 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
   }
 }
 
+// Column-major full-square form (64 < n <= 256, n_up % 4 == 0): pf_lazy with 16-step panels,
+// the square in the global workspace (200 KB per image at n = 224).
+constexpr int kPanelCm = 16;
+
+template <int D>
+__global__ void __launch_bounds__(256, 4) k_image_pf_cm(int N, int LX, int LY, int n, int n_up,
+                                                     const int8_t* __restrict__ s,
+                                                     const int32_t* __restrict__ item_b,
+                                                     const int32_t* __restrict__ item_hop, int img0,
+                                                     int img_end, int ipi, const float* __restrict__ Y,
+                                                     const float* __restrict__ wt,
+                                                     const float* __restrict__ F0,
+                                                     float* __restrict__ img_sign,
+                                                     double* __restrict__ img_logabs,
+                                                     float* __restrict__ gmat, size_t gstride) {
+  extern __shared__ float lds[];
+  const int img = img0 + blockIdx.x;
+  if (img >= img_end) return;
+  image_pf_cm<D, 256, kPanelCm, 8>(img, N, LX, LY, n, n_up, s, item_b, item_hop, img0, ipi, Y, wt, F0,
+                                   img_sign, img_logabs, gmat, gstride, lds);
+}
+
 // ------------------------------------------------------------------ BCS image kernel
 
 // GM (bcs_global_matrix): the image's n x (n + 1) matrix sits in the global workspace gmat at
@@ -646,9 +656,30 @@ int64_t image_chunk(const nh_model* m) {
     return !bcs_global_matrix(m) || m->n_up <= 256 ? 32768 : 1024;  // 0.8 MB per image at n = 448
   const int n = m->n_up + m->n_dn;
   if (n <= kLdsTriMaxN) return 32768;
-  // each image of a chunk also keeps its triangle (<= 256 KB / 1.6 MB); at n = 224, 32768 images
-  // per launch (4.3 GB of workspace) give 5 % more E_loc throughput than 8192
+  // each image of a chunk also keeps its matrix (the square and its first compacted complement,
+  // 347 KB at n = 224; the packed triangle, 1.6 MB, at n = 896): 32768 images per launch at
+  // n <= 256 (11.4 GB of workspace at n = 224)
   return n <= 256 ? 32768 : 1024;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+static size_t pf_cm_lds(int n, int N, int D) {
+  switch (D) {
+    case 4: return pf_cm_lds_bytes<4, 256, kPanelCm>(n, N);
+    case 8: return pf_cm_lds_bytes<8, 256, kPanelCm>(n, N);
+    case 16: return pf_cm_lds_bytes<16, 256, kPanelCm>(n, N);
+    case 32: return pf_cm_lds_bytes<32, 256, kPanelCm>(n, N);
+    case 64: return pf_cm_lds_bytes<64, 256, kPanelCm>(n, N);
+    default: return kMaxLds + 1;
+  }
+}
+
+// Pfaffian images that run k_image_pf_cm (the others above n = 64: k_image_pf on the packed triangle).
+static bool pf_column_form(const nh_model* m) {
+  const int n = m->n_up + m->n_dn;
+  return m->kind == NH_KIND_PFAFFIAN && n > kLdsTriMaxN && n <= 256 && n % 2 == 0 && m->n_up % 4 == 0 &&
+         pf_cm_lds(n, m->LX * m->LY, feature_dim(m)) <= kMaxLds;
 }
 
 size_t image_global_floats(const nh_model* m) {
@@ -658,10 +689,9 @@ size_t image_global_floats(const nh_model* m) {
   }
   const int n = m->n_up + m->n_dn;
   if (n <= kLdsTriMaxN) return 0;
+  if (pf_column_form(m)) return pfl_matrix_floats<kPanelCm>(n);
   return (size_t)((((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63);
 }
-
-constexpr size_t kMaxLds = 160 * 1024;
 
 template <int D>
 static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, const int32_t* item_b,
@@ -680,6 +710,10 @@ static int launch_image_d(hipStream_t st, const nh_model* m, const int8_t* s, co
                          nullptr, (size_t)0);
     } else if (!gtri) {
       return NH_ERR_ARG;
+    } else if (pf_column_form(m)) {
+      hipLaunchKernelGGL((k_image_pf_cm<D>), dim3(nimg), dim3(256), pf_cm_lds(n, N, D), st, N, m->LX, m->LY, n,
+                         m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,
+                         img_logabs, gtri, image_global_floats(m));
     } else if (n <= 256) {
       hipLaunchKernelGGL((k_image_pf<D, 256, true>), dim3(nimg), dim3(256), lds, st, N, m->LX, m->LY, n,
                          m->n_up, s, item_b, item_hop, img0, img_end, ipi, Y, m->wt, m->base, img_sign,

@@ -17,12 +17,12 @@ __device__ inline double wave_sum_f64(double x) {
   return x;
 }
 
-// Blocked Parlett-Reid with a short per-step chain (64 < n <= 256, n even), on the packed
+// Blocked Parlett-Reid with a short per-step chain (64 < n <= NT, n <= 2^KB, n even), on the packed
 // lower triangle `tri` (followed by kWave dummy floats).  Panels of up to P elimination steps:
 //   (1) rows i > k: effective column c_i = A[i][k] + sum_{q<s} (U_q[i] W_q[k] - W_q[i] U_q[k])
 //       (left-looking over the panel's earlier steps), one 32-bit key per row, |c_i| with its
-//       low 8 mantissa bits replaced by 255 - i, and a wave max of the keys;
-//   (2) every thread reduces the wave keys to the pivot row b (largest |c| to 2^-15 relative,
+//       low KB mantissa bits replaced by 2^KB - 1 - i, and a wave max of the keys;
+//   (2) every thread reduces the wave keys to the pivot row b (largest |c| to 2^(KB-23) relative,
 //       smallest index among those), reads the pivot, and rows i >= k + 2 form the new panel
 //       column U_s[i] = A'[i][k] / A'[a][k], W_s[i] = A'[i][a] of A' = P A P (P the
 //       transposition (a b)), and move row/column a's triangle entries to b;

@@ -24,6 +24,9 @@ constexpr int kWave = 64;
 struct alignas(16) float4 {
   float x, y, z, w;
 };
+struct alignas(16) int4 {
+  int x, y, z, w;
+};
 struct alignas(16) uint4 {
   uint32_t x, y, z, w;
 };
@@ -31,6 +34,7 @@ struct alignas(8) float2 {
   float x, y;
 };
 inline float2 make_float2(float a, float b) { return {a, b}; }
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 inline uint32_t __float_as_uint(float f) {
   uint32_t u;
   std::memcpy(&u, &f, 4);
@@ -43,6 +47,7 @@ inline float __uint_as_float(uint32_t u) {
 }
 using std::isfinite;
 using std::max;
+using std::min;
 typedef float nh_f32x4 __attribute__((vector_size(16)));
 
 struct EmuIdx {

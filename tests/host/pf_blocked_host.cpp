@@ -11,7 +11,10 @@
 //                                        (n (n - 1) / 2 float32, row i at i (i - 1) / 2)
 //   pf_blocked_host lu NT n mat.bin   ->  "sign logabs" of det of the row-major n x n float32
 //                                        matrix, held as the BCS kernels hold it (leading
-//                                        dimension n + 1, 16-column panels, 9-bit row keys)
+//                                        dimension n + 1, 16-column panels, 9-bit row keys: the
+//                                        workspace form of the BCS image kernel)
+//   pf_blocked_host lu8 NT n mat.bin  ->  the same with 8-bit row keys (the LDS form of the BCS
+//                                        image kernel and nh_slogdet, n <= 256)
 
 #include "emu_wave.h"
 
@@ -21,7 +24,7 @@ inline int toff(int i, int j) { return ((i * (i - 1)) >> 1) + j; }
 
 #include "nh_lu_blocked.h"
 
-template <int NT>
+template <int NT, int KB>
 static void run_lu(int n, const std::vector<float>& in, float& sg, double& la) {
   const int ld = n + 1;
   float* M = static_cast<float*>(std::malloc(sizeof(float) * ((size_t)n * ld + kWave)));
@@ -40,7 +43,7 @@ static void run_lu(int n, const std::vector<float>& in, float& sg, double& la) {
       threadIdx.x = t;
       float s;
       double l;
-      lu_blocked<NT, 16, 9>(M, n, ld, scr, s, l);
+      lu_blocked<NT, 16, KB>(M, n, ld, scr, s, l);
       if (t == 0) {
         sg = s;
         la = l;
@@ -83,22 +86,25 @@ static void run(int n, const std::vector<float>& in, float& sg, double& la) {
 
 static int main_lu(int argc, char** argv) {
   const int NT = std::atoi(argv[2]), n = std::atoi(argv[3]);
-  if (n < 1 || n > NT || n > 512) return 2;
+  const bool k8 = std::strcmp(argv[1], "lu8") == 0;
+  if (n < 1 || n > NT || n > (k8 ? 256 : 512)) return 2;
   std::vector<float> in((size_t)n * n);
   FILE* f = std::fopen(argv[4], "rb");
   if (!f || std::fread(in.data(), sizeof(float), in.size(), f) != in.size()) return 2;
   std::fclose(f);
   float sg = 0.f;
   double la = 0.0;
-  if (NT == 256) run_lu<256>(n, in, sg, la);
-  else if (NT == 512) run_lu<512>(n, in, sg, la);
+  if (NT == 256 && k8) run_lu<256, 8>(n, in, sg, la);
+  else if (NT == 512 && k8) run_lu<512, 8>(n, in, sg, la);
+  else if (NT == 256) run_lu<256, 9>(n, in, sg, la);
+  else if (NT == 512) run_lu<512, 9>(n, in, sg, la);
   else return 2;
   std::printf("%.1f %.17g\n", sg, la);
   return 0;
 }
 
 int main(int argc, char** argv) {
-  if (argc == 5 && std::strcmp(argv[1], "lu") == 0) return main_lu(argc, argv);
+  if (argc == 5 && (std::strcmp(argv[1], "lu") == 0 || std::strcmp(argv[1], "lu8") == 0)) return main_lu(argc, argv);
   if (argc != 5) {
     std::fprintf(stderr, "usage: %s NT P n tri.bin\n", argv[0]);
     return 2;
@@ -112,14 +118,15 @@ int main(int argc, char** argv) {
   float sg = 0.f;
   double la = 0.0;
   // 256 threads: the image kernels' global-triangle form (10-bit row keys); 1024 threads: the
-  // standalone nh_slogpf form (8-bit row keys, n <= 256)
-  if (NT == 1024 && n > 256) return 2;
+  // standalone nh_slogpf form (8-bit row keys, n <= 256) and, above n = 256, the global-triangle
+  // form of the 32x32 image kernel and of nh_slogpf_ws (10-bit row keys)
   if (NT == 256 && P == 4) run<256, 4, 10>(n, in, sg, la);
   else if (NT == 256 && P == 8) run<256, 8, 10>(n, in, sg, la);
   else if (NT == 256 && P == 12) run<256, 12, 10>(n, in, sg, la);
   else if (NT == 256 && P == 16) run<256, 16, 10>(n, in, sg, la);
-  else if (NT == 1024 && P == 4) run<1024, 4, 8>(n, in, sg, la);
-  else if (NT == 1024 && P == 8) run<1024, 8, 8>(n, in, sg, la);
+  else if (NT == 1024 && P == 4 && n <= 256) run<1024, 4, 8>(n, in, sg, la);
+  else if (NT == 1024 && P == 8 && n <= 256) run<1024, 8, 8>(n, in, sg, la);
+  else if (NT == 1024 && P == 8) run<1024, 8, 10>(n, in, sg, la);
   else return 2;
   std::printf("%.1f %.17g\n", sg, la);
   return 0;

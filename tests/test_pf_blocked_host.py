@@ -6,8 +6,9 @@ stays inside the matrix and the scratch the kernels allot, no value read from sc
 never written reaches the result (the scratch starts as NaN), readfirstlane only sees
 wave-uniform values, and (sign, log|Pf|) matches the FP64 oracle, (sign, log|det|) LAPACK.
 Pfaffian: the panel widths in use (8 steps, 4 in the LDS form above n = 240) and wider ones (12,
-16), at the sizes of the image kernels (n = 224 for 16x16 at 112 + 112 electrons) and the
-standalone nh_slogpf form (1024 threads).  LU: the BCS kernels' 16-column panels at n = 100 to
+16), at the sizes of the image kernels (n = 224 for 16x16 at 112 + 112 electrons; n = 896 for
+32x32 at 448 + 448 with 1024 threads and 10-bit row keys, the form of the 32x32 image kernel and
+of nh_slogpf_ws) and the standalone nh_slogpf form (1024 threads, 8-bit row keys).  LU: the BCS kernels' 16-column panels at n = 100 to
 448 (the 32x32 BCS state)."""
 
 import os
@@ -21,7 +22,7 @@ from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-CASES = [(256, 8, 224), (256, 4, 130), (256, 12, 224), (256, 16, 256), (256, 12, 66), (1024, 8, 224),
+CASES = [(256, 8, 224), (256, 4, 130), (256, 12, 224), (256, 16, 256), (256, 12, 66), (1024, 8, 224), (1024, 8, 896),
          (1024, 4, 256)]
 
 
@@ -80,23 +81,26 @@ def test_pf_blocked_host_pins_a_known_pfaffian(harness, tmp_path):
     assert abs(la - np.sum(np.log(np.abs(d.astype(np.float32).astype(np.float64))))) < 1e-4
 
 
-def _run_lu(exe, tmp_path, NT, A):
+def _run_lu(exe, tmp_path, NT, A, keys="lu"):
     n = A.shape[0]
-    path = str(tmp_path / f"mat_{NT}_{n}.bin")
+    path = str(tmp_path / f"mat_{keys}_{NT}_{n}.bin")
     np.ascontiguousarray(A, dtype=np.float32).tofile(path)
-    r = subprocess.run([exe, "lu", str(NT), str(n), path], capture_output=True, text=True, timeout=600)
+    r = subprocess.run([exe, keys, str(NT), str(n), path], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     sg, la = r.stdout.split()
     return float(sg), float(la)
 
 
-@pytest.mark.parametrize("NT,n", [(256, 100), (256, 256), (512, 130), (512, 448)])
-def test_lu_blocked_host_matches_lapack(harness, tmp_path, NT, n):
-    """lu_blocked (the BCS determinant: 16-column panels, rows kept in place, 9-bit row keys) at
-    the LDS sizes and at n = 448 of the 32x32 BCS state, against LAPACK's slogdet in FP64."""
-    rng = np.random.default_rng(n + NT)
+@pytest.mark.parametrize("NT,n,keys", [(256, 100, "lu"), (256, 256, "lu"), (512, 130, "lu"), (512, 448, "lu"),
+                                        (512, 112, "lu8"), (512, 192, "lu8"), (256, 256, "lu8")])
+def test_lu_blocked_host_matches_lapack(harness, tmp_path, NT, n, keys):
+    """lu_blocked (the BCS determinant: 16-column panels, rows kept in place) with 9-bit row keys
+    (the workspace form, up to n = 448 of the 32x32 BCS state) and with the 8-bit keys of the
+    LDS form and nh_slogdet (n = 112 of the 16x16 BCS workload, 192, 256), against LAPACK's
+    slogdet in FP64."""
+    rng = np.random.default_rng(n + NT + (7 if keys == "lu8" else 0))
     A = rng.standard_normal((n, n)).astype(np.float32)
-    sg, la = _run_lu(harness, tmp_path, NT, A)
+    sg, la = _run_lu(harness, tmp_path, NT, A, keys)
     rs, rl = np.linalg.slogdet(A.astype(np.float64))
     assert np.isfinite(la), "a NaN scratch value reached the result"
     assert sg == rs
