@@ -57,7 +57,7 @@ if ROOT not in sys.path:
 METRIC = "samples/s (log-psi+E_loc, 16x16 Pfaffian): mixed+surrogate vs plain FP32"
 PEAK = {"fp32": 157.3e12, "fp16": 2.5e15, "bf16": 2.5e15}
 CHAINS_FILE = os.path.join(ROOT, "tests", "golden", "pf16_chains_8192.npz")
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_image_pf_pf16_hbm_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_image_pf_cm_pf16_hbm_traffic.md")
 TRAFFIC_SUMMARY_RESCONV = os.path.join(ROOT, "profiles", "k_resconv_bf16_tpm02_L16_hbm_traffic.md")
 
 
@@ -350,7 +350,7 @@ def main():
     elif dom == "features" and resconv and arith == "bf16":
         traffic = read_traffic(images / max(1, launches), TRAFFIC_SUMMARY_RESCONV)
     cnn_kernel = ("k_resconv_bf16" if arith == "bf16" else "k_resconv_fs") if resconv else "k_cnn_mfma"
-    roof = {"bound": "mfma", "kernel": "k_image_pf" if dom == "pairing+pfaffian" else cnn_kernel,
+    roof = {"bound": "mfma", "kernel": "k_image_pf_cm" if dom == "pairing+pfaffian" else cnn_kernel,
             "achieved": achieved / 1e12, "peak": PEAK[arith] / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK[arith], "traffic": traffic, "launches": launches,
             "avg_launch_ms": ms / max(1, launches), "flops_per_launch": flops[dom] / max(1, launches),
