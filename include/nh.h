@@ -210,7 +210,8 @@ int nh_hops(void* stream, int32_t LX, int32_t LY, double t, double tprime, int32
 /* ---- surrogate log-amplitude network (no reference counterpart: the learned stage one of
  * delayed-acceptance Metropolis).  z1 = conv3x3(x0) + b1, z2 = conv3x3(gelu(z1)) + b2,
  * z3 = conv3x3(gelu(z2)) + b3 (1 channel), f(s) = sum over sites of z3; x0 = the 2-channel +-1
- * occupancy, periodic 3x3 convolutions, channels 16 or 32, N <= 256.  params (device, FP32):
+ * occupancy, periodic 3x3 convolutions, channels 16 or 32 for N <= 256, 16 for N <= 1024.
+ * params (device, FP32):
  * w1 [9][2][C], b1 [C], w2 [9][C][C], b2 [C], w3 [9][C], b3 [1]; nh_sur_nparams gives the count. */
 typedef struct nh_surrogate {
   int32_t LX, LY, channels, reserved;

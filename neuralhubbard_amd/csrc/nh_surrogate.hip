@@ -7,8 +7,9 @@
 //   f(s) = sum_sites z3             the surrogate log|psi|
 // Parameters, flattened: w1 [9][2][Cs], b1 [Cs], w2 [9][Cs][Cs], b2 [Cs], w3 [9][Cs], b3 [1].
 //
-// One block evaluates IPB whole configurations (one slot = one site per thread); activations
-// of neighbouring sites are exchanged through LDS.  Training computes per-configuration
+// One block evaluates IPB whole configurations (one slot = one site per thread; 256 slots up to
+// 256 sites, one configuration of up to 1024 sites per 1024-thread block above); activations of
+// neighbouring sites are exchanged through LDS.  Training computes per-configuration
 // gradients in one block (forward, then the backward pass through the same LDS tiles) and a
 // second kernel sums them over the batch in a fixed order, so results are deterministic.
 
@@ -60,8 +61,8 @@ __device__ inline void neighbour_slots(int li, int site, int N, int LX, int LY, 
 }
 
 // Forward of configurations item -> s[item_b[item]] with item_hop applied (no group map).
-template <int CS>
-__global__ void __launch_bounds__(256) k_sur_forward(int N, int LX, int LY, const float* __restrict__ P,
+template <int CS, int NT>
+__global__ void __launch_bounds__(NT) k_sur_forward(int N, int LX, int LY, const float* __restrict__ P,
                                                      const int8_t* __restrict__ s, int two_n,
                                                      const int32_t* __restrict__ item_b,
                                                      const int32_t* __restrict__ item_hop, int n_items,
@@ -127,8 +128,8 @@ __global__ void __launch_bounds__(256) k_sur_forward(int N, int LX, int LY, cons
 
 // Per-configuration gradient of 0.5 * wgt * (f(s) - target)^2 into G[cfg][nparams]; loss into
 // L[cfg].  One configuration per block (slots = N).
-template <int CS>
-__global__ void __launch_bounds__(256) k_sur_grad(int N, int LX, int LY, const float* __restrict__ P,
+template <int CS, int NT>
+__global__ void __launch_bounds__(NT) k_sur_grad(int N, int LX, int LY, const float* __restrict__ P,
                                                   const int8_t* __restrict__ s, int two_n,
                                                   const float* __restrict__ target, float wgt, int batch,
                                                   float* __restrict__ G, float* __restrict__ L) {
@@ -305,12 +306,15 @@ __global__ void k_adam(int np, float* __restrict__ w, const float* __restrict__ 
   w[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-static int sur_slots(int N) { return N <= 256 ? (256 / N) * N : -1; }
+static int sur_slots(int N) { return N <= 256 ? (256 / N) * N : N; }
 
+// Up to 256 sites: 16 or 32 channels; up to 1024 sites (one configuration per 1024-thread block):
+// 16 channels, whose gradient activations (2 x 16 x 1024 floats) fit LDS.
 int check_sur(const nh_surrogate* q) {
   if (!q || !q->params) return NH_ERR_ARG;
-  if (q->LX < 3 || q->LY < 3 || q->LX * q->LY > 256) return NH_ERR_SHAPE;
+  if (q->LX < 3 || q->LY < 3 || q->LX * q->LY > 1024) return NH_ERR_SHAPE;
   if (q->channels != 16 && q->channels != 32) return NH_ERR_SHAPE;
+  if (q->LX * q->LY > 256 && q->channels != 16) return NH_ERR_SHAPE;
   return NH_OK;
 }
 
@@ -320,12 +324,15 @@ int launch_sur_forward(hipStream_t st, const nh_surrogate* q, const int8_t* s, c
   const int N = q->LX * q->LY, S = sur_slots(N), IPB = S / N;
   const int blocks = (n_items + IPB - 1) / IPB;
   const size_t lds = sizeof(float) * (size_t)(q->channels * S + 3 * S);
-  if (q->channels == 32)
-    hipLaunchKernelGGL(k_sur_forward<32>, dim3(blocks), dim3(S), lds, st, N, q->LX, q->LY, q->params, s, 2 * N,
-                       item_b, item_hop, n_items, out);
+  if (S > 256)
+    hipLaunchKernelGGL((k_sur_forward<16, 1024>), dim3(blocks), dim3(S), lds, st, N, q->LX, q->LY, q->params, s,
+                       2 * N, item_b, item_hop, n_items, out);
+  else if (q->channels == 32)
+    hipLaunchKernelGGL((k_sur_forward<32, 256>), dim3(blocks), dim3(S), lds, st, N, q->LX, q->LY, q->params, s,
+                       2 * N, item_b, item_hop, n_items, out);
   else
-    hipLaunchKernelGGL(k_sur_forward<16>, dim3(blocks), dim3(S), lds, st, N, q->LX, q->LY, q->params, s, 2 * N,
-                       item_b, item_hop, n_items, out);
+    hipLaunchKernelGGL((k_sur_forward<16, 256>), dim3(blocks), dim3(S), lds, st, N, q->LX, q->LY, q->params, s,
+                       2 * N, item_b, item_hop, n_items, out);
   NH_LAUNCH_CHECK();
   return NH_OK;
 }
@@ -362,12 +369,15 @@ extern "C" int nh_sur_grad(void* stream, const nh_surrogate* q, int32_t batch, c
   float* L = G + (size_t)batch * np;
   const size_t lds = sizeof(float) * (size_t)(2 * q->channels * N + 3 * N + q->channels);
   const int threads = N < 64 ? 64 : ((N + 63) / 64) * 64;
-  if (q->channels == 32)
-    hipLaunchKernelGGL(k_sur_grad<32>, dim3(batch), dim3(threads), lds, st, N, q->LX, q->LY, q->params, s, 2 * N,
-                       target, weight, batch, G, L);
+  if (threads > 256)
+    hipLaunchKernelGGL((k_sur_grad<16, 1024>), dim3(batch), dim3(threads), lds, st, N, q->LX, q->LY, q->params, s,
+                       2 * N, target, weight, batch, G, L);
+  else if (q->channels == 32)
+    hipLaunchKernelGGL((k_sur_grad<32, 256>), dim3(batch), dim3(threads), lds, st, N, q->LX, q->LY, q->params, s,
+                       2 * N, target, weight, batch, G, L);
   else
-    hipLaunchKernelGGL(k_sur_grad<16>, dim3(batch), dim3(threads), lds, st, N, q->LX, q->LY, q->params, s, 2 * N,
-                       target, weight, batch, G, L);
+    hipLaunchKernelGGL((k_sur_grad<16, 256>), dim3(batch), dim3(threads), lds, st, N, q->LX, q->LY, q->params, s,
+                       2 * N, target, weight, batch, G, L);
   NH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sur_reduce, dim3((np + 255) / 256), dim3(256), 0, st, np, batch, G, L, grad, loss);
   NH_LAUNCH_CHECK();

@@ -38,7 +38,7 @@ def _sur(LX, LY, C, seed=3):
     return Surrogate(LX, LY, C, seed=seed)
 
 
-@pytest.mark.parametrize("LX,LY,C", [(4, 4, 16), (16, 16, 32), (3, 5, 16)])
+@pytest.mark.parametrize("LX,LY,C", [(4, 4, 16), (16, 16, 32), (3, 5, 16), (20, 20, 16), (32, 32, 16)])
 def test_surrogate_forward_matches_oracle(LX, LY, C):
     sur = _sur(LX, LY, C)
     # larger last layer so the output is not tiny
@@ -53,14 +53,17 @@ def test_surrogate_forward_matches_oracle(LX, LY, C):
         assert abs(f[b] - ref) < 1e-4 * max(1.0, abs(ref)), (b, f[b], ref)
 
 
-def test_surrogate_gradient_matches_finite_differences():
-    LX, LY, C = 4, 4, 16
+@pytest.mark.parametrize("LX,LY,n_up,n_dn", [(4, 4, 5, 4), (32, 32, 448, 448)])
+def test_surrogate_gradient_matches_finite_differences(LX, LY, n_up, n_dn):
+    """The gradient kernel (one 64-thread block per configuration at 4x4, the 1024-thread form of
+    lattices above 256 sites at 32x32) against central differences of the FP64 oracle's loss."""
+    C = 16
     sur = _sur(LX, LY, C, seed=5)
     p = sur.params.cpu().numpy().astype(np.float64)
     p[-(9 * C + 1):] *= 10.0
     sur.params.copy_(torch.tensor(p.astype(np.float32)))
     p = sur.params.cpu().numpy().astype(np.float64)
-    geom = L.Geometry(LX, LY, 5, 4)
+    geom = L.Geometry(LX, LY, n_up, n_dn)
     S = L.random_configurations(geom, 3, 12)
     T = np.array([0.3, -0.7, 1.1])
     g, loss = sur.grad(torch.tensor(S, device="cuda"), torch.tensor(T, dtype=torch.float32, device="cuda"), 0.5)
