@@ -33,9 +33,8 @@ static T* exact_alloc(size_t count, T fill) {
   return p;
 }
 
-template <int D, int P>
+template <int D, int P, int NT, int KB>
 static int run(const int* h, FILE* f) {
-  constexpr int NT = 256;
   const int N = h[0], LX = h[1], LY = h[2], n = h[3], n_up = h[4], g = h[6], hop = h[7], has_f0 = h[8];
   const int two_n = 2 * N;
   if (n <= 64 || n > NT || n % 2 || n_up % 4 || N != LX * LY || g < 0 || g > 7) return 2;
@@ -65,8 +64,8 @@ static int run(const int* h, FILE* f) {
   for (int t = 0; t < NT; ++t)
     th.emplace_back([&, t] {
       threadIdx.x = t;
-      image_pf_cm<D, NT, P, 8>(g, N, LX, LY, n, n_up, s, nullptr, item_hop, g, kGroup, Y, wt, F0, img_sign,
-                               img_logabs, gmat, gstride, lds);
+      image_pf_cm<D, NT, P, KB>(g, N, LX, LY, n, n_up, s, nullptr, item_hop, g, kGroup, Y, wt, F0, img_sign,
+                                img_logabs, gmat, gstride, lds);
     });
   for (auto& x : th) x.join();
   std::printf("%.1f %.17g\n", img_sign[g], img_logabs[g]);
@@ -91,11 +90,12 @@ int main(int argc, char** argv) {
   int h[10];
   if (!f || std::fread(h, sizeof(int), 10, f) != 10) return 2;
   const int D = h[5], P = h[9];
+  // 256 threads, 8-bit row keys, as the library launches the kernel (n <= 256)
   int rc = 2;
-  if (D == 16 && P == 16) rc = run<16, 16>(h, f);
-  else if (D == 16 && P == 8) rc = run<16, 8>(h, f);
-  else if (D == 32 && P == 16) rc = run<32, 16>(h, f);
-  else if (D == 8 && P == 16) rc = run<8, 16>(h, f);
+  if (D == 16 && P == 16) rc = run<16, 16, 256, 8>(h, f);
+  else if (D == 16 && P == 8) rc = run<16, 8, 256, 8>(h, f);
+  else if (D == 32 && P == 16) rc = run<32, 16, 256, 8>(h, f);
+  else if (D == 8 && P == 16) rc = run<8, 16, 256, 8>(h, f);
   std::fclose(f);
   return rc;
 }

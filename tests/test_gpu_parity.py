@@ -276,6 +276,29 @@ def test_log_psi_pf16_matches_oracle():
         assert sg[b] == rs and abs(la[b] - rl) < log_tol(1e-3, c), (b, la[b], rl)
 
 
+@pytest.mark.parametrize("LX,LY,n_up,n_dn,channels", [
+    (10, 10, 36, 38, 16),  # column form, n = 74: the last 4-column group reaches the padding rows
+    (9, 10, 44, 44, 8),    # column form at d = 4 features
+    (10, 10, 37, 37, 16),  # n_up % 4 != 0: the packed-triangle kernel
+    (12, 12, 60, 62, 32),  # column form, n = 122, d = 16, five 16-step panels
+])
+def test_log_psi_image_forms_match_oracle(LX, LY, n_up, n_dn, channels):
+    """Both Pfaffian image kernels above n = 64 (the column-major lazy-pivot form, used when
+    n_up % 4 == 0, and the packed-triangle form otherwise) against the FP64 oracle's projected
+    amplitude, with t' != 0 and a base matrix F0, on sizes where their edge cases arise."""
+    from neuralhubbard_amd.backflow import Variational
+
+    wk = small_workload(kind="pfaffian", LX=LX, LY=LY, n_up=n_up, n_dn=n_dn, channels=channels,
+                        n_layers=2, base_f0=True, tprime=-0.2)
+    dev, ref = _model(wk)
+    s = make_configurations(wk, 6)
+    sg, la = Variational(dev)(torch.tensor(s, device="cuda")).numpy()
+    for b in range(6):
+        rs, rl = ref.psi(s[b])
+        c = cancellation(ref, s[b])
+        assert sg[b] == rs and abs(la[b] - rl) < log_tol(2e-4, c), (b, la[b], rl, c)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
 def test_log_psi_pf32_matches_oracle(precision):
     """32x32 Pfaffian backflow (896 x 896 Pfaffians, triangles in the global workspace; the

