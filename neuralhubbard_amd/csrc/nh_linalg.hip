@@ -1,10 +1,11 @@
 // Pairing blocks, skew Pfaffian and determinant on gfx950 (FP32 arithmetic, FP64 log-sum).
 //
-// One workgroup owns one matrix.  The Pfaffian image kernel for 64 < n <= 256 (k_image_pf_cm)
+// One workgroup owns one matrix.  The Pfaffian image kernel for 146 <= n <= 256 (k_image_pf_cm)
 // builds each image as a full skew square, column-major, in a global workspace and eliminates it
-// with pf_lazy (lazy pivoting, compacted Schur complements; nh_pf_lazy.h).  The standalone
-// nh_slogpf and the n > 256 image kernel keep the packed strict lower triangle (row i at
-// i(i-1)/2), in LDS up to n = 256 and in a global workspace above (pf_blocked2 works on
+// with pf_lazy (lazy pivoting, compacted Schur complements; nh_pf_lazy.h), and so does the
+// workspace entry nh_slogpf_ws at those sizes (k_slogpf_cm).  nh_slogpf, the other sizes of
+// nh_slogpf_ws and the other Pfaffian image kernels keep the packed strict lower triangle (row i
+// at i(i-1)/2), in LDS up to n = 256 and in a global workspace above (pf_blocked2 works on
 // either).  A determinant matrix lives in LDS as a dense row-major square.
 //   n <= 64:  pf_eliminate / lu_eliminate, one wave, rank-2 (rank-1) updates per step.
 //   n > 64:   pf_blocked2 (blocked Parlett-Reid: left-looking panels, packed pivot keys, two
@@ -485,9 +486,11 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
   }
 }
 
-// Column-major full-square form (64 < n <= 256, n_up % 4 == 0): pf_lazy with 16-step panels,
-// the square in the global workspace (200 KB per image at n = 224).
+// Column-major full-square form (any 64 < n <= 256 with n_up % 4 == 0; dispatched from
+// n = kCmMinN on): pf_lazy with 16-step panels, the square in the global workspace (200 KB per
+// image at n = 224).
 constexpr int kPanelCm = 16;
+constexpr int kCmMinN = 146;  // smallest n of the column forms (image kernel and nh_slogpf_ws)
 
 template <int D>
 __global__ void __launch_bounds__(256, 4) k_image_pf_cm(int N, int LX, int LY, int n, int n_up,
@@ -676,11 +679,14 @@ static size_t pf_cm_lds(int n, int N, int D) {
 }
 
 // Pfaffian images that run k_image_pf_cm; the others above n = 64 run k_image_pf on the packed
-// triangle.  Above n = 256 (the 32x32 lattice, n = 896) a 1024-thread column form measured no
-// faster than the packed triangle (20.3 against 20.1 us per image) for 4x its workspace.
+// triangle.  The column form is the faster one from n = 160 on (per image: 0.190 against 0.195
+// us at n = 160, 0.239 against 0.283 at 192, 0.337 against 0.408 at 224) and the slower one up to
+// n = 144 (0.080 against 0.070 us at n = 80, 0.163 against 0.159 at 144).  Above n = 256 (the
+// 32x32 lattice, n = 896) a 1024-thread column form measured no faster than the packed triangle
+// (20.3 against 20.1 us per image) for 4x its workspace.
 static bool pf_column_form(const nh_model* m) {
   const int n = m->n_up + m->n_dn;
-  return m->kind == NH_KIND_PFAFFIAN && n > kLdsTriMaxN && n <= 256 && n % 2 == 0 && m->n_up % 4 == 0 &&
+  return m->kind == NH_KIND_PFAFFIAN && n >= kCmMinN && n <= 256 && n % 2 == 0 && m->n_up % 4 == 0 &&
          pf_cm_lds(n, m->LX * m->LY, feature_dim(m)) <= kMaxLds;
 }
 
@@ -820,6 +826,66 @@ __global__ void __launch_bounds__(NT) k_slogpf_ws(int n, int b0, const float* __
     sign[b] = sg;
     logabs[b] = la;
   }
+}
+
+// Column form of the workspace entry (146 <= n <= 256; per matrix 0.233 against 0.257 us at
+// n = 162, 0.400 against 0.536 at 224, 0.534 against 0.753 at 256, and the slower form up to
+// n = 130, 0.173 against 0.150 us): matrix b0 + blockIdx.x's strict lower
+// triangle becomes a column-major skew square in its workspace slice (M[j * ld + i] = A[i][j]
+// for i > j, -A[j][i] for i < j, zero on the diagonal and in the padding rows n .. ld - 1;
+// gstride >= pfl_matrix_floats<kPanelCm>(n) floats), which pf_lazy eliminates.  The lower
+// triangle is read once, in 64 x 64 tiles with all 16 loads of a thread in flight together:
+// each tile is stored negated, as it stands, into the upper half (column i of M holds row i of
+// the tile) and transposed through LDS into the lower half.  LDS: the larger of the tile
+// (64 x 65 floats) and pf_lazy's scratch.
+__global__ void __launch_bounds__(256, 4) k_slogpf_cm(int n, int b0, const float* __restrict__ A, float* sign,
+                                                      double* logabs, float* __restrict__ gmat, size_t gstride) {
+  extern __shared__ float lds[];
+  constexpr int NT = 256, TS = kWave + 1, PER = kWave * kWave / NT;
+  const int b = b0 + blockIdx.x;
+  const int ld = pfl_ld(n);
+  float* M = gmat + (size_t)blockIdx.x * gstride;
+  const float* Ab = A + (size_t)b * n * n;
+  const int c = threadIdx.x & (kWave - 1), r0 = threadIdx.x / kWave;
+  for (int j0 = 0; j0 < n; j0 += kWave)
+    for (int i0 = j0; i0 < n; i0 += kWave) {
+      float v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int i = i0 + r0 + k * (NT / kWave), j = j0 + c;
+        v[k] = i < n && j < i ? Ab[(size_t)i * n + j] : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int r = r0 + k * (NT / kWave), i = i0 + r, j = j0 + c;
+        if (i < n && j < i) M[(size_t)i * ld + j] = -v[k];
+        lds[r * TS + c] = v[k];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int jr = r0 + k * (NT / kWave), i = i0 + c, j = j0 + jr;
+        if (i < n && j < i) M[(size_t)j * ld + i] = lds[c * TS + jr];
+      }
+      __syncthreads();
+    }
+  for (int j = threadIdx.x; j < n; j += NT) {
+    M[(size_t)j * ld + j] = 0.f;
+    for (int i = n; i < ld; ++i) M[(size_t)j * ld + i] = 0.f;
+  }
+  __syncthreads();
+  float sg;
+  double la;
+  pf_lazy<NT, kPanelCm, 8>(M, n, lds, sg, la);
+  if (threadIdx.x == 0) {
+    sign[b] = sg;
+    logabs[b] = la;
+  }
+}
+
+static size_t slogpf_cm_lds(int n) {
+  const size_t tile = (size_t)kWave * (kWave + 1), scr = (size_t)pfl_scratch_floats<256, kPanelCm>(pfl_ld(n));
+  return sizeof(float) * (tile > scr ? tile : scr);
 }
 
 template <int NT>
@@ -979,7 +1045,13 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   return NH_OK;
 }
 
-static size_t slogpf_ws_stride(int n) { return (((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63; }
+// Workspace floats per matrix: the column-major square and its first compacted complement for
+// 146 <= n <= 256 (348 KB at n = 224), the packed triangle otherwise.
+static bool slogpf_column_form(int n) { return n >= kCmMinN && n <= 256; }
+static size_t slogpf_ws_stride(int n) {
+  if (slogpf_column_form(n)) return pfl_matrix_floats<kPanelCm>(n);
+  return (((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63;
+}
 
 extern "C" size_t nh_slogpf_workspace_size(int32_t batch, int32_t n) {
   if (batch <= 0 || n <= 0 || n % 2 || n > 1024) return 0;
@@ -1001,10 +1073,13 @@ extern "C" int nh_slogpf_ws(void* stream, int32_t batch, int32_t n, const float*
   if (fit < 1) return NH_ERR_WORKSPACE;
   const int chunk = fit < batch ? (int)fit : batch;
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = sizeof(float) * (size_t)pf2_scratch_floats<kPanel>(n);
+  const size_t lds =
+      slogpf_column_form(n) ? slogpf_cm_lds(n) : sizeof(float) * (size_t)pf2_scratch_floats<kPanel>(n);
   for (int b0 = 0; b0 < batch; b0 += chunk) {
     const int nb = batch - b0 < chunk ? batch - b0 : chunk;
-    if (n <= 256)
+    if (slogpf_column_form(n))
+      hipLaunchKernelGGL(k_slogpf_cm, dim3(nb), dim3(256), lds, st, n, b0, A, sign, logabs, gtri, gs);
+    else if (n <= 256)
       hipLaunchKernelGGL(k_slogpf_ws<256>, dim3(nb), dim3(256), lds, st, n, b0, A, sign, logabs, gtri, gs);
     else
       hipLaunchKernelGGL(k_slogpf_ws<1024>, dim3(nb), dim3(1024), lds, st, n, b0, A, sign, logabs, gtri, gs);

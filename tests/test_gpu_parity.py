@@ -41,13 +41,14 @@ def test_slogpf_matches_oracle():
     for n, B in [(2, 8), (8, 16), (56, 32), (64, 8), (224, 8), (256, 4)]:
         print(f"slogpf n={n} B={B}", flush=True)
         A = np.stack([rand_skew(n, rng, 0.3) for _ in range(B)])
-        sg, la = slogpf(torch.tensor(A, dtype=torch.float32, device="cuda"))
-        sg, la = sg.cpu().numpy(), la.cpu().numpy()
         A32 = A.astype(np.float32).astype(np.float64)
-        for b in range(B):
-            s_ref, l_ref = O.slogpf(A32[b])
-            assert sg[b] == s_ref, (n, b)
-            assert abs(la[b] - l_ref) < 2e-4 * max(1.0, n / 56), (n, b, la[b], l_ref)
+        for ws in (False, True):  # the LDS form and the workspace form
+            sg, la = slogpf(torch.tensor(A, dtype=torch.float32, device="cuda"), workspace=ws)
+            sg, la = sg.cpu().numpy(), la.cpu().numpy()
+            for b in range(B):
+                s_ref, l_ref = O.slogpf(A32[b])
+                assert sg[b] == s_ref, (n, b, ws)
+                assert abs(la[b] - l_ref) < 2e-4 * max(1.0, n / 56), (n, b, ws, la[b], l_ref)
 
 
 def _cond_tol(A64, base):
@@ -66,7 +67,7 @@ def test_slogpf_blocked_sizes(n):
     rng = np.random.default_rng(n)
     B = 12
     A = np.stack([rand_skew(n, rng, 0.3) for _ in range(B)]).astype(np.float32)
-    sg, la = slogpf(torch.tensor(A, device="cuda"))
+    sg, la = slogpf(torch.tensor(A, device="cuda"), workspace=False)
     sg, la = sg.cpu().numpy(), la.cpu().numpy()
     for b in range(B):
         A64 = A[b].astype(np.float64)
@@ -77,9 +78,10 @@ def test_slogpf_blocked_sizes(n):
         assert abs(la[b] - l_ref) < _cond_tol(A64, 2e-4 * n / 56), (n, b, la[b], l_ref)
 
 
-@pytest.mark.parametrize("n", [66, 130, 224, 256, 258, 384, 512, 896, 1024])
+@pytest.mark.parametrize("n", [66, 96, 130, 144, 146, 194, 224, 256, 258, 384, 512, 896, 1024])
 def test_slogpf_workspace_form(n):
-    """nh_slogpf_ws (the triangles in a workspace, n <= 1024; n = 896 is the 32x32 Pfaffian):
+    """nh_slogpf_ws (column-major squares in the workspace for 146 <= n <= 256, packed triangles
+    at the other sizes, n <= 1024; n = 896 is the 32x32 Pfaffian):
     sign against the FP64 oracle, log|Pf| against LAPACK's log|det| / 2 within the
     condition-scaled tolerance of the LDS form, and a workspace of two matrices (the batch in
     chunks) gives bit-identical results to one of the whole batch."""
@@ -277,15 +279,18 @@ def test_log_psi_pf16_matches_oracle():
 
 
 @pytest.mark.parametrize("LX,LY,n_up,n_dn,channels", [
-    (10, 10, 36, 38, 16),  # column form, n = 74: the last 4-column group reaches the padding rows
-    (9, 10, 44, 44, 8),    # column form at d = 4 features
-    (10, 10, 37, 37, 16),  # n_up % 4 != 0: the packed-triangle kernel
-    (12, 12, 60, 62, 32),  # column form, n = 122, d = 16, five 16-step panels
+    (12, 13, 72, 74, 16),  # column form at its smallest n = 146: the last 4-column group reaches the padding rows
+    (12, 13, 76, 76, 8),   # column form at d = 4 features
+    (14, 14, 96, 98, 32),  # column form, n = 194, d = 16, seven 16-step panels
+    (13, 13, 74, 76, 16),  # n_up % 4 != 0: the packed-triangle kernel at n = 150
+    (12, 12, 60, 62, 32),  # n = 122, below the column form's sizes: the packed-triangle kernel
+    (10, 10, 37, 37, 16),  # n = 74, packed triangle
 ])
 def test_log_psi_image_forms_match_oracle(LX, LY, n_up, n_dn, channels):
-    """Both Pfaffian image kernels above n = 64 (the column-major lazy-pivot form, used when
-    n_up % 4 == 0, and the packed-triangle form otherwise) against the FP64 oracle's projected
-    amplitude, with t' != 0 and a base matrix F0, on sizes where their edge cases arise."""
+    """Both Pfaffian image kernels above n = 64 (the column-major lazy-pivot form, used for
+    146 <= n <= 256 when n_up % 4 == 0, and the packed-triangle form otherwise) against the FP64
+    oracle's projected amplitude, with t' != 0 and a base matrix F0, on sizes where their edge
+    cases arise."""
     from neuralhubbard_amd.backflow import Variational
 
     wk = small_workload(kind="pfaffian", LX=LX, LY=LY, n_up=n_up, n_dn=n_dn, channels=channels,
