@@ -178,6 +178,29 @@ size_t nh_log_psi_workspace_size(const nh_model* model, int32_t batch);
 int nh_log_psi(void* stream, const nh_model* model, int32_t batch, const int8_t* s, float* sign,
                double* logabs, void* workspace, size_t workspace_bytes);
 
+/* ---- f2: per-sample Jacobian of log|psi_P|  (replaces the per-sample gradient quantax's AdamSR
+ * takes through the model, code/train_pfaffian_16x16_tpm02.py:100-101, with the pairing block's
+ * custom_vjp backward, code/backflow/pf_kernel.py:354-365)
+ * J[b][k] = d log|psi_P(s_b)| / d theta_k (the bare network's amplitude when model->project is
+ * 0), columns in the order
+ *   the packed conv weights model->conv_w (synthetic network: the first convolution [9][2][C],
+ *     then [9][C][C] per layer; residual CNN: the embedding [4][C], [9][C][C] per convolution,
+ *     the BCS head [2][C][d]) -- [tap][c_in][c_out], tap = (dx + 1) * 3 + (dy + 1);
+ *   the conv biases model->conv_b;
+ *   W of build_wtilde ([3][N][d]: W_ud, V_uu, V_dd) or of build_wt_ud ([N][d]);
+ *   theta (the envelope parameter, lambda = softplus(theta); model->wt must be built from it);
+ *   with model->base: F0 [2N][2N] (every entry independent) or phi0 [N].
+ * nh_log_psi_jacobian_size gives the column count (0 for an unsupported model).  sign / logabs
+ * receive psi_P as nh_log_psi computes it.  FP32 throughout (conv_w, not the mixed form's
+ * low-precision weights); the image matrices' inverses by the Gauss-Jordan kernel of
+ * nh_slogpf_vjp, so an image with a singular matrix gives NaN rows.  batch * 8 <= 65535 (with
+ * the projection); the workspace grows with every layer's activations of every image
+ * (nh_log_psi_jacobian_workspace_size). */
+size_t nh_log_psi_jacobian_size(const nh_model* model);
+size_t nh_log_psi_jacobian_workspace_size(const nh_model* model, int32_t batch);
+int nh_log_psi_jacobian(void* stream, const nh_model* model, double theta, int32_t batch, const int8_t* s,
+                        float* sign, double* logabs, float* J, void* workspace, size_t workspace_bytes);
+
 /* ---- allocation-free projected amplitude (no counterpart in the reference: for callers that
  * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  The same result as nh_log_psi, with the workspace taken from a static

@@ -80,6 +80,9 @@ SIGNATURES = {
     "nh_log_psi_workspace_size": (_S, [_MP, _I]),
     "nh_log_psi": (_I, [_P, _MP, _I, _P, _P, _P, _P, _S]),
     "nh_log_psi_noalloc": (_I, [_P, _MP, _I, _P, _P, _P]),
+    "nh_log_psi_jacobian_size": (_S, [_MP]),
+    "nh_log_psi_jacobian_workspace_size": (_S, [_MP, _I]),
+    "nh_log_psi_jacobian": (_I, [_P, _MP, _D, _I, _P, _P, _P, _P, _P, _S]),
     "nh_local_energy_workspace_size": (_S, [_MP, _D, _I]),
     "nh_local_energy": (_I, [_P, _MP, _D, _D, _D, _I, _P, _P, _P, _P, _I, _P, _S]),
     "nh_hops": (_I, [_P, _I, _I, _D, _D, _I, _I, _P, _I, _P, _P, _P, _P]),

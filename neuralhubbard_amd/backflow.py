@@ -92,17 +92,17 @@ def build_wt_ud(W, theta, LX, LY) -> np.ndarray:
     return (np.asarray(W, np.float64) * pref[:, None]).astype(np.float32)
 
 
-def pack_conv(params: Params):
+def pack_conv(params: Params, dtype=np.float32):
     """Conv weights as [conv][tap = (dx+1)*3 + (dy+1)][c_in][c_out] and biases [conv][C]; a
     residual CNN's occupation embedding goes first as [class][C], a linear head last as
     [k][c][d]."""
-    w = [np.ascontiguousarray(np.transpose(np.asarray(cw, np.float32), (2, 3, 1, 0))).reshape(-1)
+    w = [np.ascontiguousarray(np.transpose(np.asarray(cw, dtype), (2, 3, 1, 0))).reshape(-1)
          for cw in params.conv_w]
     if params.emb is not None:
-        w.insert(0, np.ascontiguousarray(np.asarray(params.emb, np.float32).T).reshape(-1))
+        w.insert(0, np.ascontiguousarray(np.asarray(params.emb, dtype).T).reshape(-1))
     if params.head is not None:  # (2, d, C) -> [k][c][d]
-        w.append(np.ascontiguousarray(np.transpose(np.asarray(params.head, np.float32), (0, 2, 1))).reshape(-1))
-    b = [np.asarray(cb, np.float32).reshape(-1) for cb in params.conv_b]
+        w.append(np.ascontiguousarray(np.transpose(np.asarray(params.head, dtype), (0, 2, 1))).reshape(-1))
+    b = [np.asarray(cb, dtype).reshape(-1) for cb in params.conv_b]
     return np.concatenate(w), np.concatenate(b)
 
 
@@ -406,6 +406,48 @@ class Variational:
                                       _lib.ptr(sign[b0:]), _lib.ptr(logabs[b0:]), _lib.ptr(ws), ws.numel()),
                        "nh_log_psi")
         return LogArray(sign, logabs)
+
+    def jacobian(self, spins, workspace_cap: int = 4 << 30):
+        """(LogArray of psi, J (B, P) float32): J[b, k] = d log|psi(s_b)| / d theta_k, the
+        per-sample Jacobian SR training takes (``AdamSR``, ``train_pfaffian_16x16_tpm02.py:100-101``),
+        columns in the order of `param_vector`.  Batches run in chunks whose workspace fits
+        `workspace_cap` bytes."""
+        torch = _torch()
+        s = torch.as_tensor(spins, dtype=torch.int8, device="cuda").contiguous()
+        if s.ndim == 1:
+            s = s[None]
+        B = s.shape[0]
+        m = self.model.struct(self.symm)
+        lib = _lib.load()
+        P = lib.nh_log_psi_jacobian_size(ctypes.byref(m))
+        if P == 0:
+            raise _lib.NhError("model not supported by nh_log_psi_jacobian")
+        per = max(1, lib.nh_log_psi_jacobian_workspace_size(ctypes.byref(m), 1))
+        nb_max = max(1, min(B, workspace_cap // per, 65535 // (8 if self.symm else 1)))
+        sign = torch.empty(B, dtype=torch.float32, device="cuda")
+        logabs = torch.empty(B, dtype=torch.float64, device="cuda")
+        J = torch.empty((B, P), dtype=torch.float32, device="cuda")
+        theta = float(self.model.params.theta)
+        for b0 in range(0, B, nb_max):
+            nb = min(nb_max, B - b0)
+            ws = self._ws.get(lib.nh_log_psi_jacobian_workspace_size(ctypes.byref(m), nb))
+            _lib.check(lib.nh_log_psi_jacobian(_lib.stream_handle(), ctypes.byref(m), theta, nb, _lib.ptr(s[b0:]),
+                                               _lib.ptr(sign[b0:]), _lib.ptr(logabs[b0:]), _lib.ptr(J[b0:]),
+                                               _lib.ptr(ws), ws.numel()), "nh_log_psi_jacobian")
+        return LogArray(sign, logabs), J
+
+
+def param_vector(params: Params) -> np.ndarray:
+    """The parameters as one float64 vector in the column order of `Variational.jacobian`
+    (``nh_log_psi_jacobian``): packed conv weights, conv biases, W, theta, then F0 or phi0
+    when the state has a base table."""
+    w, b = pack_conv(params, np.float64)
+    parts = [w, b, np.asarray(params.W).ravel(), [params.theta]]
+    if params.F0 is not None:
+        parts.append(np.asarray(params.F0).ravel())
+    elif params.phi0 is not None:
+        parts.append(np.asarray(params.phi0).ravel())
+    return np.concatenate([np.asarray(p, np.float64).ravel() for p in parts])
 
 
 @dataclass

@@ -150,6 +150,21 @@ static int run_items(hipStream_t st, const nh_model* m, const int8_t* s, const i
 
 static int z_of(double tprime) { return tprime != 0.0 ? 8 : 4; }
 
+size_t image_amplitudes_bytes(const nh_model* m, int64_t batch) {
+  Carver c(nullptr);
+  carve_items(c, m, batch);
+  return c.off;
+}
+
+int image_amplitudes(hipStream_t st, const nh_model* m, const int8_t* s, int64_t batch, void* ws, float** img_sign,
+                     double** img_logabs, float* sign, double* logabs) {
+  Carver c(ws);
+  const ItemBuffers b = carve_items(c, m, batch);
+  *img_sign = b.img_sign;
+  *img_logabs = b.img_logabs;
+  return run_items(st, m, s, nullptr, nullptr, batch, b, sign, logabs);
+}
+
 }  // namespace nh
 
 using namespace nh;

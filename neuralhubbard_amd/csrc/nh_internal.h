@@ -41,6 +41,13 @@ size_t image_global_floats(const nh_model* m);
 // images whose features (and global matrices) are resident at once
 int64_t image_chunk(const nh_model* m);
 
+// The projected amplitudes of `batch` configurations with every image's signed log kept: the
+// nh_log_psi pass on a 256-byte aligned buffer of image_amplitudes_bytes(m, batch) bytes; the
+// image values (indexed img = b * ipi + g) point into it.
+size_t image_amplitudes_bytes(const nh_model* m, int64_t batch);
+int image_amplitudes(hipStream_t st, const nh_model* m, const int8_t* s, int64_t batch, void* ws, float** img_sign,
+                     double** img_logabs, float* sign, double* logabs);
+
 // Signed log-sum over the ipi images of each item in [item0, item_end).
 int launch_combine(hipStream_t st, const int32_t* item_hop, int item0, int item_end, int ipi,
                    const float* img_sign, const double* img_logabs, float* sign, double* logabs);

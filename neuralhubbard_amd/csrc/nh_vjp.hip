@@ -228,10 +228,9 @@ __global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __re
   const int b = blockIdx.x, t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
   float* A = G + (size_t)b * n * n;
   const float* Fb = F + (size_t)b * n * n;
-  for (size_t e = t; e < (size_t)n * n; e += kVjpThreads) {
-    const int i = (int)(e / n), j = (int)(e - (size_t)i * n);
-    A[e] = Fb[(size_t)j * n + i];
-  }
+  constexpr int NW = kVjpThreads / kWave;  // rows by wave, columns by lane: no index division
+  for (int i = wave; i < n; i += NW)
+    for (int j = lane; j < n; j += kWave) A[(size_t)i * n + j] = Fb[(size_t)j * n + i];
   if (t == 0) flag[0] = 0;
   __syncthreads();
   for (int k = 0; k < n; ++k) {
@@ -263,18 +262,20 @@ __global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __re
     }
     __syncthreads();
     const float dinv = 1.0f / rowk[k];
-    for (size_t e = t; e < (size_t)n * n; e += kVjpThreads) {
-      const int i = (int)(e / n), j = (int)(e - (size_t)i * n);
-      const float rk = (j == k) ? 1.0f : rowk[j];
-      float v;
-      if (i == k) {
-        v = rk * dinv;
-      } else {
-        const float base = (j == k) ? 0.0f : (i == p ? oldk[j] : A[e]);
-        const float f = colk[i == p ? k : i];  // column k entry of row i after the interchange
-        v = base - f * rk * dinv;
+    for (int i = wave; i < n; i += NW) {
+      float* Ai = A + (size_t)i * n;
+      const float f = colk[i == p ? k : i];  // column k entry of row i after the interchange
+      for (int j = lane; j < n; j += kWave) {
+        const float rk = (j == k) ? 1.0f : rowk[j];
+        float v;
+        if (i == k) {
+          v = rk * dinv;
+        } else {
+          const float base = (j == k) ? 0.0f : (i == p ? oldk[j] : Ai[j]);
+          v = base - f * rk * dinv;
+        }
+        Ai[j] = v;
       }
-      A[e] = v;
     }
     __syncthreads();
   }

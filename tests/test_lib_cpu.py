@@ -201,3 +201,50 @@ def test_gpu_entry_points_refuse_without_gpu():
         pytest.skip("GPU present")
     with pytest.raises(_lib.NhError):
         bf.slogpf(torch.zeros((1, 4, 4)))
+
+
+def _jac_model(wk, p):
+    kind = _lib.NH_KIND_PFAFFIAN if wk.kind == "pfaffian" else _lib.NH_KIND_BCS
+    if wk.network == "conv":
+        net = _lib.NH_NET_CONV
+    else:
+        net = _lib.NH_NET_RESCONV_HEAD if p.head is not None else _lib.NH_NET_RESCONV
+    base = 0 if (p.F0 is None and p.phi0 is None) else 16
+    return _lib.NhModel(kind, wk.LX, wk.LY, wk.n_up, wk.n_dn, wk.channels, wk.n_layers, 0, 1, net, 16, 16, 16, base,
+                        0, 0)
+
+
+@pytest.mark.parametrize("name", ["pf_base", "bcs", "pf_res", "bcs_head", "pf16", "tpm02_L16"])
+def test_jacobian_columns_match_param_vector(name):
+    """nh_log_psi_jacobian_size counts exactly the entries of `param_vector` (the column order the
+    Jacobian documents) for every network and kind."""
+    wk = {
+        "pf_base": small_workload(kind="pfaffian", n_up=3, n_dn=3, channels=8, n_layers=2, base_f0=True),
+        "bcs": small_workload(kind="bcs", n_up=3, n_dn=3, channels=8, n_layers=2),
+        "pf_res": small_workload(kind="pfaffian", n_up=3, n_dn=3, channels=64, n_layers=2, network="resconv"),
+        "bcs_head": small_workload(kind="bcs", n_up=3, n_dn=3, channels=64, n_layers=2, network="resconv",
+                                   features=64),
+    }.get(name) or WORKLOADS[name]
+    p = make_params(wk)
+    lib = _lib.load()
+    m = _jac_model(wk, p)
+    assert lib.nh_log_psi_jacobian_size(ctypes.byref(m)) == bf.param_vector(p).size
+    assert lib.nh_log_psi_jacobian_workspace_size(ctypes.byref(m), 2) > lib.nh_log_psi_jacobian_workspace_size(
+        ctypes.byref(m), 1)
+
+
+def test_jacobian_argument_errors_without_gpu():
+    """Refusals happen before any launch: bad batch, null buffers, too small a workspace, more
+    than 65535 images."""
+    lib = _lib.load()
+    wk = small_workload(kind="pfaffian", n_up=3, n_dn=3, channels=8, n_layers=2)
+    m = _jac_model(wk, make_params(wk))
+    v = ctypes.c_void_p(256)
+    need = lib.nh_log_psi_jacobian_workspace_size(ctypes.byref(m), 4)
+    assert lib.nh_log_psi_jacobian(None, ctypes.byref(m), 0.5, -1, v, v, v, v, v, need) == _lib.NH_ERR_ARG
+    assert lib.nh_log_psi_jacobian(None, ctypes.byref(m), 0.5, 4, v, v, v, None, v, need) == _lib.NH_ERR_ARG
+    assert lib.nh_log_psi_jacobian(None, ctypes.byref(m), float("nan"), 4, v, v, v, v, v, need) == _lib.NH_ERR_ARG
+    assert lib.nh_log_psi_jacobian(None, ctypes.byref(m), 0.5, 4, v, v, v, v, v, need - 1) == _lib.NH_ERR_WORKSPACE
+    assert lib.nh_log_psi_jacobian(None, ctypes.byref(m), 0.5, 8192, v, v, v, v, v, 1 << 40) == _lib.NH_ERR_SHAPE
+    m.channels = 12  # not a network this library builds
+    assert lib.nh_log_psi_jacobian_size(ctypes.byref(m)) == 0
