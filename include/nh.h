@@ -138,7 +138,8 @@ int nh_pairing_occ_vjp(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t 
 /* ---- f2: Pfaffian VJP (the reverse mode of fermix.slogpf, code/backflow/models.py:210)
  * G[b] = cot[b] * d log|Pf F[b]| / dF[b] = cot[b] * (1/2) F[b]^{-T}, every entry of F taken as
  * independent; cot NULL means 1.  F, G [batch][n][n] (device, distinct buffers), n <= 1024.
- * Gauss-Jordan with partial pivoting in FP32 on G itself; a singular F gives NaN. */
+ * Blocked Gauss-Jordan with partial pivoting (16-column panels, rank-16 MFMA updates) in FP32 on
+ * G itself; a singular F gives NaN. */
 int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G);
 
 /* ---- f2: reverse mode of the BCS up x down pairing block

@@ -21,10 +21,9 @@
 //
 // nh_slogpf_vjp is the Pfaffian VJP: d log|Pf F| / dF = (1/2) F^{-T} (d Pf = (1/2) Pf tr(F^{-1} dF)),
 // scaled by the cotangent of log|Pf|.  It inverts the transposed copy of F in place in the output
-// buffer by Gauss-Jordan elimination with partial pivoting, one 256-thread block per matrix:
-// per step a one-wave pivot search, the pivot row, the old row k and column k staged in LDS, then
-// every element written once by its owner thread; the row interchanges are undone at the end as
-// one column permutation.
+// buffer by blocked Gauss-Jordan elimination with partial pivoting (k_inv_gj: 16-column panels,
+// rank-16 MFMA updates), one block per matrix (256 threads up to n = 256, 1024 above); the row
+// interchanges are undone at the end as one column permutation.
 
 #include "nh_common.h"
 
@@ -33,6 +32,7 @@ namespace nh {
 namespace {
 
 constexpr int kVjpThreads = 256;
+typedef float nh_vf32x4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kVjpThreads) k_vjp_dy(int LX, int LY, int d, int n, const float* __restrict__ Y,
                                                         const int32_t* __restrict__ idx, const float* __restrict__ Wt,
@@ -213,74 +213,208 @@ __global__ void __launch_bounds__(kVjpThreads)
   dWt[((size_t)b * N + r) * d + a] = acc;
 }
 
-// In-place inverse of the transposed copy of F (so X = F^{-T}), scaled by factor * cot
-// (factor 1/2 for the Pfaffian, 1 for the determinant).
-__global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __restrict__ F,
-                                                        const float* __restrict__ cot, float factor,
-                                                        float* __restrict__ G) {
-  extern __shared__ float vf_lds[];
-  float* rowk = vf_lds;             // [n] pivot row before scaling
-  float* oldk = rowk + n;           // [n] row k before the interchange
-  float* colk = oldk + n;           // [n] column k before the interchange
-  int* piv = reinterpret_cast<int*>(colk + n);  // [n]
-  int* perm = piv + n;                           // [n]
-  int* flag = perm + n;                          // [1]
+// In-place inverse of the transposed copy of F (so X = F^{-T}), scaled by factor * cot (factor
+// 1/2 for the Pfaffian, 1 for the determinant): blocked Gauss-Jordan with partial pivoting
+// (Quintana-Orti, Quintana-Orti, Sun, van de Geijn, "A note on parallel matrix inversion", SIAM
+// J. Sci. Comput. 22 (2001) 1762, variant GJE with the pivots of each panel's LU).  For the
+// panel K of 16 columns at k0: the pivots of an LU with partial pivoting of its rows k0 ..
+// n - 1 (in LDS); those row interchanges on the whole matrix; then with A11 the diagonal block,
+// C the panel columns and R the panel rows outside it,
+//   C <- -C A11^{-1},  A[i][j] += C[i] R[:, j] (i, j outside K; 16 x 16 FP32 MFMA tiles),
+//   R <- A11^{-1} R,   A11 <- A11^{-1},
+// and at the end the interchanges undone as one column permutation.  The matrix stays in the
+// output buffer; each panel reads and writes it once (an unblocked sweep does so per column).
+// LDS: the panel columns [nr][17], the panel rows [16][nr + 4], A11 [16][17], reduction slots,
+// pivots and the final permutation (nr = n rounded up to 16).
+constexpr int kGjB = 16;
+
+__host__ __device__ inline size_t gj_lds_bytes(int n, int nt) {
+  const size_t nr = (size_t)((n + 15) & ~15);
+  return sizeof(float) * (nr * (kGjB + 1) + kGjB * (nr + 4) + kGjB * (kGjB + 1) + 2 * (nt / kWave)) +
+         sizeof(int) * 2 * (size_t)n;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_inv_gj(int n, const float* __restrict__ F, const float* __restrict__ cot,
+                                               float factor, float* __restrict__ G) {
+  static_assert(NT >= kGjB * kGjB, "one thread per entry of the diagonal block");
+  constexpr int NW = NT / kWave;
+  constexpr int LC = kGjB + 1;
+  extern __shared__ float gj_lds[];
+  const int nr = (n + 15) & ~15, ldr = nr + 4;
+  float* Cp = gj_lds;                              // [nr][LC]
+  float* Rp = Cp + (size_t)nr * LC;                // [16][ldr]
+  float* Ai = Rp + (size_t)kGjB * ldr;             // [16][LC]
+  float* redv = Ai + kGjB * LC;                    // [NW]
+  int* redi = reinterpret_cast<int*>(redv + NW);   // [NW]
+  int* piv = redi + NW;                            // [n]
+  int* perm = piv + n;                             // [n]
   const int b = blockIdx.x, t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
   float* A = G + (size_t)b * n * n;
   const float* Fb = F + (size_t)b * n * n;
-  constexpr int NW = kVjpThreads / kWave;  // rows by wave, columns by lane: no index division
   for (int i = wave; i < n; i += NW)
     for (int j = lane; j < n; j += kWave) A[(size_t)i * n + j] = Fb[(size_t)j * n + i];
-  if (t == 0) flag[0] = 0;
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    if (wave == 0) {
+  bool singular = false;
+  for (int k0 = 0; k0 < n && !singular; k0 += kGjB) {
+    const int kb = min(kGjB, n - k0), m = n - k0;
+    // (a) the panel's pivots: LU with partial pivoting of its rows k0 .. n - 1
+    for (int r = t; r < m; r += NT)
+      for (int c = 0; c < kb; ++c) Cp[r * LC + c] = A[(size_t)(k0 + r) * n + k0 + c];
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
       float best = -1.f;
-      int bi = k;
-      for (int i = k + lane; i < n; i += kWave) {
-        const float v = fabsf(A[(size_t)i * n + k]);
-        if (v > best) { best = v; bi = i; }
+      int bi = m;
+      for (int r = j + t; r < m; r += NT) {
+        const float v = fabsf(Cp[r * LC + j]);
+        if (v > best) {
+          best = v;
+          bi = r;
+        }
       }
 #pragma unroll
       for (int o = kWave / 2; o > 0; o >>= 1) {
         const float ob = __shfl_xor(best, o, kWave);
         const int oi = __shfl_xor(bi, o, kWave);
-        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
       }
       if (lane == 0) {
-        piv[k] = bi;
-        if (!(best > 0.f)) flag[0] = 1;  // singular (or NaN)
+        redv[wave] = best;
+        redi[wave] = bi;
       }
-    }
-    __syncthreads();
-    if (flag[0]) break;
-    const int p = piv[k];
-    for (int j = t; j < n; j += kVjpThreads) {
-      rowk[j] = A[(size_t)p * n + j];
-      oldk[j] = A[(size_t)k * n + j];
-      colk[j] = A[(size_t)j * n + k];
-    }
-    __syncthreads();
-    const float dinv = 1.0f / rowk[k];
-    for (int i = wave; i < n; i += NW) {
-      float* Ai = A + (size_t)i * n;
-      const float f = colk[i == p ? k : i];  // column k entry of row i after the interchange
-      for (int j = lane; j < n; j += kWave) {
-        const float rk = (j == k) ? 1.0f : rowk[j];
-        float v;
-        if (i == k) {
-          v = rk * dinv;
-        } else {
-          const float base = (j == k) ? 0.0f : (i == p ? oldk[j] : Ai[j]);
-          v = base - f * rk * dinv;
+      __syncthreads();
+      best = redv[0];
+      bi = redi[0];
+      for (int w = 1; w < NW; ++w)
+        if (redv[w] > best || (redv[w] == best && redi[w] < bi)) {
+          best = redv[w];
+          bi = redi[w];
         }
-        Ai[j] = v;
+      if (!(best > 0.f) || !isfinite(best)) {  // every thread reads the same slots: uniform
+        singular = true;
+        break;
+      }
+      if (t == 0) piv[k0 + j] = k0 + bi;
+      if (bi != j)
+        for (int c = t; c < kb; c += NT) {
+          const float x = Cp[j * LC + c];
+          Cp[j * LC + c] = Cp[bi * LC + c];
+          Cp[bi * LC + c] = x;
+        }
+      __syncthreads();
+      const float rinv = 1.f / Cp[j * LC + j];
+      for (int r = j + 1 + t; r < m; r += NT) {
+        const float l = Cp[r * LC + j] * rinv;
+        for (int c = j + 1; c < kb; ++c) Cp[r * LC + c] = fmaf(-l, Cp[j * LC + c], Cp[r * LC + c]);
+      }
+      __syncthreads();
+    }
+    if (singular) break;
+    // (b) the interchanges on the whole matrix, one column per thread
+    for (int c = t; c < n; c += NT)
+      for (int jj = 0; jj < kb; ++jj) {
+        const int k = k0 + jj, p = piv[k];
+        if (p != k) {
+          const float x = A[(size_t)k * n + c];
+          A[(size_t)k * n + c] = A[(size_t)p * n + c];
+          A[(size_t)p * n + c] = x;
+        }
+      }
+    __syncthreads();
+    // (c) panel columns and rows after the interchanges (zero padded to 16), A11 padded with the identity
+    for (int r = t; r < nr; r += NT)
+      for (int c = 0; c < kGjB; ++c) Cp[r * LC + c] = (r < n && c < kb) ? A[(size_t)r * n + k0 + c] : 0.f;
+    for (int r = 0; r < kGjB; ++r)
+      for (int c = t; c < ldr; c += NT) Rp[r * ldr + c] = (r < kb && c < n) ? A[(size_t)(k0 + r) * n + c] : 0.f;
+    __syncthreads();
+    if (t < kGjB * kGjB) {
+      const int i = t / kGjB, c = t % kGjB;
+      Ai[i * LC + c] = (i < kb && c < kb) ? Cp[(k0 + i) * LC + c] : (i == c ? 1.f : 0.f);
+    }
+    __syncthreads();
+    // A11^{-1} by Gauss-Jordan without interchanges (its rows are already pivoted)
+    for (int s = 0; s < kGjB; ++s) {
+      float v = 0.f;
+      const int i = t / kGjB, c = t % kGjB;
+      if (t < kGjB * kGjB) {
+        const float d = 1.f / Ai[s * LC + s];
+        const float ais = Ai[i * LC + s], asc = Ai[s * LC + c], aic = Ai[i * LC + c];
+        if (i == s)
+          v = c == s ? d : asc * d;
+        else
+          v = c == s ? -ais * d : fmaf(-ais * d, asc, aic);
+      }
+      __syncthreads();
+      if (t < kGjB * kGjB) Ai[i * LC + c] = v;
+      __syncthreads();
+    }
+    // (d) C <- -C A11^{-1} in place, one row per thread
+    for (int r = t; r < nr; r += NT) {
+      if (r >= k0 && r < k0 + kGjB) continue;
+      float row[kGjB], out[kGjB];
+#pragma unroll
+      for (int c = 0; c < kGjB; ++c) row[c] = Cp[r * LC + c];
+#pragma unroll
+      for (int c = 0; c < kGjB; ++c) {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < kGjB; ++q) acc = fmaf(row[q], Ai[q * LC + c], acc);
+        out[c] = -acc;
+      }
+#pragma unroll
+      for (int c = 0; c < kGjB; ++c) Cp[r * LC + c] = out[c];
+    }
+    __syncthreads();
+    // (e) A[i][j] += C[i] R[:, j] outside the panel's rows and columns, 16 x 16 MFMA tiles
+    const int nt = nr / kGjB, pb = k0 / kGjB;
+    const int li = lane & 15, kk = lane >> 4;
+    for (int tile = wave; tile < nt * nt; tile += NW) {
+      const int I = tile / nt, J = tile - (tile / nt) * nt;
+      if (I == pb || J == pb) continue;  // uniform over the wave
+      const int col = J * kGjB + li;
+      nh_vf32x4 acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = I * kGjB + 4 * kk + r;
+        acc[r] = (row < n && col < n) ? A[(size_t)row * n + col] : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float av = Cp[(I * kGjB + li) * LC + 4 * q + kk];
+        const float bv = Rp[(4 * q + kk) * ldr + col];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = I * kGjB + 4 * kk + r;
+        if (row < n && col < n) A[(size_t)row * n + col] = acc[r];
       }
     }
+    // (d') C into the panel columns; (f) A11^{-1} R into the panel rows, A11^{-1} on the diagonal block
+    for (int r = t; r < n; r += NT) {
+      if (r >= k0 && r < k0 + kb) continue;
+      for (int c = 0; c < kb; ++c) A[(size_t)r * n + k0 + c] = Cp[r * LC + c];
+    }
+    for (int rr = 0; rr < kb; ++rr)
+      for (int c = t; c < n; c += NT) {
+        float v;
+        if (c >= k0 && c < k0 + kb) {
+          v = Ai[rr * LC + (c - k0)];
+        } else {
+          v = 0.f;
+#pragma unroll
+          for (int q = 0; q < kGjB; ++q) v = fmaf(Ai[rr * LC + q], Rp[q * ldr + c], v);
+        }
+        A[(size_t)(k0 + rr) * n + c] = v;
+      }
     __syncthreads();
   }
-  if (flag[0]) {
-    for (size_t e = t; e < (size_t)n * n; e += kVjpThreads) A[e] = NAN;
+  if (singular) {
+    for (int i = wave; i < n; i += NW)
+      for (int j = lane; j < n; j += kWave) A[(size_t)i * n + j] = NAN;
     return;
   }
   // undo the row interchanges as one column permutation, then scale
@@ -296,11 +430,12 @@ __global__ void __launch_bounds__(kVjpThreads) k_pf_vjp(int n, const float* __re
     }
   }
   const float scale = factor * (cot ? cot[b] : 1.0f);
+  float* rowk = Cp;
   for (int i = 0; i < n; ++i) {
     __syncthreads();  // perm ready; the previous row's reads of rowk are done
-    for (int j = t; j < n; j += kVjpThreads) rowk[j] = A[(size_t)i * n + j];
+    for (int j = t; j < n; j += NT) rowk[j] = A[(size_t)i * n + j];
     __syncthreads();
-    for (int j = t; j < n; j += kVjpThreads) A[(size_t)i * n + j] = scale * rowk[perm[j]];
+    for (int j = t; j < n; j += NT) A[(size_t)i * n + j] = scale * rowk[perm[j]];
   }
 }
 
@@ -342,8 +477,12 @@ static int inverse_transpose(void* stream, int32_t batch, int32_t n, const float
   if (batch == 0 || n == 0) return NH_OK;
   if (!F || !G || F == G) return NH_ERR_ARG;
   if (n > 1024) return NH_ERR_SHAPE;
-  const size_t lds = sizeof(float) * 3 * (size_t)n + sizeof(int) * (2 * (size_t)n + 1);
-  hipLaunchKernelGGL(k_pf_vjp, dim3(batch), dim3(kVjpThreads), lds, (hipStream_t)stream, n, F, cot, factor, G);
+  if (n <= 256)
+    hipLaunchKernelGGL(k_inv_gj<256>, dim3(batch), dim3(256), gj_lds_bytes(n, 256), (hipStream_t)stream, n, F, cot,
+                       factor, G);
+  else
+    hipLaunchKernelGGL(k_inv_gj<1024>, dim3(batch), dim3(1024), gj_lds_bytes(n, 1024), (hipStream_t)stream, n, F,
+                       cot, factor, G);
   NH_LAUNCH_CHECK();
   return NH_OK;
 }

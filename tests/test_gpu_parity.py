@@ -308,8 +308,8 @@ def test_log_psi_image_forms_match_oracle(LX, LY, n_up, n_dn, channels):
 def test_log_psi_pf32_matches_oracle(precision):
     """32x32 Pfaffian backflow (896 x 896 Pfaffians, triangles in the global workspace; the
     feature network on VALU in FP32, on split-FP16 MFMA with 32-site rows in the mixed form) on 8
-    configurations against the FP64 C restatement: same sign and |d log psi| < 1e-2 (the
-    measured error is printed; FP32 elimination over 448 pivot steps)."""
+    configurations against the FP64 C restatement: same sign and |d log psi| < 1e-3, the 16x16
+    bar held at 32x32 too (measured up to 1.1e-4; FP32 elimination over 448 pivot steps)."""
     from neuralhubbard_amd.backflow import Variational
     from oracle.cpu_port import CpuPath
 
@@ -322,7 +322,7 @@ def test_log_psi_pf32_matches_oracle(precision):
     for b in range(len(s)):
         print(f"pf32 {precision} b={b} gpu={la[b]:.6f} fp64={rl[b]:.6f} |d|={abs(la[b] - rl[b]):.2e}", flush=True)
     assert np.array_equal(sg, rs)
-    assert np.abs(la - rl).max() < 1e-2
+    assert np.abs(la - rl).max() < 1e-3
 
 
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
