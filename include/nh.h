@@ -225,6 +225,22 @@ int nh_local_energy(void* stream, const nh_model* model, double t, double tprime
                     int32_t batch, const int8_t* s, const float* sign_s, const double* logabs_s,
                     double* eloc, int32_t chunk, void* workspace, size_t workspace_bytes);
 
+/* ---- a13 with low-rank hop ratios (the north star's low-rank Pfaffian-ratio updates for the
+ * nearest- and next-nearest-neighbour hops of the local energy; replaces the ratios inside
+ * quantax Hubbard.Oloc, code/quick_start.py:33, for the synthetic network)
+ * The same E_loc as nh_local_energy, with ψ(s') / ψ(s) of every hop as a rank-2k update of the
+ * sample's eight image matrices (k = the occupied modes whose features the hop changes: its
+ * two ends' receptive-field squares), Pf(F') / Pf(F) = (-1)^(k(k+1)/2) Pf(-J + B^T F^{-1} B),
+ * F^{-1} once per sample image by the blocked Gauss-Jordan kernel refined by one FP64 Newton
+ * step, the GEMMs on rocBLAS.  Pfaffian
+ * states with the projection and network NH_NET_CONV, where 2k <= 1024 holds by the receptive
+ * field (the 32x32 workload: k <= 508); other models return NH_ERR_UNSUPPORTED.  chunk samples
+ * per pass (chunk * 8 <= the image chunk: 128 at n = 896); the call synchronises the stream. */
+size_t nh_local_energy_lowrank_workspace_size(const nh_model* model, double tprime, int32_t chunk);
+int nh_local_energy_lowrank(void* stream, const nh_model* model, double t, double tprime, double U,
+                            int32_t batch, const int8_t* s, const float* sign_s, const double* logabs_s,
+                            double* eloc, int32_t chunk, void* workspace, size_t workspace_bytes);
+
 /* Hop enumeration used by nh_local_energy, exposed for checking:
  * for each sample, hops in order (occupied mode ascending, partner in the order
  * (+x, -x, +y, -y, +x+y, +x-y, -x+y, -x-y)); out_from/out_to/out_coef [batch][max_hops],

@@ -84,6 +84,8 @@ SIGNATURES = {
     "nh_log_psi_jacobian_workspace_size": (_S, [_MP, _I]),
     "nh_log_psi_jacobian": (_I, [_P, _MP, _D, _I, _P, _P, _P, _P, _P, _S]),
     "nh_local_energy_workspace_size": (_S, [_MP, _D, _I]),
+    "nh_local_energy_lowrank_workspace_size": (_S, [_MP, _D, _I]),
+    "nh_local_energy_lowrank": (_I, [_P, _MP, _D, _D, _D, _I, _P, _P, _P, _P, _I, _P, _S]),
     "nh_local_energy": (_I, [_P, _MP, _D, _D, _D, _I, _P, _P, _P, _P, _I, _P, _S]),
     "nh_hops": (_I, [_P, _I, _I, _D, _D, _I, _I, _P, _I, _P, _P, _P, _P]),
     "nh_sur_nparams": (_I, [_I]),

@@ -462,9 +462,13 @@ class Hubbard:
     """Hubbard model (``quantax.operator.Hubbard``; built at ``code/states.py:244-247``):
     H = -t sum_<ij>,s (c+_is c_js + h.c.) - t' sum_<<ij>>,s (...) + U sum_i n_iu n_id."""
 
-    def __init__(self, U: float = 8.0, t: float = 1.0, tprime: float = 0.0, chunk: int = 16):
+    def __init__(self, U: float = 8.0, t: float = 1.0, tprime: float = 0.0, chunk: int = 16,
+                 lowrank: bool = False):
+        """lowrank: the hop ratios as low-rank updates of each sample's image matrices
+        (``nh_local_energy_lowrank``: Pfaffian states on the synthetic network, e.g. 32x32)."""
         self.U, self.t, self.tprime = float(U), float(t), float(tprime)
         self.chunk = int(chunk)
+        self.lowrank = bool(lowrank)
         self._ws = _Workspace()
 
     def Oloc(self, state: Variational, samples) -> "object":
@@ -481,15 +485,17 @@ class Hubbard:
         m = state.model.struct(state.symm)
         lib = _lib.load()
         chunk = max(1, min(self.chunk, B))
-        need = lib.nh_local_energy_workspace_size(ctypes.byref(m), self.tprime, chunk)
+        size = lib.nh_local_energy_lowrank_workspace_size if self.lowrank else lib.nh_local_energy_workspace_size
+        need = size(ctypes.byref(m), self.tprime, chunk)
         if need == 0:
-            raise _lib.NhError("invalid model description")
+            raise _lib.NhError("model or chunk not supported" if self.lowrank else "invalid model description")
         ws = self._ws.get(need)
         sign = psi.sign.contiguous().float()
         logabs = psi.logabs.contiguous().double()
-        _lib.check(lib.nh_local_energy(_lib.stream_handle(), ctypes.byref(m), self.t, self.tprime, self.U, B,
-                                       _lib.ptr(s), _lib.ptr(sign), _lib.ptr(logabs), _lib.ptr(eloc), chunk,
-                                       _lib.ptr(ws), ws.numel()), "nh_local_energy")
+        fn = lib.nh_local_energy_lowrank if self.lowrank else lib.nh_local_energy
+        _lib.check(fn(_lib.stream_handle(), ctypes.byref(m), self.t, self.tprime, self.U, B, _lib.ptr(s),
+                      _lib.ptr(sign), _lib.ptr(logabs), _lib.ptr(eloc), chunk, _lib.ptr(ws), ws.numel()),
+                   "nh_local_energy_lowrank" if self.lowrank else "nh_local_energy")
         return eloc
 
 

@@ -27,7 +27,8 @@ FLAGS = [
 def build(verbose: bool = False) -> str:
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    # rocBLAS: the plain GEMMs of the low-rank hop ratios (nh_lowrank.hip)
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs, "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

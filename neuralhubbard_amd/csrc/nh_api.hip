@@ -12,14 +12,6 @@
 
 namespace nh {
 
-int launch_hops(hipStream_t st, int LX, int LY, float t, float tp, int z, int batch, int b0,
-                const int8_t* s, int max_hops, int32_t* count, int32_t* item_b, int32_t* item_hop,
-                float* coef, int32_t* out_from, int32_t* out_to);
-int launch_eloc_reduce(hipStream_t st, int N, double U, int batch, int b0, const int8_t* s,
-                       const float* sign_s, const double* logabs_s, int max_hops, const int32_t* count,
-                       const float* coef, const float* hop_sign, const double* hop_logabs, double* eloc);
-
-
 // ---- measurement hook: HIP events around every launch of a kernel class, on its stream
 struct ProfRec {
   hipEvent_t a, b;

@@ -48,6 +48,20 @@ size_t image_amplitudes_bytes(const nh_model* m, int64_t batch);
 int image_amplitudes(hipStream_t st, const nh_model* m, const int8_t* s, int64_t batch, void* ws, float** img_sign,
                      double** img_logabs, float* sign, double* logabs);
 
+// G[b] = factor * F[b]^{-T} for batch n x n matrices (blocked Gauss-Jordan; NaN if singular);
+// factor -1 gives F^{-1} for skew F.
+int launch_inverse_t(hipStream_t st, int batch, int n, const float* F, float factor, float* G);
+
+// The hop list of samples [b0, b0 + batch): max_hops slots per sample, valid hops first
+// (count[lb] of them), coef = -t_ij (-1)^(occupied modes strictly between the two modes).
+int launch_hops(hipStream_t st, int LX, int LY, float t, float tp, int z, int batch, int b0, const int8_t* s,
+                int max_hops, int32_t* count, int32_t* item_b, int32_t* item_hop, float* coef, int32_t* out_from,
+                int32_t* out_to);
+// E_loc from the hop items' amplitudes (hop_sign / hop_logabs per slot).
+int launch_eloc_reduce(hipStream_t st, int N, double U, int batch, int b0, const int8_t* s, const float* sign_s,
+                       const double* logabs_s, int max_hops, const int32_t* count, const float* coef,
+                       const float* hop_sign, const double* hop_logabs, double* eloc);
+
 // Signed log-sum over the ipi images of each item in [item0, item_end).
 int launch_combine(hipStream_t st, const int32_t* item_hop, int item0, int item_end, int ipi,
                    const float* img_sign, const double* img_logabs, float* sign, double* logabs);

@@ -487,6 +487,12 @@ static int inverse_transpose(void* stream, int32_t batch, int32_t n, const float
   return NH_OK;
 }
 
+namespace nh {
+int launch_inverse_t(hipStream_t st, int batch, int n, const float* F, float factor, float* G) {
+  return inverse_transpose(st, batch, n, F, nullptr, factor, G);
+}
+}  // namespace nh
+
 extern "C" int nh_slogpf_vjp(void* stream, int32_t batch, int32_t n, const float* F, const float* cot, float* G) {
   return inverse_transpose(stream, batch, n, F, cot, 0.5f, G);
 }
