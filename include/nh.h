@@ -235,7 +235,8 @@ int nh_local_energy(void* stream, const nh_model* model, double t, double tprime
  * step, the GEMMs on rocBLAS.  Pfaffian
  * states with the projection and network NH_NET_CONV, where 2k <= 1024 holds by the receptive
  * field (the 32x32 workload: k <= 508); other models return NH_ERR_UNSUPPORTED.  chunk samples
- * per pass (chunk * 8 <= the image chunk: 128 at n = 896); the call synchronises the stream. */
+ * per pass (chunk * 8 <= the image chunk: 128 at n = 896); the call synchronises the stream, and
+ * calls on one device run one at a time (they share that device's rocBLAS handle). */
 size_t nh_local_energy_lowrank_workspace_size(const nh_model* model, double tprime, int32_t chunk);
 int nh_local_energy_lowrank(void* stream, const nh_model* model, double t, double tprime, double U,
                             int32_t batch, const int8_t* s, const float* sign_s, const double* logabs_s,

@@ -281,14 +281,17 @@ __global__ void k_lr_to_f32(size_t total, const double* __restrict__ Gd, float* 
   if (e < total) G[e] = (float)Gd[e];
 }
 
-std::mutex g_rb_mu;
+// One rocBLAS handle per device, created on first use; a call holds its device's lock for its
+// whole duration (the handle carries the caller's stream), so calls on one device serialise.
+std::mutex g_rb_mu[kLrMaxDev];
 rocblas_handle g_rb[kLrMaxDev] = {};
 
-int rb_handle(hipStream_t st, rocblas_handle* h) {
-  int dev = 0;
-  NH_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= kLrMaxDev) return NH_ERR_HIP;
-  std::lock_guard<std::mutex> lk(g_rb_mu);
+int rb_device(int* dev) {
+  NH_HIP(hipGetDevice(dev));
+  return (*dev < 0 || *dev >= kLrMaxDev) ? NH_ERR_HIP : NH_OK;
+}
+
+int rb_handle(int dev, hipStream_t st, rocblas_handle* h) {
   if (!g_rb[dev] && rocblas_create_handle(&g_rb[dev]) != rocblas_status_success) return NH_ERR_HIP;
   if (rocblas_set_stream(g_rb[dev], st) != rocblas_status_success) return NH_ERR_HIP;
   *h = g_rb[dev];
@@ -396,8 +399,12 @@ extern "C" int nh_local_energy_lowrank(void* stream, const nh_model* model, doub
   if (!s || !sign_s || !logabs_s || !eloc || !workspace) return NH_ERR_ARG;
   if (workspace_bytes < nh_local_energy_lowrank_workspace_size(model, tprime, chunk)) return NH_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
+  int dev = 0;
+  rc = rb_device(&dev);
+  if (rc != NH_OK) return rc;
+  std::lock_guard<std::mutex> lk(g_rb_mu[dev]);
   rocblas_handle rb;
-  rc = rb_handle(st, &rb);
+  rc = rb_handle(dev, st, &rb);
   if (rc != NH_OK) return rc;
   LrCarver cv{(char*)(((size_t)workspace + 255) & ~(size_t)255)};
   const LrBuffers bf = lr_carve(cv, model, tprime, chunk);
