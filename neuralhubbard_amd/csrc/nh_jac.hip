@@ -351,12 +351,25 @@ __global__ void __launch_bounds__(kJT) k_jwgrad(int N, int LX, int LY, int Cin, 
   if (threadIdx.x == 0) row[b_off + o] = (float)bt;
   for (int tc = threadIdx.x; tc < 9 * Cin; tc += kJT) {
     const int tap = tc / Cin, c = tc % Cin;
+    const int dx = tap / 3 - 1, dy = tap % 3 - 1;
     double acc = 0.0;
     for (int g = 0; g < ipi; ++g) {
       const float* a = A + (((size_t)b * ipi + g) * Cin + c) * N;
       const float* dg = dl + g * N;
       float part = 0.f;
-      for (int site = 0; site < N; ++site) part = fmaf(dg[site], a[tap_site(site, tap, 1, LX, LY)], part);
+      for (int x = 0; x < LX; ++x) {  // the lattice row by row: neighbour (x + dx, y + dy), periodic
+        int xx = x + dx;
+        xx += xx < 0 ? LX : 0;
+        xx -= xx >= LX ? LX : 0;
+        const float* ar = a + xx * LY;
+        const float* dr = dg + x * LY;
+        for (int y = 0; y < LY; ++y) {
+          int yy = y + dy;
+          yy += yy < 0 ? LY : 0;
+          yy -= yy >= LY ? LY : 0;
+          part = fmaf(dr[y], ar[yy], part);
+        }
+      }
       acc += part;
     }
     row[w_off + (size_t)tc * Cout + o] = (float)acc;
