@@ -169,7 +169,21 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
         return;
       }
       const int beta = (int)(kIdx - (km & kIdx));
+      // next kap: the smallest live row other than kap and beta.  Its column is untouched until
+      // the panel's trailing update, so its load goes out together with column beta's: one memory
+      // latency per step instead of two in a row
+      int knext = kNoRow;
+#pragma unroll
+      for (int w = 0; w < NW; w += 2) {
+        const int4 lw = *reinterpret_cast<const int4*>(lows + 2 * w);  // waves w, w + 1
+        const int a = __builtin_amdgcn_readfirstlane(lw.x == beta ? lw.y : lw.x);
+        const int b = __builtin_amdgcn_readfirstlane(lw.z == beta ? lw.w : lw.z);
+        knext = min(knext, min(a, b));
+      }
+      knext = knext < m ? knext : 0;
+      const bool alive_next = cand && t != beta;
       float v = alive ? Acur[beta * ldc + t] : 0.f;
+      if (s + 1 < steps) cnext = alive_next ? Acur[knext * ldc + t] : 0.f;
       const float piv = uniform_f32(cbuf[beta]);
 #pragma unroll
       for (int q = 0; q < s; ++q) {
@@ -194,20 +208,9 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
         pu[s] = tau;
         pw[s] = v;
       }
-      alive = cand && t != beta;
-      // next kap: the smallest live row other than kap and beta
-      int knext = kNoRow;
-#pragma unroll
-      for (int w = 0; w < NW; w += 2) {
-        const int4 lw = *reinterpret_cast<const int4*>(lows + 2 * w);  // waves w, w + 1
-        const int a = __builtin_amdgcn_readfirstlane(lw.x == beta ? lw.y : lw.x);
-        const int b = __builtin_amdgcn_readfirstlane(lw.z == beta ? lw.w : lw.z);
-        knext = min(knext, min(a, b));
-      }
-      kap = knext < m ? knext : 0;
+      alive = alive_next;
+      kap = knext;
       blast = beta;
-      // the next column is untouched until the panel's trailing update: load it now
-      if (s + 1 < steps) cnext = alive ? Acur[kap * ldc + t] : 0.f;
       __syncthreads();
     };
     StaticSteps<0, P>::run(step);
