@@ -40,6 +40,16 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
   uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
   return max(max(a, b), max(c, d));
 }
+// Sum over each row of 16 lanes of a float, lane 0's row taken for the whole wave (uniform):
+// ((Q0 + Q3) + (Q2 + Q1)) of the quad sums Q_k = (v_4k + v_4k+1) + (v_4k+2 + v_4k+3).
+#define NH_DPPF(v, ctrl) __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false))
+__device__ inline float row16_sum_f32(float v) {
+  v += NH_DPPF(v, 0xB1);   // quad_perm [1,0,3,2]
+  v += NH_DPPF(v, 0x4E);   // quad_perm [2,3,0,1]
+  v += NH_DPPF(v, 0x124);  // row_ror:4
+  v += NH_DPPF(v, 0x128);  // row_ror:8
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+}
 __device__ inline int wave_min_i32(int v) {
   v = min(v, (int)NH_DPP(v, 0xB1));
   v = min(v, (int)NH_DPP(v, 0x4E));
@@ -828,9 +838,9 @@ __global__ void __launch_bounds__(NT) k_slogpf_ws(int n, int b0, const float* __
   }
 }
 
-// Column form of the workspace entry (146 <= n <= 256; per matrix 0.233 against 0.257 us at
-// n = 162, 0.400 against 0.536 at 224, 0.534 against 0.753 at 256, and the slower form up to
-// n = 130, 0.173 against 0.150 us): matrix b0 + blockIdx.x's strict lower
+// Column form of the workspace entry (146 <= n <= 256; per matrix 0.172 against 0.257 us at
+// n = 162, 0.297 against 0.536 at 224, 0.427 against 0.753 at 256 for the packed workspace
+// triangle): matrix b0 + blockIdx.x's strict lower
 // triangle becomes a column-major skew square in its workspace slice (M[j * ld + i] = A[i][j]
 // for i > j, -A[j][i] for i < j, zero on the diagonal and in the padding rows n .. ld - 1;
 // gstride >= pfl_matrix_floats<kPanelCm>(n) floats), which pf_lazy eliminates.  The lower
@@ -853,25 +863,25 @@ __global__ void __launch_bounds__(256, 4) k_slogpf_cm(int n, int b0, const float
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int i = i0 + r0 + k * (NT / kWave), j = j0 + c;
-        v[k] = i < n && j < i ? Ab[(size_t)i * n + j] : 0.f;
+        v[k] = i < n && j < i ? Ab[i * n + j] : 0.f;  // 32-bit offsets (n <= 256): one register each
       }
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int r = r0 + k * (NT / kWave), i = i0 + r, j = j0 + c;
-        if (i < n && j < i) M[(size_t)i * ld + j] = -v[k];
+        if (i < n && j < i) M[i * ld + j] = -v[k];
         lds[r * TS + c] = v[k];
       }
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int jr = r0 + k * (NT / kWave), i = i0 + c, j = j0 + jr;
-        if (i < n && j < i) M[(size_t)j * ld + i] = lds[c * TS + jr];
+        if (i < n && j < i) M[j * ld + i] = lds[c * TS + jr];
       }
       __syncthreads();
     }
   for (int j = threadIdx.x; j < n; j += NT) {
-    M[(size_t)j * ld + j] = 0.f;
-    for (int i = n; i < ld; ++i) M[(size_t)j * ld + i] = 0.f;
+    M[j * ld + j] = 0.f;
+    for (int i = n; i < ld; ++i) M[j * ld + i] = 0.f;
   }
   __syncthreads();
   float sg;

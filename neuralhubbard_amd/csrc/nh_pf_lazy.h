@@ -1,7 +1,8 @@
 // Blocked Parlett-Reid Pfaffian with lazy pivoting (pf_lazy) on a full skew square stored
 // column-major, and its scratch size.  A fragment, not a standalone header: it is included
-// inside namespace nh of nh_linalg.hip after kWave, wave_max_u32, wave_sum_f64 and nh_f32x4, and
-// by the host emulation under tests/host, which supplies those and the wave builtins itself.
+// inside namespace nh of nh_linalg.hip after kWave, wave_max_u32, row16_sum_f32, wave_sum_f64 and
+// nh_f32x4, and by the host emulation under tests/host, which supplies those and the wave
+// builtins itself.
 //
 // Why this form: every step of Parlett-Reid reads two columns of the remaining matrix (the
 // pivot column and the column of the chosen pivot row).  With the matrix as a packed row-major
@@ -43,14 +44,15 @@ __host__ __device__ inline size_t pfl_matrix_floats(int n) {
 }
 
 // Scratch floats: the panel rows UW [ld][2P + 2] (u_q, w_q interleaved; after each panel the
-// same space holds the q-major operands UT [P][ldq], WT [P][ldq]), the column buffer [NT], pivot
-// keys (16 words), the waves' live-row masks (32 words) and two smallest live rows (32 words),
-// pivots [NT / 2], the elimination order [NT], the original row of each current row (two
-// arrays [NT]), the current row of each compacted row [NT], parity partials (16 words).
+// same space holds the q-major operands UT [P][ldq], WT [P][ldq]), the column buffer [NT], and
+// for even and odd steps each: pivot keys (16 words), the waves' live-row masks (32 words) and
+// two smallest live rows (32 words); pivots [NT / 2], the elimination order [NT], the original
+// row of each current row (two arrays [NT]), the current row of each compacted row [NT], parity
+// partials (16 words).
 template <int NT, int P>
 __host__ __device__ inline int pfl_scratch_floats(int ld) {
   const int uw = ld * (2 * P + 2), ops = 2 * P * pfl_ldq(ld);
-  return ((uw > ops ? uw : ops) + NT + 16 + 32 + 32 + NT / 2 + NT + 3 * NT + 16 + 3) & ~3;
+  return ((uw > ops ? uw : ops) + NT + 2 * (16 + 32 + 32) + NT / 2 + NT + 3 * NT + 16 + 3) & ~3;
 }
 
 // Calls f(StepIndex<S>{}) for S = 0 .. N - 1 with S a compile-time constant, so that the
@@ -100,10 +102,10 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
   float* UT = scr;
   float* WT = scr + P * ldq;
   float* cbuf = scr + (uw > ops ? uw : ops);
-  uint32_t* keys = reinterpret_cast<uint32_t*>(cbuf + NT);
-  uint32_t* amask = keys + 16;  // per wave: 64-bit mask of its live rows other than kap
-  int* lows = reinterpret_cast<int*>(amask + 32);  // per wave: its two smallest such rows
-  float* pivs = reinterpret_cast<float*>(lows + 32);
+  uint32_t* keys2 = reinterpret_cast<uint32_t*>(cbuf + NT);  // [2][16]: even and odd steps
+  uint32_t* amask2 = keys2 + 32;  // [2][32], per wave: 64-bit mask of its live rows other than kap
+  int* lows2 = reinterpret_cast<int*>(amask2 + 64);  // [2][32], per wave: its two smallest such rows
+  float* pivs = reinterpret_cast<float*>(lows2 + 64);
   int* order = reinterpret_cast<int*>(pivs + NT / 2);  // original rows kap_0, beta_0, kap_1, ...
   int* orig = order + NT;                               // original row of each current row
   int* orig2 = orig + NT;
@@ -132,18 +134,23 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
       pu[q] = 0.f;
       pw[q] = 0.f;
     }
+    float ku = 0.f, kw = 0.f;  // the newest pair (u, w) of row kap, not yet read back from UW
     auto step = [&](auto sidx) {
       constexpr int s = decltype(sidx)::value;
       if (s >= steps || singular) return;  // uniform
+      uint32_t* keys = keys2 + 16 * (s & 1);
+      uint32_t* amask = amask2 + 32 * (s & 1);
+      int* lows = lows2 + 32 * (s & 1);
       // (1) column kap with the panel's earlier pairs, the wave's pivot key and live rows
       float c = cnext;
 #pragma unroll
-      for (int q = 0; q < s; ++q) {
+      for (int q = 0; q + 1 < s; ++q) {
         const float2 x = *reinterpret_cast<const float2*>(UW + kap * LS + 2 * q);  // u_q, w_q of row kap
         c = fmaf(pu[q], x.y, fmaf(-pw[q], x.x, c));
       }
-      cbuf[t] = c;
+      if (s > 0) c = fmaf(pu[s - 1], kw, fmaf(-pw[s - 1], ku, c));
       const bool cand = alive && t != kap;
+      if (cand) cbuf[t] = c;
       uint32_t key = cand ? ((__float_as_uint(fabsf(c)) & ~kIdx) | (kIdx - (uint32_t)t)) : 0u;
       key = wave_max_u32(key);
       const uint64_t am = __ballot(cand);
@@ -155,8 +162,12 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
         lows[2 * wave] = am ? kWave * wave + __builtin_ctzll(am) : kNoRow;
         lows[2 * wave + 1] = am2 ? kWave * wave + __builtin_ctzll(am2) : kNoRow;
       }
+      // The step's only barrier.  The next step writes the other half of keys / amask / lows, and
+      // cbuf only for its candidate rows, which are neither this step's beta (dead) nor its next
+      // kap (the next step's kap): the two entries of cbuf read below.  UW rows are read below
+      // at pairs q < s only, while the next step writes pair s + 1.
       __syncthreads();
-      // (2) pivot row, column beta, the step's pair (u, w), the next kap
+      // (2) pivot row, column beta, the step's pair (u, w), the next kap and its pair
       uint32_t km = 0u;
 #pragma unroll
       for (int w = 0; w < NW; w += 4) {
@@ -180,11 +191,28 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
         const int b = __builtin_amdgcn_readfirstlane(lw.z == beta ? lw.w : lw.z);
         knext = min(knext, min(a, b));
       }
-      knext = knext < m ? knext : 0;
+      knext = knext < m ? knext : 0;  // (a live row whenever the panel has another step)
       const bool alive_next = cand && t != beta;
       float v = alive ? Acur[beta * ldc + t] : 0.f;
-      if (s + 1 < steps) cnext = alive_next ? Acur[knext * ldc + t] : 0.f;
+      cnext = alive_next ? Acur[knext * ldc + t] : 0.f;
+      // Every wave forms the next kap's pair itself, so that the next step has it in registers
+      // and no barrier has to publish it: u = c[knext] / piv as row knext does, w = A[knext][beta]
+      // + sum_{q<s} (u_q(knext) w_q(beta) - w_q(knext) u_q(beta)), one term per lane q of each
+      // row of 16 lanes (the rounding of that sum may differ from row knext's own w; the pair
+      // only corrects the column of kap, which is eliminated in the next step)
+      const float ak = Acur[beta * ldc + knext];
       const float piv = uniform_f32(cbuf[beta]);
+      const float ck = cbuf[knext];
+      float dk = 0.f;
+      if (s > 0) {
+        const int q = lane & 15;
+        if (q < s) {
+          const float2 x = *reinterpret_cast<const float2*>(UW + beta * LS + 2 * q);
+          const float2 y = *reinterpret_cast<const float2*>(UW + knext * LS + 2 * q);
+          dk = fmaf(y.x, x.y, -y.y * x.x);
+        }
+        dk = row16_sum_f32(dk);
+      }
 #pragma unroll
       for (int q = 0; q < s; ++q) {
         const float2 x = *reinterpret_cast<const float2*>(UW + beta * LS + 2 * q);
@@ -197,6 +225,8 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
       float rp = __builtin_amdgcn_rcpf(piv);
       rp = fmaf(rp, fmaf(-piv, rp, 1.f), rp);  // one Newton step
       const float tau = c * rp;
+      ku = ck * rp;
+      kw = ak + dk;
       if (piv > 0.f) sign = -sign;  // Pf *= A[kap][beta] = -piv
       if (t == 0) {
         pivs[done + s] = piv;
@@ -211,9 +241,9 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
       alive = alive_next;
       kap = knext;
       blast = beta;
-      __syncthreads();
     };
     StaticSteps<0, P>::run(step);
+    __syncthreads();  // the steps' last reads of UW, cbuf and the masks are done
     if (singular) break;
     if (done + steps >= nsteps) break;
     // Schur complement of the live rows, compacted: a live row's new index is the number of live
@@ -221,6 +251,7 @@ __device__ void pf_lazy(float* __restrict__ A, int n, float* scr, float& sign_ou
     const int mn = m - 2 * steps;
     const int ldn = pfl_ld(mn);
     int pos = 0;
+    const uint32_t* amask = amask2 + 32 * ((steps - 1) & 1);
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
       uint64_t mw = (uint64_t)amask[2 * w] | ((uint64_t)amask[2 * w + 1] << 32);

@@ -99,6 +99,23 @@ inline uint32_t wave_max_u32(uint32_t v) {
   return (uint32_t)m;
 }
 
+// row16_sum_f32: lane 0's row of 16 summed in the grouping of the device's DPP sequence
+// (quad_perm [1,0,3,2], [2,3,0,1], row_ror:4, row_ror:8, then lane 0): ((Q0 + Q3) + (Q2 + Q1)).
+inline float row16_sum_f32(float v) {
+  EmuWave& w = my_wave();
+  w.d[my_lane()] = v;
+  w.bar.arrive_and_wait();
+  float q[4];
+  for (int k = 0; k < 4; ++k) {
+    const float a = (float)w.d[4 * k], b = (float)w.d[4 * k + 1], c = (float)w.d[4 * k + 2],
+                d = (float)w.d[4 * k + 3];
+    q[k] = (a + b) + (c + d);
+  }
+  const float r = (q[0] + q[3]) + (q[2] + q[1]);
+  w.bar.arrive_and_wait();
+  return r;
+}
+
 inline double __shfl_xor(double x, int o, int width) {
   EmuWave& w = my_wave();
   const int lane = my_lane();

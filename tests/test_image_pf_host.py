@@ -37,6 +37,22 @@ def harness(tmp_path_factory):
     return exe
 
 
+@pytest.fixture(scope="module")
+def harness_tsan(tmp_path_factory):
+    """The same harness under ThreadSanitizer: the emulated workgroup's threads synchronise only
+    through __syncthreads (a std::barrier) and the wave exchanges, so an access to LDS or to the
+    workspace that a barrier does not order against another thread's write is reported."""
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    exe = str(tmp_path_factory.mktemp("imgt") / "image_pf_tsan")
+    cmd = ["g++", "-std=c++20", "-O1", "-g", "-fsanitize=thread", "-fno-strict-aliasing", "-pthread",
+           "-Wno-unknown-pragmas", "-I", os.path.join(ROOT, "neuralhubbard_amd", "csrc"),
+           os.path.join(HERE, "host", "image_pf_host.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
 def _case(LX, LY, n_up, n_dn, D, seed, f0=False):
     rng = np.random.default_rng(seed)
     N = LX * LY
@@ -67,7 +83,7 @@ def _expected(LX, LY, s, Y, Wt, F0, g, hop):
     return sg * (-1) ** (ne * (ne - 1) // 2), la
 
 
-def _run(exe, tmp_path, LX, LY, n_up, n_dn, D, P, g, s, Y, Wt, F0, hop):
+def _run(exe, tmp_path, LX, LY, n_up, n_dn, D, P, g, s, Y, Wt, F0, hop, raw=False):
     N = LX * LY
     hp = -1 if hop is None else (hop[0] | (hop[1] << 16))
     head = np.array([N, LX, LY, n_up + n_dn, n_up, D, g, hp, int(F0 is not None), P], np.int32)
@@ -80,6 +96,8 @@ def _run(exe, tmp_path, LX, LY, n_up, n_dn, D, P, g, s, Y, Wt, F0, hop):
         if F0 is not None:
             f.write(F0.astype(np.float32).tobytes())
     r = subprocess.run([exe, path], capture_output=True, text=True, timeout=900)
+    if raw:
+        return r
     assert r.returncode == 0, r.stderr[-3000:]
     sg, la = r.stdout.split()
     return float(sg), float(la)
@@ -121,3 +139,33 @@ def test_image_pf_cm_host_matches_oracle(harness, tmp_path, LX, LY, n_up, n_dn, 
     assert np.isfinite(la), "a value never written reached the result"
     assert sg == rs
     assert abs(la - rl) < 1e-4  # FP32 elimination: <= 3.4e-5 measured
+
+
+def _first_hop(s, N, LY):
+    occ = np.flatnonzero(s[:N] > 0)
+    for m in occ:
+        x, y = divmod(int(m), LY)
+        to = x * LY + (y + 1) % LY
+        if s[to] < 0:
+            return int(m), to
+    return None
+
+
+@pytest.mark.parametrize("LX,LY,n_up,n_dn,D,P,g,hop,f0", [
+    (16, 16, 112, 112, 16, 16, 5, True, False),
+    (10, 10, 44, 44, 8, 16, 1, False, True),
+    (9, 10, 40, 38, 16, 8, 4, True, False),
+])
+def test_image_pf_cm_host_no_data_race(harness_tsan, tmp_path, LX, LY, n_up, n_dn, D, P, g, hop, f0):
+    """No data race in the image kernel's body: the pairing build, the workspace square and every
+    pivot step and trailing update of pf_lazy (each step's LDS exchanges, the two buffers of the
+    compacted complements), with the result still the oracle's."""
+    s, Y, Wt, F0 = _case(LX, LY, n_up, n_dn, D, 1000 * LX + 10 * D + g, f0)
+    h = _first_hop(s, LX * LY, LY) if hop else None
+    r = _run(harness_tsan, tmp_path, LX, LY, n_up, n_dn, D, P, g, s, Y, Wt, F0, h, raw=True)
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
+    assert r.returncode == 0, r.stderr[-3000:]
+    sg, la = (float(x) for x in r.stdout.split())
+    rs, rl = _expected(LX, LY, s, Y, Wt, F0, g, h)
+    assert sg == rs
+    assert abs(la - rl) < 1e-4
