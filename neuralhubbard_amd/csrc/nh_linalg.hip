@@ -1,6 +1,6 @@
 // Pairing blocks, skew Pfaffian and determinant on gfx950 (FP32 arithmetic, FP64 log-sum).
 //
-// One workgroup owns one matrix.  The Pfaffian image kernel for 146 <= n <= 256 (k_image_pf_cm)
+// One workgroup owns one matrix.  The Pfaffian image kernel for 100 <= n <= 256 (k_image_pf_cm)
 // builds each image as a full skew square, column-major, in a global workspace and eliminates it
 // with pf_lazy (lazy pivoting, compacted Schur complements; nh_pf_lazy.h), and so does the
 // workspace entry nh_slogpf_ws at those sizes (k_slogpf_cm).  nh_slogpf, the other sizes of
@@ -500,7 +500,11 @@ __global__ void __launch_bounds__(NT) k_image_pf(int N, int LX, int LY, int n, i
 // n = kCmMinN on): pf_lazy with 16-step panels, the square in the global workspace (200 KB per
 // image at n = 224).
 constexpr int kPanelCm = 16;
-constexpr int kCmMinN = 146;  // smallest n of the column forms (image kernel and nh_slogpf_ws)
+// smallest n of the column forms (image kernel and nh_slogpf_ws): per matrix of nh_slogpf_ws
+// 0.096 / 0.116 us against 0.111 / 0.150 for the packed workspace triangle at n = 114 / 130, but
+// 0.083 against 0.077 at n = 98; per 16x16-style configuration (8 images) 0.928 against 0.996 us
+// at n = 128 and 0.685 against 0.694 at n = 96
+constexpr int kCmMinN = 100;
 
 template <int D>
 __global__ void __launch_bounds__(256, 4) k_image_pf_cm(int N, int LX, int LY, int n, int n_up,
@@ -838,7 +842,7 @@ __global__ void __launch_bounds__(NT) k_slogpf_ws(int n, int b0, const float* __
   }
 }
 
-// Column form of the workspace entry (146 <= n <= 256; per matrix 0.172 against 0.257 us at
+// Column form of the workspace entry (100 <= n <= 256; per matrix 0.172 against 0.257 us at
 // n = 162, 0.297 against 0.536 at 224, 0.427 against 0.753 at 256 for the packed workspace
 // triangle): matrix b0 + blockIdx.x's strict lower
 // triangle becomes a column-major skew square in its workspace slice (M[j * ld + i] = A[i][j]
@@ -1056,7 +1060,7 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
 }
 
 // Workspace floats per matrix: the column-major square and its first compacted complement for
-// 146 <= n <= 256 (348 KB at n = 224), the packed triangle otherwise.
+// 100 <= n <= 256 (348 KB at n = 224), the packed triangle otherwise.
 static bool slogpf_column_form(int n) { return n >= kCmMinN && n <= 256; }
 static size_t slogpf_ws_stride(int n) {
   if (slogpf_column_form(n)) return pfl_matrix_floats<kPanelCm>(n);

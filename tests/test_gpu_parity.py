@@ -78,9 +78,9 @@ def test_slogpf_blocked_sizes(n):
         assert abs(la[b] - l_ref) < _cond_tol(A64, 2e-4 * n / 56), (n, b, la[b], l_ref)
 
 
-@pytest.mark.parametrize("n", [66, 96, 130, 144, 146, 194, 224, 256, 258, 384, 512, 896, 1024])
+@pytest.mark.parametrize("n", [66, 96, 98, 100, 130, 144, 146, 194, 224, 256, 258, 384, 512, 896, 1024])
 def test_slogpf_workspace_form(n):
-    """nh_slogpf_ws (column-major squares in the workspace for 146 <= n <= 256, packed triangles
+    """nh_slogpf_ws (column-major squares in the workspace for 100 <= n <= 256, packed triangles
     at the other sizes, n <= 1024; n = 896 is the 32x32 Pfaffian):
     sign against the FP64 oracle, log|Pf| against LAPACK's log|det| / 2 within the
     condition-scaled tolerance of the LDS form, and a workspace of two matrices (the batch in
@@ -279,16 +279,18 @@ def test_log_psi_pf16_matches_oracle():
 
 
 @pytest.mark.parametrize("LX,LY,n_up,n_dn,channels", [
-    (12, 13, 72, 74, 16),  # column form at its smallest n = 146: the last 4-column group reaches the padding rows
+    (12, 13, 72, 74, 16),  # column form, n = 146: the last 4-column group reaches the padding rows
     (12, 13, 76, 76, 8),   # column form at d = 4 features
     (14, 14, 96, 98, 32),  # column form, n = 194, d = 16, seven 16-step panels
+    (10, 10, 48, 52, 16),  # column form at its smallest n = 100 (four panels, the last of 2 steps)
+    (12, 12, 60, 62, 32),  # column form, n = 122: the last 4-column group reaches the padding rows
     (13, 13, 74, 76, 16),  # n_up % 4 != 0: the packed-triangle kernel at n = 150
-    (12, 12, 60, 62, 32),  # n = 122, below the column form's sizes: the packed-triangle kernel
+    (10, 10, 48, 50, 16),  # n = 98, below the column form's sizes: the packed-triangle kernel
     (10, 10, 37, 37, 16),  # n = 74, packed triangle
 ])
 def test_log_psi_image_forms_match_oracle(LX, LY, n_up, n_dn, channels):
     """Both Pfaffian image kernels above n = 64 (the column-major lazy-pivot form, used for
-    146 <= n <= 256 when n_up % 4 == 0, and the packed-triangle form otherwise) against the FP64
+    100 <= n <= 256 when n_up % 4 == 0, and the packed-triangle form otherwise) against the FP64
     oracle's projected amplitude, with t' != 0 and a base matrix F0, on sizes where their edge
     cases arise."""
     from neuralhubbard_amd.backflow import Variational

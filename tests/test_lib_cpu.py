@@ -156,11 +156,12 @@ def test_argument_errors_without_gpu():
     assert lib.nh_slogpf_ws(None, 1, 895, v, v, v, v, 1 << 30) == _lib.NH_ERR_SHAPE
     assert lib.nh_slogpf_ws(None, 2, 896, v, v, v, v, lib.nh_slogpf_workspace_size(1, 896) - 1024) == _lib.NH_ERR_WORKSPACE
     assert lib.nh_slogpf_workspace_size(1, 896) == 4 * ((896 * 895 // 2 + 64 + 63) // 64 * 64) + 256
-    # 146 <= n <= 256: the column-major square (leading dimension n rounded up to 16) and the
-    # first compacted complement (n - 32 rows); n = 144: the packed triangle
+    # 100 <= n <= 256: the column-major square (leading dimension n rounded up to 16) and the
+    # first compacted complement (n - 32 rows); n = 98: the packed triangle
     assert lib.nh_slogpf_workspace_size(3, 224) == 4 * 3 * (224 * 224 + 192 * 192) + 256
     assert lib.nh_slogpf_workspace_size(1, 146) == 4 * (160 * 160 + 128 * 128) + 256
-    assert lib.nh_slogpf_workspace_size(1, 144) == 4 * ((144 * 143 // 2 + 64 + 63) // 64 * 64) + 256
+    assert lib.nh_slogpf_workspace_size(1, 100) == 4 * (112 * 112 + 80 * 80) + 256
+    assert lib.nh_slogpf_workspace_size(1, 98) == 4 * ((98 * 97 // 2 + 64 + 63) // 64 * 64) + 256
     # the layer-by-layer network needs its workspace: too small a one is refused before any launch
     R = _lib.NH_NET_RESCONV
     big = _lib.NhModel(0, 24, 24, 252, 252, 64, 16, 0, 1, R, 16, 16, 16, 0, 0)
