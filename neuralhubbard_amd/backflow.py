@@ -153,9 +153,10 @@ _SLOGPF_WS_CAP = 1 << 30  # bytes of workspace per launch; larger batches run in
 
 def slogpf(F, workspace: bool | None = None):
     """Batched (sign, log|Pf|) of antisymmetric (B, n, n) float32 (``fermix.slogpf``), n <= 1024.
-    workspace: use `nh_slogpf_ws` (the matrices in a workspace); default for n >= 146, where
-    its column-major form is faster than the LDS form (0.40 against 0.52 us per matrix at
-    n = 224), and above n = 256, where the LDS form does not reach."""
+    workspace: use `nh_slogpf_ws` (the matrices in a workspace); default for n >= 100, where
+    its column-major form is faster than the LDS form (0.116 against 0.128 us per matrix at
+    n = 130, 0.297 against 0.510 at n = 224), and above n = 256, where the LDS form does not
+    reach; below n = 100 the LDS form is the faster (0.068 against 0.075 us at n = 98)."""
     global _SLOGPF_WS
     torch = _torch()
     _lib.require_gpu()
@@ -165,7 +166,7 @@ def slogpf(F, workspace: bool | None = None):
     logabs = torch.empty(B, dtype=torch.float64, device=F.device)
     lib = _lib.load()
     if workspace is None:
-        workspace = n >= 146
+        workspace = n >= 100
     if not workspace:
         _lib.check(lib.nh_slogpf(_lib.stream_handle(), B, n, _lib.ptr(F), _lib.ptr(sign), _lib.ptr(logabs)),
                    "nh_slogpf")

@@ -1051,8 +1051,13 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
   const int small = 2 * (n + kWave) + 2 * 16 + 4;  // pf_eliminate: tau, vv, part
   const int big = pf_elim_floats_lds(n);
   const size_t lds = sizeof(float) * (size_t)(((ntri + kWave + 3) & ~3) + (n <= 64 ? small : big));
+  // 256 threads while two or more triangles share a CU (per matrix 0.032 / 0.068 / 0.128 / 0.253
+  // us at n = 66 / 98 / 130 / 162 against 0.120 / 0.184 / 0.255 / 0.336 with 1024 threads), one
+  // 1024-thread workgroup per CU above (0.436 against 0.660 us at n = 194)
   if (n <= 64)
     hipLaunchKernelGGL(k_slogpf<64>, dim3(batch), dim3(64), lds, st, n, A, sign, logabs);
+  else if (n <= 176)
+    hipLaunchKernelGGL(k_slogpf<256>, dim3(batch), dim3(256), lds, st, n, A, sign, logabs);
   else
     hipLaunchKernelGGL(k_slogpf<1024>, dim3(batch), dim3(1024), lds, st, n, A, sign, logabs);
   NH_LAUNCH_CHECK();

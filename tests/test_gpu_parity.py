@@ -57,10 +57,11 @@ def _cond_tol(A64, base):
     return base + 1e-6 * sv[0] / max(sv[-1], 1e-300)
 
 
-@pytest.mark.parametrize("n", [66, 70, 96, 130, 192, 222, 240, 242, 250, 256])
+@pytest.mark.parametrize("n", [66, 70, 96, 130, 176, 178, 192, 222, 240, 242, 250, 256])
 def test_slogpf_blocked_sizes(n):
-    """Blocked Pfaffian at sizes around its panel boundaries (8-step panels up to n = 240, 4-step
-    panels above, partial last panels): sign against the FP64 oracle, log|Pf| against
+    """Blocked Pfaffian of nh_slogpf at sizes around its boundaries (256-thread workgroups up to
+    n = 176, 1024 threads above; 8-step panels up to n = 240, 4-step panels above, partial last
+    panels): sign against the FP64 oracle, log|Pf| against
     LAPACK's log|det| / 2 within a condition-scaled tolerance."""
     from neuralhubbard_amd.backflow import slogpf
 
