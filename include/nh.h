@@ -108,7 +108,7 @@ int nh_pairing_ud(void* stream, int32_t LX, int32_t LY, int32_t d, int32_t batch
 int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A, float* sign, double* logabs);
 /* The same for n <= 1024 (fermix.slogpf at the 32x32 sizes, n = 896) with a workspace: each
  * matrix is eliminated in its workspace slice by a blocked Parlett-Reid routine of the image
- * kernels, for 100 <= n <= 256 as a column-major skew square with lazy pivoting (the two
+ * kernels, for 100 <= n <= 352 as a column-major skew square with lazy pivoting (the two
  * triangles, 348 KB per matrix at n = 224), for the other n as the packed strict lower
  * triangle (panel vectors in LDS).  Only the strict lower triangle of A is read.  A workspace of
  * nh_slogpf_workspace_size(batch, n) bytes takes the batch in one launch; a smaller one (at

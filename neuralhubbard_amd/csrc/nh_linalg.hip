@@ -3,7 +3,7 @@
 // One workgroup owns one matrix.  The Pfaffian image kernel for 100 <= n <= 256 (k_image_pf_cm)
 // builds each image as a full skew square, column-major, in a global workspace and eliminates it
 // with pf_lazy (lazy pivoting, compacted Schur complements; nh_pf_lazy.h), and so does the
-// workspace entry nh_slogpf_ws at those sizes (k_slogpf_cm).  nh_slogpf, the other sizes of
+// workspace entry nh_slogpf_ws for 100 <= n <= 352 (k_slogpf_cm).  nh_slogpf, the other sizes of
 // nh_slogpf_ws and the other Pfaffian image kernels keep the packed strict lower triangle (row i
 // at i(i-1)/2), in LDS up to n = 256 and in a global workspace above (pf_blocked2 works on
 // either).  A determinant matrix lives in LDS as a dense row-major square.
@@ -842,20 +842,22 @@ __global__ void __launch_bounds__(NT) k_slogpf_ws(int n, int b0, const float* __
   }
 }
 
-// Column form of the workspace entry (100 <= n <= 256; per matrix 0.172 against 0.257 us at
+// Column form of the workspace entry (100 <= n <= 352; per matrix 0.172 against 0.257 us at
 // n = 162, 0.297 against 0.536 at 224, 0.427 against 0.753 at 256 for the packed workspace
 // triangle): matrix b0 + blockIdx.x's strict lower
 // triangle becomes a column-major skew square in its workspace slice (M[j * ld + i] = A[i][j]
 // for i > j, -A[j][i] for i < j, zero on the diagonal and in the padding rows n .. ld - 1;
-// gstride >= pfl_matrix_floats<kPanelCm>(n) floats), which pf_lazy eliminates.  The lower
+// gstride >= pfl_matrix_floats<P>(n) floats), which pf_lazy eliminates.  The lower
 // triangle is read once, in 64 x 64 tiles with all 16 loads of a thread in flight together:
 // each tile is stored negated, as it stands, into the upper half (column i of M holds row i of
 // the tile) and transposed through LDS into the lower half.  LDS: the larger of the tile
 // (64 x 65 floats) and pf_lazy's scratch.
-__global__ void __launch_bounds__(256, 4) k_slogpf_cm(int n, int b0, const float* __restrict__ A, float* sign,
-                                                      double* logabs, float* __restrict__ gmat, size_t gstride) {
+template <int NT, int KB, int P>
+__global__ void __launch_bounds__(NT, NT == 256 ? 4 : 2) k_slogpf_cm(int n, int b0, const float* __restrict__ A,
+                                                                     float* sign, double* logabs,
+                                                                     float* __restrict__ gmat, size_t gstride) {
   extern __shared__ float lds[];
-  constexpr int NT = 256, TS = kWave + 1, PER = kWave * kWave / NT;
+  constexpr int TS = kWave + 1, PER = kWave * kWave / NT;
   const int b = b0 + blockIdx.x;
   const int ld = pfl_ld(n);
   float* M = gmat + (size_t)blockIdx.x * gstride;
@@ -867,7 +869,7 @@ __global__ void __launch_bounds__(256, 4) k_slogpf_cm(int n, int b0, const float
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int i = i0 + r0 + k * (NT / kWave), j = j0 + c;
-        v[k] = i < n && j < i ? Ab[i * n + j] : 0.f;  // 32-bit offsets (n <= 256): one register each
+        v[k] = i < n && j < i ? Ab[i * n + j] : 0.f;  // 32-bit offsets (n <= 512): one register each
       }
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
@@ -890,15 +892,16 @@ __global__ void __launch_bounds__(256, 4) k_slogpf_cm(int n, int b0, const float
   __syncthreads();
   float sg;
   double la;
-  pf_lazy<NT, kPanelCm, 8>(M, n, lds, sg, la);
+  pf_lazy<NT, P, KB>(M, n, lds, sg, la);
   if (threadIdx.x == 0) {
     sign[b] = sg;
     logabs[b] = la;
   }
 }
 
+template <int NT, int P>
 static size_t slogpf_cm_lds(int n) {
-  const size_t tile = (size_t)kWave * (kWave + 1), scr = (size_t)pfl_scratch_floats<256, kPanelCm>(pfl_ld(n));
+  const size_t tile = (size_t)kWave * (kWave + 1), scr = (size_t)pfl_scratch_floats<NT, P>(pfl_ld(n));
   return sizeof(float) * (tile > scr ? tile : scr);
 }
 
@@ -1065,10 +1068,20 @@ extern "C" int nh_slogpf(void* stream, int32_t batch, int32_t n, const float* A,
 }
 
 // Workspace floats per matrix: the column-major square and its first compacted complement for
-// 100 <= n <= 256 (348 KB at n = 224), the packed triangle otherwise.
-static bool slogpf_column_form(int n) { return n >= kCmMinN && n <= 256; }
+// 100 <= n <= 352 (348 KB at n = 224), the packed triangle otherwise.  Above n = 256 the column
+// form runs 512 threads with 12-step panels (109 VGPRs, two workgroups per CU; per matrix 0.619 /
+// 0.984 / 1.332 us at n = 258 / 320 / 352 against 0.831 / 1.222 / ~1.5 for the packed triangle in
+// batches of 16384).  Above n = 352 it would need 16-step panels (132 registers, one workgroup
+// per CU): 1.722 / 2.702 / 3.616 us at n = 384 / 460 / 512 against 1.867 / 2.932 / 3.737 in
+// batches of 16384, but slower than the packed triangle in the low-rank hop ratios' batches of
+// 512 Pfaffians of n ~ 460 (E_loc 363 against 349 ms per 32x32 sample), so the packed triangle
+// keeps those sizes.
+constexpr int kCmMaxWsN = 352;
+static bool slogpf_column_form(int n) { return n >= kCmMinN && n <= kCmMaxWsN; }
+static int slogpf_cm_panel(int n) { return n > 256 ? 12 : kPanelCm; }
 static size_t slogpf_ws_stride(int n) {
-  if (slogpf_column_form(n)) return pfl_matrix_floats<kPanelCm>(n);
+  if (slogpf_column_form(n))
+    return slogpf_cm_panel(n) == 12 ? pfl_matrix_floats<12>(n) : pfl_matrix_floats<kPanelCm>(n);
   return (((size_t)n * (n - 1)) / 2 + kWave + 63) & ~(size_t)63;
 }
 
@@ -1092,12 +1105,17 @@ extern "C" int nh_slogpf_ws(void* stream, int32_t batch, int32_t n, const float*
   if (fit < 1) return NH_ERR_WORKSPACE;
   const int chunk = fit < batch ? (int)fit : batch;
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds =
-      slogpf_column_form(n) ? slogpf_cm_lds(n) : sizeof(float) * (size_t)pf2_scratch_floats<kPanel>(n);
+  const bool cm = slogpf_column_form(n);
+  const size_t lds = !cm       ? sizeof(float) * (size_t)pf2_scratch_floats<kPanel>(n)
+                     : n <= 256 ? slogpf_cm_lds<256, kPanelCm>(n)
+                                : slogpf_cm_lds<512, 12>(n);
   for (int b0 = 0; b0 < batch; b0 += chunk) {
     const int nb = batch - b0 < chunk ? batch - b0 : chunk;
-    if (slogpf_column_form(n))
-      hipLaunchKernelGGL(k_slogpf_cm, dim3(nb), dim3(256), lds, st, n, b0, A, sign, logabs, gtri, gs);
+    if (cm && n <= 256)
+      hipLaunchKernelGGL((k_slogpf_cm<256, 8, kPanelCm>), dim3(nb), dim3(256), lds, st, n, b0, A, sign, logabs,
+                         gtri, gs);
+    else if (cm)  // 256 < n <= 352: 512 threads, 12-step panels
+      hipLaunchKernelGGL((k_slogpf_cm<512, 10, 12>), dim3(nb), dim3(512), lds, st, n, b0, A, sign, logabs, gtri, gs);
     else if (n <= 256)
       hipLaunchKernelGGL(k_slogpf_ws<256>, dim3(nb), dim3(256), lds, st, n, b0, A, sign, logabs, gtri, gs);
     else

@@ -361,7 +361,13 @@ LrBuffers lr_carve(LrCarver& c, const nh_model* m, double tprime, int cs) {
   b.M = c.take<float>(ic * 4 * kb * kb);
   b.psg = c.take<float>(ic);
   b.plg = c.take<double>(ic);
-  b.pws_bytes = nh_slogpf_workspace_size((int32_t)ic, (int32_t)(2 * kb));
+  // a pass's 2k x 2k Pfaffians have 2k <= 2 kb; the workspace form's column form takes more bytes
+  // per matrix than the packed triangle of a larger n, so the size is the largest over every 2k
+  b.pws_bytes = 0;
+  for (int32_t two_k = 2; two_k <= (int32_t)(2 * kb); two_k += 2) {
+    const size_t need = nh_slogpf_workspace_size((int32_t)ic, two_k);
+    if (need > b.pws_bytes) b.pws_bytes = need;
+  }
   b.pws = c.take<char>(b.pws_bytes);
   return b;
 }

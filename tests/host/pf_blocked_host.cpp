@@ -15,6 +15,11 @@
 //                                        workspace form of the BCS image kernel)
 //   pf_blocked_host lu8 NT n mat.bin  ->  the same with 8-bit row keys (the LDS form of the BCS
 //                                        image kernel and nh_slogdet, n <= 256)
+//   pf_blocked_host lazy NT n tri.bin ->  "sign logabs" of the packed strict lower triangle by
+//                                        pf_lazy (nh_pf_lazy.h) on the column-major skew square
+//                                        that nh_slogpf_ws's column form builds (16-step panels;
+//                                        256 threads with 8-bit row keys, 512 with 10-bit ones)
+//   pf_blocked_host lazy12 NT n tri.bin -> the same with 12-step panels (512 threads, 256 < n <= 352)
 
 #include "emu_wave.h"
 
@@ -23,6 +28,44 @@ inline int toff(int i, int j) { return ((i * (i - 1)) >> 1) + j; }
 #include "nh_pf_blocked.h"
 
 #include "nh_lu_blocked.h"
+
+#include "nh_pf_lazy.h"
+
+// the column form of nh_slogpf_ws: M[j * ld + i] = A[i][j] below the diagonal, -A[j][i] above,
+// zero on the diagonal and in the padding rows, then room for the first compacted complement
+// (NaN, so a value read before it is written shows)
+template <int NT, int KB, int P>
+static void run_lazy(int n, const std::vector<float>& in, float& sg, double& la) {
+  const int ld = pfl_ld(n);
+  const size_t nm = pfl_matrix_floats<P>(n);
+  float* M = static_cast<float*>(std::malloc(sizeof(float) * nm));
+  for (size_t i = 0; i < nm; ++i) M[i] = std::numeric_limits<float>::quiet_NaN();
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < ld; ++i)
+      M[(size_t)j * ld + i] = i >= n || i == j ? 0.f : (i > j ? in[toff(i, j)] : -in[toff(j, i)]);
+  const int nscr = pfl_scratch_floats<NT, P>(ld);
+  float* scr = static_cast<float*>(std::malloc(sizeof(float) * nscr));
+  for (int i = 0; i < nscr; ++i) scr[i] = std::numeric_limits<float>::quiet_NaN();
+  std::barrier<> block(NT);
+  std::vector<EmuWave> waves(NT / kWave);
+  g_block = &block;
+  g_waves = waves.data();
+  std::vector<std::thread> th;
+  for (int t = 0; t < NT; ++t)
+    th.emplace_back([&, t] {
+      threadIdx.x = t;
+      float s;
+      double l;
+      pf_lazy<NT, P, KB>(M, n, scr, s, l);
+      if (t == 0) {
+        sg = s;
+        la = l;
+      }
+    });
+  for (auto& x : th) x.join();
+  std::free(scr);
+  std::free(M);
+}
 
 template <int NT, int KB>
 static void run_lu(int n, const std::vector<float>& in, float& sg, double& la) {
@@ -103,8 +146,27 @@ static int main_lu(int argc, char** argv) {
   return 0;
 }
 
+static int main_lazy(char** argv) {
+  const int NT = std::atoi(argv[2]), n = std::atoi(argv[3]);
+  if (n < 2 || n % 2 || n > NT) return 2;
+  std::vector<float> in((size_t)n * (n - 1) / 2);
+  FILE* f = std::fopen(argv[4], "rb");
+  if (!f || std::fread(in.data(), sizeof(float), in.size(), f) != in.size()) return 2;
+  std::fclose(f);
+  float sg = 0.f;
+  double la = 0.0;
+  const bool p12 = std::strcmp(argv[1], "lazy12") == 0;
+  if (NT == 256 && !p12) run_lazy<256, 8, 16>(n, in, sg, la);
+  else if (NT == 512 && !p12) run_lazy<512, 10, 16>(n, in, sg, la);
+  else if (NT == 512) run_lazy<512, 10, 12>(n, in, sg, la);
+  else return 2;
+  std::printf("%.1f %.17g\n", sg, la);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc == 5 && (std::strcmp(argv[1], "lu") == 0 || std::strcmp(argv[1], "lu8") == 0)) return main_lu(argc, argv);
+  if (argc == 5 && (std::strcmp(argv[1], "lazy") == 0 || std::strcmp(argv[1], "lazy12") == 0)) return main_lazy(argv);
   if (argc != 5) {
     std::fprintf(stderr, "usage: %s NT P n tri.bin\n", argv[0]);
     return 2;

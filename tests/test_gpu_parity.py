@@ -79,9 +79,10 @@ def test_slogpf_blocked_sizes(n):
         assert abs(la[b] - l_ref) < _cond_tol(A64, 2e-4 * n / 56), (n, b, la[b], l_ref)
 
 
-@pytest.mark.parametrize("n", [66, 96, 98, 100, 130, 144, 146, 194, 224, 256, 258, 384, 512, 896, 1024])
+@pytest.mark.parametrize("n", [66, 96, 98, 100, 130, 144, 146, 194, 224, 256, 258, 320, 352, 354, 384, 460, 512, 514,
+                               896, 1024])
 def test_slogpf_workspace_form(n):
-    """nh_slogpf_ws (column-major squares in the workspace for 100 <= n <= 256, packed triangles
+    """nh_slogpf_ws (column-major squares in the workspace for 100 <= n <= 352, packed triangles
     at the other sizes, n <= 1024; n = 896 is the 32x32 Pfaffian):
     sign against the FP64 oracle, log|Pf| against LAPACK's log|det| / 2 within the
     condition-scaled tolerance of the LDS form, and a workspace of two matrices (the batch in

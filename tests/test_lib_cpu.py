@@ -162,6 +162,11 @@ def test_argument_errors_without_gpu():
     assert lib.nh_slogpf_workspace_size(1, 146) == 4 * (160 * 160 + 128 * 128) + 256
     assert lib.nh_slogpf_workspace_size(1, 100) == 4 * (112 * 112 + 80 * 80) + 256
     assert lib.nh_slogpf_workspace_size(1, 98) == 4 * ((98 * 97 // 2 + 64 + 63) // 64 * 64) + 256
+    # 256 < n <= 352: 512 threads, 12-step panels (first complement n - 24 rows); above: the
+    # packed triangle
+    assert lib.nh_slogpf_workspace_size(1, 258) == 4 * (272 * 272 + 240 * 240) + 256
+    assert lib.nh_slogpf_workspace_size(1, 352) == 4 * (352 * 352 + 336 * 336) + 256
+    assert lib.nh_slogpf_workspace_size(1, 354) == 4 * ((354 * 353 // 2 + 64 + 63) // 64 * 64) + 256
     # the layer-by-layer network needs its workspace: too small a one is refused before any launch
     R = _lib.NH_NET_RESCONV
     big = _lib.NhModel(0, 24, 24, 252, 252, 64, 16, 0, 1, R, 16, 16, 16, 0, 0)

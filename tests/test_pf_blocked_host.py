@@ -8,8 +8,10 @@ wave-uniform values, and (sign, log|Pf|) matches the FP64 oracle, (sign, log|det
 Pfaffian: the panel widths in use (8 steps, 4 in the LDS form above n = 240) and wider ones (12,
 16), at the sizes of the image kernels (n = 224 for 16x16 at 112 + 112 electrons; n = 896 for
 32x32 at 448 + 448 with 1024 threads and 10-bit row keys, the form of the 32x32 image kernel and
-of nh_slogpf_ws) and the standalone nh_slogpf form (1024 threads, 8-bit row keys).  LU: the BCS kernels' 16-column panels at n = 100 to
-448 (the 32x32 BCS state)."""
+of nh_slogpf_ws) and the standalone nh_slogpf form (1024 threads, 8-bit row keys); the lazy-pivot
+pf_lazy (nh_pf_lazy.h) of nh_slogpf_ws's column form with 256 and 512 threads, the latter also
+under ThreadSanitizer.  LU: the BCS kernels' 16-column panels at n = 100 to 448 (the 32x32 BCS
+state)."""
 
 import os
 import shutil
@@ -119,3 +121,59 @@ def test_lu_blocked_host_pins_a_permuted_diagonal(harness, tmp_path):
     sg, la = _run_lu(harness, tmp_path, 256, A.astype(np.float32))
     assert sg == np.linalg.det(np.eye(n)[perm]) * np.sign(np.prod(d))
     assert abs(la - np.sum(np.log(np.abs(d.astype(np.float32).astype(np.float64))))) < 1e-4
+
+
+@pytest.fixture(scope="module")
+def harness_tsan(tmp_path_factory):
+    """The harness under ThreadSanitizer (no dummy slots in pf_lazy: any report is a race)."""
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    exe = str(tmp_path_factory.mktemp("pft") / "pf_blocked_tsan")
+    cmd = ["g++", "-std=c++20", "-O1", "-g", "-fsanitize=thread", "-fno-strict-aliasing", "-pthread",
+           "-Wno-unknown-pragmas", "-I", os.path.join(ROOT, "neuralhubbard_amd", "csrc"),
+           os.path.join(HERE, "host", "pf_blocked_host.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def _run_lazy(exe, tmp_path, NT, A, mode="lazy"):
+    n = A.shape[0]
+    i, j = np.tril_indices(n, -1)
+    path = str(tmp_path / f"{mode}_{NT}_{n}.bin")
+    A[i, j].astype(np.float32).tofile(path)
+    return subprocess.run([exe, mode, str(NT), str(n), path], capture_output=True, text=True, timeout=600)
+
+
+@pytest.mark.parametrize("mode,NT,n", [("lazy", 256, 100), ("lazy", 256, 224), ("lazy12", 512, 258),
+                                       ("lazy12", 512, 352), ("lazy12", 512, 512), ("lazy", 512, 460)])
+def test_pf_lazy_host_matches_oracle(harness, tmp_path, mode, NT, n):
+    """pf_lazy on the column-major square of nh_slogpf_ws's column form, in the two forms the entry
+    runs: 256 threads, 8-bit row keys, 16-step panels (100 <= n <= 256); 512 threads, 10-bit keys,
+    12-step panels (256 < n <= 352); and at n = 460 / 512 with 512 threads (measured, not used:
+    the packed triangle is faster there in the low-rank hop ratios' batches), against the FP64
+    oracle."""
+    rng = np.random.default_rng(3000 + n)
+    A = (rng.standard_normal((n, n)) * 0.3).astype(np.float32)
+    A = A - A.T
+    r = _run_lazy(harness, tmp_path, NT, A, mode)
+    assert r.returncode == 0, r.stderr[-3000:]
+    sg, la = (float(x) for x in r.stdout.split())
+    rs, rl = O.slogpf(A.astype(np.float64))
+    assert np.isfinite(la), "a NaN value reached the result"
+    assert sg == rs
+    assert abs(la - rl) < 1e-4 * n
+
+
+def test_pf_lazy_host_no_data_race(harness_tsan, tmp_path):
+    """The 512-thread pf_lazy (eight waves: the step's double-buffered keys, masks and smallest
+    rows, the column buffer written by candidate rows only) shows no data race at n = 460."""
+    rng = np.random.default_rng(460)
+    A = (rng.standard_normal((460, 460)) * 0.3).astype(np.float32)
+    A = A - A.T
+    r = _run_lazy(harness_tsan, tmp_path, 512, A)
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
+    assert r.returncode == 0, r.stderr[-3000:]
+    sg, la = (float(x) for x in r.stdout.split())
+    rs, rl = O.slogpf(A.astype(np.float64))
+    assert sg == rs and abs(la - rl) < 1e-4 * 460
