@@ -153,13 +153,15 @@ def _first_hop(s, N, LY):
 
 @pytest.mark.parametrize("LX,LY,n_up,n_dn,D,P,g,hop,f0", [
     (16, 16, 112, 112, 16, 16, 5, True, False),
+    (16, 16, 112, 112, 32, 16, 3, False, False),
     (10, 10, 44, 44, 8, 16, 1, False, True),
     (9, 10, 40, 38, 16, 8, 4, True, False),
 ])
 def test_image_pf_cm_host_no_data_race(harness_tsan, tmp_path, LX, LY, n_up, n_dn, D, P, g, hop, f0):
     """No data race in the image kernel's body: the pairing build, the workspace square and every
     pivot step and trailing update of pf_lazy (each step's LDS exchanges, the two buffers of the
-    compacted complements), with the result still the oracle's."""
+    compacted complements), with the result still the oracle's; d = 16 (the synthetic 16x16
+    workload) and d = 32 (the released states)."""
     s, Y, Wt, F0 = _case(LX, LY, n_up, n_dn, D, 1000 * LX + 10 * D + g, f0)
     h = _first_hop(s, LX * LY, LY) if hop else None
     r = _run(harness_tsan, tmp_path, LX, LY, n_up, n_dn, D, P, g, s, Y, Wt, F0, h, raw=True)
