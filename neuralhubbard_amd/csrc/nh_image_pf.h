@@ -47,11 +47,19 @@ __host__ __device__ inline int swz(int r) {
   return C <= 1 ? 0 : (r >> SH) & (C - 1);
 }
 
-// Floats of the LDS region shared by the pairing build (occupied features [n][D], one Wt spin
-// block [N][D], both swizzled) and by pf_lazy's scratch.
+// The pairing build stages the occupied features in LDS for D <= 16; for D >= 32 it reads them
+// from the image's feature rows through the caches (a lane's own row as vector loads, the
+// wave-uniform column rows through the scalar cache), which keeps the LDS region at pf_lazy's
+// scratch, so four workgroups share a CU instead of two (released 16x16 state, d = 32: 3.13
+// against 4.05 us per configuration of 8 images; at d = 16 the staged form is 1.4 % faster).
+template <int D>
+constexpr bool kPfYCached = D >= 32;
+
+// Floats of the LDS region shared by the pairing build (occupied features [n][D] for D <= 16,
+// one Wt spin block [N][D], both swizzled) and by pf_lazy's scratch.
 template <int D, int NT, int P>
 __host__ __device__ inline int pf_cm_region_floats(int n, int N) {
-  const int build = n * D + N * D;
+  const int build = (kPfYCached<D> ? 0 : n * D) + N * D;
   const int elim = pfl_scratch_floats<NT, P>(pfl_ld(n));
   return ((build > elim ? build : elim) + 3) & ~3;
 }
@@ -73,8 +81,9 @@ __host__ __device__ inline size_t pf_cm_lds_bytes(int n, int N) {
 // (Wt[2 s_q + s_p][-r] = -Wt[2 s_p + s_q][r] bit for bit), so the square is.  n_up % 4 == 0.
 template <int D, int NT>
 __device__ void pf_build_cm(float* __restrict__ A, int ld, int n, int n_up, int N, int LX, int LY, int two_n,
-                            int g, const float* yocc, float* wtl, const int* modes, const int* coord,
-                            const float* __restrict__ wt, const float* __restrict__ F0) {
+                            int g, const float* yocc, const float* __restrict__ Yi, float* wtl,
+                            const int* modes, const int* coord, const float* __restrict__ wt,
+                            const float* __restrict__ F0) {
   constexpr int NW = NT / kWave;
   constexpr int C4 = D / 4;
   const int t = threadIdx.x, lane = t & (kWave - 1);
@@ -101,10 +110,12 @@ __device__ void pf_build_cm(float* __restrict__ A, int ld, int n, int n_up, int 
       const int qb = qc0 + 16 * cb;
       if (diag && qb > plast) continue;  // uniform: the block lies above this chunk's rows
       const bool prow = p < pr1;
+      const int mp = prow ? modes[p] : 0;
       float yp[D];
       {
-        const float4* y4 = reinterpret_cast<const float4*>(yocc + p * D);
-        const int zp = swz<D>(p);
+        const float4* y4 = kPfYCached<D> ? reinterpret_cast<const float4*>(Yi + (size_t)mp * D)
+                                         : reinterpret_cast<const float4*>(yocc + p * D);
+        const int zp = kPfYCached<D> ? 0 : swz<D>(p);
 #pragma unroll
         for (int a4 = 0; a4 < C4; ++a4) {
           const float4 y = prow ? y4[a4 ^ zp] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -116,7 +127,6 @@ __device__ void pf_build_cm(float* __restrict__ A, int ld, int n, int n_up, int 
       }
       const int cp = prow ? coord[p] : 0;
       const int sxp = cp >> 16, syp = cp & 0xffff;
-      const int mp = prow ? modes[p] : 0;
       for (int q0 = qb; q0 < qb + 16 && q0 < qc1; q0 += 4) {
         if (diag && q0 > plast) break;  // uniform
         // the four columns' table rows first, then four independent accumulations
@@ -132,9 +142,15 @@ __device__ void pf_build_cm(float* __restrict__ A, int ld, int n, int n_up, int 
           dy += dy < 0 ? LY : 0;
           const int r = dx * LY + dy;
           w4[j] = reinterpret_cast<const float4*>(wtl + r * D);
-          y4[j] = reinterpret_cast<const float4*>(yocc + q * D);
+          if constexpr (kPfYCached<D>) {  // one row for the whole wave, through the scalar cache
+            const int mq = __builtin_amdgcn_readfirstlane(modes[q]);
+            y4[j] = reinterpret_cast<const float4*>(Yi + (size_t)mq * D);
+            zq[j] = 0;
+          } else {
+            y4[j] = reinterpret_cast<const float4*>(yocc + q * D);
+            zq[j] = swz<D>(q);
+          }
           zr[j] = swz<D>(r);
-          zq[j] = swz<D>(q);
         }
         float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -185,8 +201,8 @@ __device__ void image_pf_cm(int img, int N, int LX, int LY, int n, int n_up, con
   const int b = item_b ? item_b[item] : item;
   const int two_n = 2 * N;
   float* A = gmat + (size_t)(img - img0) * gstride;
-  float* yocc = lds;
-  float* wtl = yocc + n * D;
+  float* yocc = lds;  // (D <= 16)
+  float* wtl = kPfYCached<D> ? lds : yocc + n * D;
   int* modes = reinterpret_cast<int*>(lds + pf_cm_region_floats<D, NT, P>(n, N));
   int* coord = modes + n;  // (x << 16) | y of each mode's site
   int* flag = coord + n;
@@ -205,7 +221,7 @@ __device__ void image_pf_cm(int img, int N, int LX, int LY, int n, int n_up, con
     return;
   }
   const float* Yi = Y + (size_t)(img - img0) * (two_n * D);
-  for (int e = t; e < n * C4; e += NT) {
+  for (int e = t; !kPfYCached<D> && e < n * C4; e += NT) {
     const int p = e / C4, a4 = e - p * C4;
     reinterpret_cast<float4*>(yocc)[p * C4 + (a4 ^ swz<D>(p))] =
         reinterpret_cast<const float4*>(Yi + (size_t)modes[p] * D)[a4];
@@ -215,7 +231,7 @@ __device__ void image_pf_cm(int img, int N, int LX, int LY, int n, int n_up, con
     const int site = m - (m >= N ? N : 0);
     coord[p] = ((site / LY) << 16) | (site % LY);
   }
-  pf_build_cm<D, NT>(A, pfl_ld(n), n, n_up, N, LX, LY, two_n, g, yocc, wtl, modes, coord, wt, F0);
+  pf_build_cm<D, NT>(A, pfl_ld(n), n, n_up, N, LX, LY, two_n, g, yocc, Yi, wtl, modes, coord, wt, F0);
   __syncthreads();
   float sg;
   double la;
